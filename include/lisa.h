@@ -1,0 +1,185 @@
+/*
+ * lisa.h -- C ABI of the MI355X-native LiSA path tracer (liblisa.so).
+ *
+ * This is the drop-in boundary for LiSA's OptiX render call.  Plain C types
+ * only: pointers + sizes, integer status codes, no exception crosses it.
+ * Every call returns LISA_OK (0) or a negative lisa_status; the message of
+ * the last failure on a context is lisa_last_error(ctx) (lisa_last_error(NULL)
+ * gives the last failure of a call made without a context).
+ *
+ * Threading: one HIP stream per context; calls are synchronous unless noted;
+ * a context is not thread-safe; distinct contexts are independent.
+ * Ownership: inputs are copied at call time, the caller keeps them; output
+ * buffers are caller-owned; the context owns all device memory.
+ */
+#ifndef LISA_H
+#define LISA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t lisa_status;
+#define LISA_OK 0
+#define LISA_ERR_INVALID_ARG (-1)
+#define LISA_ERR_HIP (-2)
+#define LISA_ERR_ALLOC (-3)
+#define LISA_ERR_NO_SCENE (-4)
+#define LISA_ERR_PARSE (-5)
+#define LISA_ERR_IO (-6)
+#define LISA_ERR_UNSUPPORTED (-7)
+
+/* Material, same fields and meaning as `Material` in
+ * src/LiSA/include/structs.hh:16-23: alpha < 1 => dielectric with index n;
+ * otherwise opaque Lambertian with roughness; emit != 0 => light with
+ * emission (other fields unused on that branch).  40 bytes. */
+typedef struct lisa_material {
+  float roughness;
+  float alpha;
+  float n;
+  float diffuse[3];
+  int32_t emit;
+  float emission[3];
+} lisa_material;
+
+/* Launch parameters; the fields of `OptixParams`
+ * (src/LiSA/include/structs.hh:60-75) minus the buffers and the handle,
+ * which the context owns.  U, V, W come from lisa_camera_uvw. */
+typedef struct lisa_render_params {
+  uint32_t width, height;
+  uint32_t spp;      /* samples_per_launch (render.cc:140 puts all spp in one launch) */
+  uint32_t bounces;  /* num_bounces */
+  uint32_t subframe; /* subframe_index: > 0 lerps with the previous accum buffer */
+  float eye[3], U[3], V[3], W[3];
+  int32_t mode;      /* LISA_MODE_* */
+  uint32_t flags;    /* LISA_FLAG_* */
+} lisa_render_params;
+
+#define LISA_MODE_PLAIN 0        /* one work-item per pixel megakernel (the drop-in) */
+#define LISA_FLAG_FP64_ACCUM 1u  /* per-pixel FP64 accumulation (reference renders >= 16k spp) */
+#define LISA_FLAG_COUNT 2u       /* also count rays / node visits / triangle tests */
+
+/* Counters (lisa_get_counters), in order. */
+#define LISA_NCOUNTERS 8
+/* 0 radiance rays, 1 occlusion rays, 2 BVH node visits, 3 triangle tests,
+ * 4 radiance hits shaded (vertex+normal gathers), 5 samples,
+ * 6 zero/non-finite-direction rays (count as rays, miss without traversal),
+ * 7 pixels */
+
+typedef struct lisa_stats {
+  float render_ms;    /* last render/trace kernel time (HIP events on the context stream) */
+  float bvh_build_ms; /* last lisa_set_scene device build time */
+  int32_t num_tris;
+  int32_t num_nodes;  /* internal BVH nodes (64 B each) */
+  uint64_t device_bytes; /* device memory held by the context */
+} lisa_stats;
+
+typedef struct lisa_ctx lisa_ctx;
+
+/* Replaces OptixWrapper::init_optix (src/LiSA/src/optix_wrapper.cc:39-52). */
+lisa_status lisa_create(lisa_ctx** out, int32_t device);
+
+/* Replaces OptixWrapper::~OptixWrapper / clean_state (optix_wrapper.cc:35-37,448-465). */
+void lisa_destroy(lisa_ctx* ctx);
+
+const char* lisa_last_error(const lisa_ctx* ctx);
+
+/* Replaces OptixWrapper::create_mesh_handler + the GAS build and the SBT's
+ * per-material records (optix_wrapper.cc:54-149, 327-417): uploads the
+ * de-indexed soup (verts, normals: 9*T floats; mat_idx: T ints, as
+ * RendererParams in structs.hh:90-103 with num_vertices = 3*T) and builds
+ * the LBVH on the device. T == 0 is allowed (every ray misses). */
+lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* normals,
+                           const int32_t* mat_idx, int32_t T,
+                           const lisa_material* mats, int32_t M);
+
+/* Replaces render() -> launchSubframe() -> optixLaunch(w, h, 1)
+ * (src/LiSA/src/render.cc:35-58, 133-148).  The float4 accum buffer stays on
+ * the device (as OptixParams::accum_buffer); accum_rgba_out (w*h*4 floats)
+ * and rgba8_out (w*h*4 bytes, make_color of helpers.h:57-66 done on the host)
+ * are optional host copies. */
+lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p,
+                        float* accum_rgba_out, uint8_t* rgba8_out);
+
+/* Copies the device accum buffer (float4 per pixel, n_pixels = w*h of the
+ * last render) to rgba_out.  No counterpart in the reference, whose
+ * frame_buffer is mapped host memory (CUDAOutputBuffer ZERO_COPY). */
+lisa_status lisa_copy_accum(lisa_ctx* ctx, float* rgba_out, int64_t n_pixels);
+
+lisa_status lisa_synchronize(lisa_ctx* ctx);
+lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out);
+/* Counters of the last call made with LISA_FLAG_COUNT (or lisa_trace_rays). */
+lisa_status lisa_get_counters(const lisa_ctx* ctx, uint64_t* counters /* LISA_NCOUNTERS */);
+/* The hipStream_t the context launches on (for external event timing). */
+void* lisa_get_stream(const lisa_ctx* ctx);
+
+/* ---- debug / parity hooks (no counterpart in the reference: OptiX hides these) ---- */
+
+/* LBVH as built: Morton codes and triangle ids in sorted order (T each),
+ * child refs (2 per internal node; bit 31 = leaf, low bits = sorted index;
+ * 0xFFFFFFFF = empty) and child boxes (12 floats per internal node:
+ * c0.lo.xyz c0.hi.xyz c1.lo.xyz c1.hi.xyz).  Internal nodes = max(T-1, 1). */
+lisa_status lisa_get_bvh(lisa_ctx* ctx, uint32_t* morton, uint32_t* prim,
+                         uint32_t* child_refs, float* child_boxes);
+
+/* Closest hit of n rays (O, D: 3n floats): prim_out = triangle id or -1,
+ * t_out = hit distance.  Closest-hit semantics of optixTrace
+ * (shader.cu:77-98) with ties resolved to the lower triangle id. */
+lisa_status lisa_trace_rays(lisa_ctx* ctx, const float* O, const float* D, int32_t n,
+                            float tmin, float tmax, int32_t* prim_out, float* t_out);
+
+/* The plain path loop on a list of pixels, recording per sample: radiance
+ * (3 floats), radiance rays traced (bounce count), occlusion rays traced,
+ * an FNV-1a hash of every ray's hit id, and the hit id of the first
+ * record_bounces radiance rays (-1 miss, -2 zero/non-finite direction,
+ * -3 not traced).  Any record pointer may be NULL. */
+lisa_status lisa_trace_subset(lisa_ctx* ctx, const lisa_render_params* p,
+                              const int32_t* pixels, int32_t n_pixels, float* accum_rgba_out,
+                              float* sample_rgb, int32_t* sample_bounces,
+                              int32_t* sample_occl, uint32_t* sample_hash,
+                              int32_t* rad_hits, int32_t record_bounces);
+
+/* ---- host-side pieces of the drop-in (no device needed) ---- */
+
+/* sutil::Camera::UVWFrame (src/sutil/Camera.cpp:34-45) with up = (0,1,0),
+ * fov vertical in degrees, aspect = width / height (optix_wrapper.cc:434-442). */
+lisa_status lisa_camera_uvw(const float eye[3], const float look_at[3], float fov,
+                            float aspect, float U[3], float V[3], float W[3]);
+
+/* make_color (src/cuda/helpers.h:35-66) on n float4 pixels. */
+lisa_status lisa_srgb_quantize(const float* rgba, int64_t n, uint8_t* rgba8);
+
+/* sutil::saveImage for UNSIGNED_BYTE4 (src/sutil/sutil.cpp:523-600, 97-117):
+ * rows flipped, P6, extension must be .ppm / .PPM. */
+lisa_status lisa_save_ppm(const char* path, const uint8_t* rgba8, int32_t width, int32_t height);
+
+/* ---- .rto scene file (SceneParser, src/LiSA/src/scene_parser.cc) ---- */
+typedef struct lisa_scene_file lisa_scene_file;
+
+typedef struct lisa_scene_info {
+  int32_t num_tris, num_materials;
+  uint32_t width, height, num_samples, num_bounces;
+  float eye[3], look_at[3], fov;
+} lisa_scene_info;
+
+/* Parses path like SceneParser::SceneParser (scene_parser.cc:35-57) and the
+ * .obj files it names (parse_obj.cc:24-69).  On failure returns
+ * LISA_ERR_PARSE / LISA_ERR_IO with the reference's message in
+ * lisa_last_error(NULL) instead of exit(1). */
+lisa_status lisa_scene_file_load(const char* path, lisa_scene_file** out);
+void lisa_scene_file_free(lisa_scene_file* f);
+lisa_status lisa_scene_file_info(const lisa_scene_file* f, lisa_scene_info* info);
+/* Borrowed pointers valid until lisa_scene_file_free. */
+const float* lisa_scene_file_vertices(const lisa_scene_file* f);  /* 9*T */
+const float* lisa_scene_file_normals(const lisa_scene_file* f);   /* 9*T */
+const int32_t* lisa_scene_file_mat_indices(const lisa_scene_file* f); /* T */
+const lisa_material* lisa_scene_file_materials(const lisa_scene_file* f);
+const char* lisa_scene_file_output_image(const lisa_scene_file* f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LISA_H */

@@ -1,0 +1,55 @@
+"""Builds liblisa.so (HIP for gfx950 + host C++) and the `lisa` CLI in-tree.
+
+    python -m lisa_amd.build        # or lisa_amd.build.build()
+
+Outputs go to lisa_amd/_build/ (git-ignored, shipped with the tree).
+FP flags: no FMA contraction, correctly rounded FP32 division and sqrt,
+denormals kept -- the FP32 results of the kernels then follow the operation
+order written in the sources.  Packed FP32 instructions are disabled.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("LISA_OFFLOAD_ARCH", "gfx950")
+
+FP_FLAGS = ["-ffp-contract=off", "-fno-fast-math"]
+HIP_FLAGS = ["-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I", SRC,
+          "-Wall", "-Wno-unused-result"]
+
+LIB = os.path.join(OUT, "liblisa.so")
+CLI = os.path.join(OUT, "lisa")
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose_resources=False):
+    os.makedirs(OUT, exist_ok=True)
+    dev_o = os.path.join(OUT, "lisa_device.o")
+    host_o = os.path.join(OUT, "lisa_host.o")
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, *FP_FLAGS, *HIP_FLAGS, *extra, "-c",
+          os.path.join(SRC, "lisa_device.hip"), "-o", dev_o])
+    _run(["g++", *COMMON, *FP_FLAGS, "-c", os.path.join(SRC, "lisa_host.cpp"), "-o", host_o])
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", dev_o, host_o, "-o", tmp])
+    os.replace(tmp, LIB)
+    _run(["g++", *COMMON, os.path.join(SRC, "lisa_cli.cpp"), "-o", CLI, "-L", OUT, "-llisa",
+          "-Wl,-rpath,$ORIGIN"])
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose_resources="-v" in sys.argv)

@@ -1,0 +1,320 @@
+"""ctypes binding of liblisa.so (include/lisa.h).
+
+The library is the product: this module only marshals arrays.  It never
+falls back to anything else -- if liblisa.so is missing or a device call
+fails, it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liblisa.so")
+CLI_PATH = os.path.join(HERE, "_build", "lisa")
+
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_HIP = -2
+ERR_ALLOC = -3
+ERR_NO_SCENE = -4
+ERR_PARSE = -5
+ERR_IO = -6
+ERR_UNSUPPORTED = -7
+
+MODE_PLAIN = 0
+FLAG_FP64_ACCUM = 1
+FLAG_COUNT = 2
+NCOUNTERS = 8
+COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
+                 "radiance_hits", "samples", "invalid_dir_rays", "pixels")
+
+# Every function declared in include/lisa.h.
+EXPORTS = (
+    "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_render",
+    "lisa_copy_accum", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
+    "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset",
+    "lisa_camera_uvw", "lisa_srgb_quantize", "lisa_save_ppm", "lisa_scene_file_load",
+    "lisa_scene_file_free", "lisa_scene_file_info", "lisa_scene_file_vertices",
+    "lisa_scene_file_normals", "lisa_scene_file_mat_indices", "lisa_scene_file_materials",
+    "lisa_scene_file_output_image",
+)
+
+
+class LisaError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"lisa status {status}: {msg}")
+        self.status = status
+
+
+class Material(C.Structure):
+    _fields_ = [("roughness", C.c_float), ("alpha", C.c_float), ("n", C.c_float),
+                ("diffuse", C.c_float * 3), ("emit", C.c_int32), ("emission", C.c_float * 3)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("bounces", C.c_uint32), ("subframe", C.c_uint32),
+                ("eye", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3),
+                ("W", C.c_float * 3), ("mode", C.c_int32), ("flags", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("render_ms", C.c_float), ("bvh_build_ms", C.c_float), ("num_tris", C.c_int32),
+                ("num_nodes", C.c_int32), ("device_bytes", C.c_uint64)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("num_tris", C.c_int32), ("num_materials", C.c_int32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("num_samples", C.c_uint32), ("num_bounces", C.c_uint32),
+                ("eye", C.c_float * 3), ("look_at", C.c_float * 3), ("fov", C.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    """Load liblisa.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m lisa_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    vp, f32p, u32p, i32p, u64p, u8p = (C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint8))
+    st = C.c_int32
+    sig = {
+        "lisa_create": (st, [C.POINTER(C.c_void_p), C.c_int32]),
+        "lisa_destroy": (None, [vp]),
+        "lisa_last_error": (C.c_char_p, [vp]),
+        "lisa_set_scene": (st, [vp, f32p, f32p, i32p, C.c_int32, C.POINTER(Material), C.c_int32]),
+        "lisa_render": (st, [vp, C.POINTER(RenderParams), f32p, u8p]),
+        "lisa_copy_accum": (st, [vp, f32p, C.c_int64]),
+        "lisa_synchronize": (st, [vp]),
+        "lisa_get_stats": (st, [vp, C.POINTER(Stats)]),
+        "lisa_get_counters": (st, [vp, u64p]),
+        "lisa_get_stream": (vp, [vp]),
+        "lisa_get_bvh": (st, [vp, u32p, u32p, u32p, f32p]),
+        "lisa_trace_rays": (st, [vp, f32p, f32p, C.c_int32, C.c_float, C.c_float, i32p, f32p]),
+        "lisa_trace_subset": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, f32p, f32p,
+                                   i32p, i32p, u32p, i32p, C.c_int32]),
+        "lisa_camera_uvw": (st, [f32p, f32p, C.c_float, C.c_float, f32p, f32p, f32p]),
+        "lisa_srgb_quantize": (st, [f32p, C.c_int64, u8p]),
+        "lisa_save_ppm": (st, [C.c_char_p, u8p, C.c_int32, C.c_int32]),
+        "lisa_scene_file_load": (st, [C.c_char_p, C.POINTER(C.c_void_p)]),
+        "lisa_scene_file_free": (None, [vp]),
+        "lisa_scene_file_info": (st, [vp, C.POINTER(SceneInfo)]),
+        "lisa_scene_file_vertices": (f32p, [vp]),
+        "lisa_scene_file_normals": (f32p, [vp]),
+        "lisa_scene_file_mat_indices": (i32p, [vp]),
+        "lisa_scene_file_materials": (C.POINTER(Material), [vp]),
+        "lisa_scene_file_output_image": (C.c_char_p, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _check(status, ctx=None):
+    if status != OK:
+        msg = lib().lisa_last_error(ctx)
+        raise LisaError(status, msg.decode() if msg else "")
+
+
+def materials_array(mats):
+    arr = (Material * max(len(mats), 1))()
+    for i, m in enumerate(mats):
+        arr[i].roughness = m.get("roughness", 0.0)
+        arr[i].alpha = m.get("alpha", 1.0)
+        arr[i].n = m.get("n", 1.0)
+        arr[i].diffuse[:] = list(m.get("diffuse", (0.0, 0.0, 0.0)))
+        arr[i].emit = 1 if m.get("emit", False) else 0
+        arr[i].emission[:] = list(m.get("emission", (0.0, 0.0, 0.0)))
+    return arr
+
+
+def camera_uvw(eye, look_at, fov, aspect):
+    U, V, W = (np.zeros(3, np.float32) for _ in range(3))
+    e, l = f32(eye), f32(look_at)
+    _check(lib().lisa_camera_uvw(_p(e, C.c_float), _p(l, C.c_float), fov, aspect,
+                                 _p(U, C.c_float), _p(V, C.c_float), _p(W, C.c_float)))
+    return U, V, W
+
+
+def srgb_quantize(rgba):
+    rgba = f32(rgba).reshape(-1, 4)
+    out = np.zeros((rgba.shape[0], 4), np.uint8)
+    _check(lib().lisa_srgb_quantize(_p(rgba, C.c_float), rgba.shape[0], _p(out, C.c_uint8)))
+    return out
+
+
+def save_ppm(path, rgba8, width, height):
+    a = np.ascontiguousarray(rgba8, dtype=np.uint8)
+    _check(lib().lisa_save_ppm(path.encode(), _p(a, C.c_uint8), width, height))
+
+
+def render_params(width, height, spp, bounces, eye, U, V, W, subframe=0, flags=0,
+                  mode=MODE_PLAIN):
+    p = RenderParams()
+    p.width, p.height, p.spp, p.bounces, p.subframe = width, height, spp, bounces, subframe
+    p.eye[:], p.U[:], p.V[:], p.W[:] = [list(map(float, v)) for v in (eye, U, V, W)]
+    p.mode, p.flags = mode, flags
+    return p
+
+
+class SceneFile:
+    """A parsed `.rto` scene (SceneParser, src/LiSA/src/scene_parser.cc)."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        _check(lib().lisa_scene_file_load(str(path).encode(), C.byref(h)))
+        self.h = h
+        info = SceneInfo()
+        _check(lib().lisa_scene_file_info(self.h, C.byref(info)))
+        T, M = info.num_tris, info.num_materials
+        L = lib()
+        self.num_tris, self.num_materials = T, M
+        self.width, self.height = info.width, info.height
+        self.num_samples, self.num_bounces = info.num_samples, info.num_bounces
+        self.eye = np.array(info.eye[:], np.float32)
+        self.look_at = np.array(info.look_at[:], np.float32)
+        self.fov = float(info.fov)
+        self.verts = np.ctypeslib.as_array(L.lisa_scene_file_vertices(self.h), (max(T, 1) * 9,))[
+            : 9 * T].reshape(T, 9).copy() if T else np.zeros((0, 9), np.float32)
+        self.normals = np.ctypeslib.as_array(L.lisa_scene_file_normals(self.h), (max(T, 1) * 9,))[
+            : 9 * T].reshape(T, 9).copy() if T else np.zeros((0, 9), np.float32)
+        self.mat_idx = np.ctypeslib.as_array(L.lisa_scene_file_mat_indices(self.h), (max(T, 1),))[
+            :T].copy() if T else np.zeros(0, np.int32)
+        mp = L.lisa_scene_file_materials(self.h)
+        self.materials = []
+        for i in range(M):
+            m = mp[i]
+            self.materials.append({"roughness": m.roughness, "alpha": m.alpha, "n": m.n,
+                                   "diffuse": tuple(m.diffuse), "emit": bool(m.emit),
+                                   "emission": tuple(m.emission)})
+        self.output_image = L.lisa_scene_file_output_image(self.h).decode()
+        L.lisa_scene_file_free(self.h)
+        self.h = None
+
+
+class Context:
+    """One device context (lisa_ctx): scene, LBVH and accum buffer in HBM."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().lisa_create(C.byref(h), device))
+        self.h = h
+        self.T = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lisa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, status):
+        _check(status, self.h)
+
+    def set_scene(self, verts, normals, mat_idx, materials):
+        v = f32(verts).reshape(-1, 9)
+        n = f32(normals).reshape(-1, 9)
+        m = np.ascontiguousarray(mat_idx, dtype=np.int32)
+        mats = materials_array(materials)
+        self.T = int(v.shape[0])
+        self._ck(lib().lisa_set_scene(self.h, _p(v, C.c_float), _p(n, C.c_float),
+                                      _p(m, C.c_int32), self.T, mats, len(materials)))
+
+    def render(self, params, want_accum=True, want_rgba8=False):
+        npx = params.width * params.height
+        acc = np.zeros((npx, 4), np.float32) if want_accum else None
+        rgb = np.zeros((npx, 4), np.uint8) if want_rgba8 else None
+        self._ck(lib().lisa_render(self.h, C.byref(params), _p(acc, C.c_float),
+                                   _p(rgb, C.c_uint8)))
+        return acc, rgb
+
+    def copy_accum(self, n_pixels):
+        acc = np.zeros((n_pixels, 4), np.float32)
+        self._ck(lib().lisa_copy_accum(self.h, _p(acc, C.c_float), n_pixels))
+        return acc
+
+    def synchronize(self):
+        self._ck(lib().lisa_synchronize(self.h))
+
+    def stats(self):
+        s = Stats()
+        self._ck(lib().lisa_get_stats(self.h, C.byref(s)))
+        return {"render_ms": s.render_ms, "bvh_build_ms": s.bvh_build_ms,
+                "num_tris": s.num_tris, "num_nodes": s.num_nodes,
+                "device_bytes": s.device_bytes}
+
+    def counters(self):
+        c = np.zeros(NCOUNTERS, np.uint64)
+        self._ck(lib().lisa_get_counters(self.h, _p(c, C.c_uint64)))
+        return c
+
+    def stream(self):
+        return lib().lisa_get_stream(self.h)
+
+    def bvh(self):
+        T = self.T
+        nI = max(T - 1, 1)
+        m = np.zeros(max(T, 1), np.uint32)
+        p = np.zeros(max(T, 1), np.uint32)
+        r = np.zeros(2 * nI, np.uint32)
+        b = np.zeros(12 * nI, np.float32)
+        self._ck(lib().lisa_get_bvh(self.h, _p(m, C.c_uint32), _p(p, C.c_uint32),
+                                    _p(r, C.c_uint32), _p(b, C.c_float)))
+        return {"morton": m[:T], "prim": p[:T], "refs": r.reshape(nI, 2),
+                "boxes": b.reshape(nI, 12)}
+
+    def trace_rays(self, O, D, tmin=1e-4, tmax=1e16):
+        O, D = f32(O).reshape(-1, 3), f32(D).reshape(-1, 3)
+        n = O.shape[0]
+        prim = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        self._ck(lib().lisa_trace_rays(self.h, _p(O, C.c_float), _p(D, C.c_float), n, tmin, tmax,
+                                       _p(prim, C.c_int32), _p(t, C.c_float)))
+        return prim, t
+
+    def trace_subset(self, params, pixels, record_bounces=None):
+        px = np.ascontiguousarray(pixels, dtype=np.int32)
+        n, spp = px.shape[0], params.spp
+        rb = params.bounces if record_bounces is None else record_bounces
+        acc = np.zeros((n, 4), np.float32)
+        srgb = np.zeros((n, spp, 3), np.float32)
+        sb = np.zeros((n, spp), np.int32)
+        so = np.zeros((n, spp), np.int32)
+        sh = np.zeros((n, spp), np.uint32)
+        rh = np.zeros((n, spp, max(rb, 1)), np.int32) if rb > 0 else None
+        self._ck(lib().lisa_trace_subset(self.h, C.byref(params), _p(px, C.c_int32), n,
+                                         _p(acc, C.c_float), _p(srgb, C.c_float),
+                                         _p(sb, C.c_int32), _p(so, C.c_int32),
+                                         _p(sh, C.c_uint32), _p(rh, C.c_int32), rb))
+        return {"accum": acc, "sample_rgb": srgb, "sample_bounces": sb, "sample_occl": so,
+                "sample_hash": sh, "rad_hits": rh, "counters": self.counters()}

@@ -1,0 +1,1288 @@
+// liblisa.so device half: LBVH builder, traversal and the plain path-tracing
+// megakernel for gfx950 (MI355X), plus the C-ABI entry points that drive
+// them (include/lisa.h).
+//
+// HBM layout per scene (all 16-byte aligned, built once by lisa_set_scene):
+//   nodes  float4[4 * nI]   one 64-byte line per internal node:
+//                           c0.lo.xyz c0.hi.xyz c1.lo.xyz c1.hi.xyz | ref0 ref1 - -
+//                           (child boxes live in the parent; ref bit 31 = leaf,
+//                           low bits = sorted triangle / node index)
+//   tris   float4[3 * T]    48 bytes per triangle in Morton order:
+//                           v0.xyz|prim id, v1.xyz|material id, v2.xyz|0
+//   nrm    float4[3 * T]    vertex normals in the same order (shading gather)
+//   mats   DevMat[M]        48 bytes per material
+//   accum  float4[w * h]    OptixParams::accum_buffer (structs.hh:62)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lisa.h"
+#include "lisa_internal.h"
+#include "lisa_math.h"
+
+using namespace lisa;
+
+namespace {
+
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT 2013).
+constexpr float kRobust = 0x1.000006p+0f;
+constexpr float kTmin = 1e-4f;  // shader.cu:114, :202
+constexpr float kTmax = 1e16f;  // shader.cu:115, :202
+constexpr int kNeeTries = 30;   // shader.cu:199
+// Per-lane traversal stack: the first kStackLds entries in LDS, the rest in
+// a private spill array.  The depth of a Karras tree over index-augmented
+// 30-bit keys is below 64, so kStackMax entries always suffice.
+constexpr int kStackLds = 32;
+constexpr int kStackMax = 64;
+constexpr int kPathBlock = 128;  // 2 wave64 per workgroup, 16x8 pixel tile
+constexpr int kTileW = 16, kTileH = 8;
+constexpr int kRayBlock = 256;
+
+struct DevMat {
+  float4 diffuse_alpha;  // diffuse.xyz, alpha
+  float4 emission_emit;  // emission.xyz, emit (1/0)
+  float4 rough_n;        // roughness, n, -, -
+};
+
+struct SceneView {
+  const float4* __restrict__ nodes;
+  const float4* __restrict__ tris;
+  const float4* __restrict__ nrm;
+  const DevMat* __restrict__ mats;
+  int32_t T;
+};
+
+// ---------------------------------------------------------------------------
+// Traversal: stack-based BVH2 walk, near child first, leaves tested as soon
+// as their box (stored in the parent) is hit; watertight FP32 ray/triangle
+// test (Woop, Benthin & Wald, JCGT 2013).  Closest hit over t in
+// (tmin, tmax], equal t resolved to the lower triangle id, so the answer does
+// not depend on traversal order.
+// ---------------------------------------------------------------------------
+struct RayT {
+  F3 O, inv;
+  float Sx, Sy, Sz;
+  int kx, ky, kz;
+};
+
+__device__ __forceinline__ bool ray_valid(F3 D) {
+  return isfinite(D.x) && isfinite(D.y) && isfinite(D.z) &&
+         (D.x != 0.0f || D.y != 0.0f || D.z != 0.0f);
+}
+
+__device__ __forceinline__ RayT make_ray(F3 O, F3 D) {
+  RayT r;
+  r.O = O;
+  r.inv = mk(1.0f / D.x, 1.0f / D.y, 1.0f / D.z);
+  const float ax = fabsf(D.x), ay = fabsf(D.y), az = fabsf(D.z);
+  int kz = 0;
+  float m = ax;
+  if (ay > m) {
+    kz = 1;
+    m = ay;
+  }
+  if (az > m) kz = 2;
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  const float dz = sel(D, kz);
+  if (dz < 0.0f) {
+    const int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  r.Sx = sel(D, kx) / dz;
+  r.Sy = sel(D, ky) / dz;
+  r.Sz = 1.0f / dz;
+  return r;
+}
+
+__device__ __forceinline__ bool watertight(const RayT& r, F3 a, F3 b, F3 c, float& t) {
+  const F3 A = a - r.O, B = b - r.O, C = c - r.O;
+  const float Akz = sel(A, r.kz), Bkz = sel(B, r.kz), Ckz = sel(C, r.kz);
+  const float Ax = sel(A, r.kx) - r.Sx * Akz;
+  const float Ay = sel(A, r.ky) - r.Sy * Akz;
+  const float Bx = sel(B, r.kx) - r.Sx * Bkz;
+  const float By = sel(B, r.ky) - r.Sy * Bkz;
+  const float Cx = sel(C, r.kx) - r.Sx * Ckz;
+  const float Cy = sel(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  const float T = U * Az + V * Bz + W * Cz;
+  t = T / det;
+  return true;
+}
+
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
+                                     const RayT& r, float tmin, float tmax, float& tn_out) {
+  const float tx0 = (lx - r.O.x) * r.inv.x, tx1 = (hx - r.O.x) * r.inv.x;
+  const float ty0 = (ly - r.O.y) * r.inv.y, ty1 = (hy - r.O.y) * r.inv.y;
+  const float tz0 = (lz - r.O.z) * r.inv.z, tz1 = (hz - r.O.z) * r.inv.z;
+  const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+  const float tfb = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * kRobust;
+  const float tf = fminf(tfb, tmax);
+  tn_out = tn;
+  return tn <= tf;
+}
+
+struct TravCount {
+  uint64_t visits = 0, tests = 0;
+};
+
+// Returns the sorted index of the closest triangle (-1 on miss); t_out its t.
+// `stk` points at this lane's first LDS stack slot; consecutive entries are
+// kStride words apart, so a wave's 64 lanes hit 64 distinct banks.
+template <bool kCount, int kStride>
+__device__ __forceinline__ int32_t closest(const SceneView& sc, const RayT& r, float tmin,
+                                           float tmax, float& t_out, uint32_t* stk,
+                                           TravCount& cnt) {
+  uint32_t spill[kStackMax - kStackLds];
+  float best_t = tmax;
+  uint32_t best_prim = 0xFFFFFFFFu;
+  int32_t best_k = -1;
+  int sp = 0;
+  uint32_t node = 0;
+  // A tree visits each internal node at most once; the bound only guards
+  // against a corrupt hierarchy (it never triggers on a valid one).
+  const uint32_t max_visits = (uint32_t)sc.T + 1u;
+  for (uint32_t it = 0; it < max_visits; ++it) {
+    if (kCount) cnt.visits++;
+    const float4* np = sc.nodes + 4 * (size_t)node;
+    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+    const uint32_t r0 = __float_as_uint(d.x), r1 = __float_as_uint(d.y);
+    float tn0 = 0.0f, tn1 = 0.0f;
+    bool h0 = (r0 != kEmpty) && slab(a.x, a.y, a.z, a.w, b.x, b.y, r, tmin, best_t, tn0);
+    bool h1 = (r1 != kEmpty) && slab(b.z, b.w, c.x, c.y, c.z, c.w, r, tmin, best_t, tn1);
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      const bool hit = ch == 0 ? h0 : h1;
+      const uint32_t ref = ch == 0 ? r0 : r1;
+      if (hit && (ref & kLeaf)) {
+        if (kCount) cnt.tests++;
+        const uint32_t k = ref & ~kLeaf;
+        const float4* tp = sc.tris + 3 * (size_t)k;
+        const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+        const uint32_t prim = __float_as_uint(t0.w);
+        float t;
+        if (watertight(r, xyz(t0), xyz(t1), xyz(t2), t) && t > tmin &&
+            (t < best_t || (t == best_t && prim < best_prim))) {
+          best_t = t;
+          best_prim = prim;
+          best_k = (int32_t)k;
+        }
+        if (ch == 0)
+          h0 = false;
+        else
+          h1 = false;
+      }
+    }
+    if (h0 && h1) {
+      uint32_t nr = r0, fr = r1;
+      if (tn1 < tn0) {
+        nr = r1;
+        fr = r0;
+      }
+      if (sp == kStackMax) break;  // unreachable for a valid tree (depth < 64)
+      if (sp < kStackLds)
+        stk[sp * kStride] = fr;
+      else
+        spill[sp - kStackLds] = fr;
+      ++sp;
+      node = nr;
+    } else if (h0) {
+      node = r0;
+    } else if (h1) {
+      node = r1;
+    } else {
+      if (sp == 0) break;
+      --sp;
+      node = sp < kStackLds ? stk[sp * kStride] : spill[sp - kStackLds];
+    }
+  }
+  t_out = best_t;
+  return best_k;
+}
+
+// ---------------------------------------------------------------------------
+// Ray batch kernel (hit-id parity, lisa_trace_rays).
+// ---------------------------------------------------------------------------
+template <bool kCount>
+__global__ __launch_bounds__(kRayBlock) void k_trace_rays(SceneView sc, const float* __restrict__ O,
+                                                          const float* __restrict__ D, int32_t n,
+                                                          float tmin, float tmax,
+                                                          int32_t* __restrict__ prim_out,
+                                                          float* __restrict__ t_out,
+                                                          unsigned long long* counters) {
+  __shared__ uint32_t stk[kStackLds * kRayBlock];
+  const int32_t i = blockIdx.x * kRayBlock + threadIdx.x;
+  if (i >= n) return;
+  const F3 o = mk(O[3 * i], O[3 * i + 1], O[3 * i + 2]);
+  const F3 d = mk(D[3 * i], D[3 * i + 1], D[3 * i + 2]);
+  TravCount cnt;
+  int32_t prim = -1;
+  float t = 0.0f;
+  if (sc.T > 0 && ray_valid(d)) {
+    const RayT r = make_ray(o, d);
+    const int32_t k = closest<kCount, kRayBlock>(sc, r, tmin, tmax, t, stk + threadIdx.x, cnt);
+    if (k >= 0) prim = (int32_t)__float_as_uint(sc.tris[3 * (size_t)k].w);
+  }
+  prim_out[i] = prim;
+  t_out[i] = prim >= 0 ? t : 0.0f;
+  if (kCount) {
+    atomicAdd(&counters[2], (unsigned long long)cnt.visits);
+    atomicAdd(&counters[3], (unsigned long long)cnt.tests);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plain megakernel: one work-item per pixel, all spp in one launch
+// (shader.cu:126-167), the bounce loop (102-124), closest-hit / miss
+// programs (172-255) and NEE (196-209) as one per-lane state machine.  Every
+// iteration traces exactly one ray, radiance or occlusion, so lanes of a
+// wave that are at different stages of their paths still traverse together.
+// ---------------------------------------------------------------------------
+struct PathArgs {
+  uint32_t width, height, spp, bounces, subframe;
+  F3 eye, U, V, W;
+  const int32_t* pixels;  // nullptr: full image in 16x8 tiles
+  int32_t n_items;
+  uint32_t tiles_x;
+  float4* accum;
+  const float4* accum_prev;
+  int32_t fp64;
+  float* s_rgb;
+  int32_t* s_b;
+  int32_t* s_o;
+  uint32_t* s_h;
+  int32_t* rad_hits;
+  int32_t rb;
+  unsigned long long* counters;
+};
+
+template <bool kRecord, bool kCount>
+__global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
+  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  uint32_t* my_stk = stk + threadIdx.x;
+  int32_t item;
+  uint32_t x, y;
+  if (p.pixels) {
+    item = blockIdx.x * kPathBlock + threadIdx.x;
+    if (item >= p.n_items) return;
+    const uint32_t pix = (uint32_t)p.pixels[item];
+    x = pix % p.width;
+    y = pix / p.width;
+  } else {
+    const uint32_t tx = blockIdx.x % p.tiles_x, ty = blockIdx.x / p.tiles_x;
+    x = tx * kTileW + threadIdx.x % kTileW;
+    y = ty * kTileH + threadIdx.x / kTileW;
+    if (x >= p.width || y >= p.height) return;
+    item = (int32_t)(y * p.width + x);
+  }
+
+  uint32_t seed = tea16(y * p.width + x, p.subframe);  // shader.cu:141
+  const float fx = (float)x, fy = (float)y;
+  const float sx = (float)p.width, sy = (float)p.height;
+
+  uint64_t c_rad = 0, c_occ = 0, c_rhit = 0, c_inv = 0;
+  TravCount tc;
+
+  F3 accum = mk(0.0f, 0.0f, 0.0f);
+  double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0;
+
+  // Per-lane path state (RayState, shader.cu:30-42, plus the NEE loop index).
+  F3 O, D, color, att, N = mk(0.0f, 0.0f, 0.0f), P = mk(0.0f, 0.0f, 0.0f);
+  F3 Din = mk(0.0f, 0.0f, 0.0f);
+  bool occl = false, hitf = false, need_new = true;
+  int32_t light_mat = 0, tries = 0;
+  float rough = 0.0f;
+  uint32_t b = 0, s = 0;
+  uint32_t hash = 0;
+  int32_t occl_cnt = 0;
+  const size_t rec_base = (size_t)item * p.spp;
+
+  for (;;) {
+    if (need_new) {
+      if (s == p.spp) break;
+      const float jx = rng(seed);
+      const float jy = rng(seed);
+      const float dx = (2.0f * fx + jx) / sx - 1.0f;
+      const float dy = (2.0f * fy + jy) / sy - 1.0f;
+      D = normalize(dx * p.U + dy * p.V + p.W);
+      O = p.eye;
+      color = mk(0.0f, 0.0f, 0.0f);
+      att = mk(1.0f, 1.0f, 1.0f);
+      hitf = false;
+      occl = false;
+      b = 0;
+      need_new = false;
+      if (kRecord) {
+        hash = 2166136261u;
+        occl_cnt = 0;
+        if (p.rad_hits)
+          for (int32_t q = 0; q < p.rb; ++q) p.rad_hits[(rec_base + s) * p.rb + q] = -3;
+      }
+      if (p.bounces == 0) {
+        need_new = true;
+        accum = accum + color;
+        if (p.fp64) {
+          ad0 += (double)color.x;
+          ad1 += (double)color.y;
+          ad2 += (double)color.z;
+        }
+        if (kRecord) {
+          if (p.s_rgb) {
+            p.s_rgb[3 * (rec_base + s)] = color.x;
+            p.s_rgb[3 * (rec_base + s) + 1] = color.y;
+            p.s_rgb[3 * (rec_base + s) + 2] = color.z;
+          }
+          if (p.s_b) p.s_b[rec_base + s] = 0;
+          if (p.s_o) p.s_o[rec_base + s] = 0;
+          if (p.s_h) p.s_h[rec_base + s] = hash;
+        }
+        ++s;
+        continue;
+      }
+    }
+
+    // ---- trace one ray (radiance or occlusion) ----
+    int32_t k = -1;
+    float t = 0.0f;
+    const bool valid = ray_valid(D);
+    if (valid && sc.T > 0) {
+      const RayT r = make_ray(O, D);
+      k = closest<kCount, kPathBlock>(sc, r, kTmin, kTmax, t, my_stk, tc);
+    }
+    if (kCount && !valid) c_inv++;
+    float4 t0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), t1 = t0;
+    int32_t prim = valid ? -1 : -2;
+    int32_t mat = -1;
+    if (k >= 0) {
+      t0 = sc.tris[3 * (size_t)k];
+      t1 = sc.tris[3 * (size_t)k + 1];
+      prim = (int32_t)__float_as_uint(t0.w);
+      mat = (int32_t)__float_as_uint(t1.w);
+    }
+    if (kRecord) hash = (hash ^ ((uint32_t)prim ^ (occl ? 0x5bd1e995u : 0u))) * 16777619u;
+
+    bool finish = false;
+    if (!occl) {
+      // radiance: miss (189-194) / closest hit (211-255)
+      if (kCount) c_rad++;
+      if (kRecord && p.rad_hits && (int32_t)b < p.rb) p.rad_hits[(rec_base + s) * p.rb + b] = prim;
+      ++b;
+      if (k < 0) {
+        color = color + mk(0.0f, 0.0f, 0.0f);
+        finish = true;
+      } else {
+        const DevMat m = sc.mats[mat];
+        if (m.emission_emit.w != 0.0f) {
+          color = color + xyz(m.emission_emit) * att;
+          finish = true;
+        } else {
+          if (kCount) c_rhit++;
+          P = O + t * D;
+          Din = D;
+          {
+            const float4 t2 = sc.tris[3 * (size_t)k + 2];
+            const float4* nq = sc.nrm + 3 * (size_t)k;
+            N = bary_normal(P, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
+          }
+          if (m.diffuse_alpha.w < 1.0f) {
+            D = Dielectric::scatter(D, N, m.rough_n.y, xyz(m.diffuse_alpha), att, seed);
+            O = P;
+            if (b == p.bounces) finish = true;
+          } else {
+            att = att * xyz(m.diffuse_alpha);
+            rough = m.rough_n.x;
+            occl = true;
+            tries = 0;
+            O = P;
+            D = hemisphere(N, seed);
+          }
+        }
+      }
+    } else {
+      // occlusion: miss (172-175) / closest hit (177-184), then the NEE loop (200-207)
+      if (kCount) c_occ++;
+      if (kRecord) occl_cnt++;
+      if (k < 0) {
+        hitf = false;
+      } else if (sc.mats[mat].emission_emit.w != 0.0f) {
+        light_mat = mat;
+        hitf = true;
+      }
+      bool end_nee = false;
+      F3 nee = mk(0.0f, 0.0f, 0.0f);
+      if (hitf) {
+        nee = xyz(sc.mats[light_mat].emission_emit) * Lambertian::eval(N, D);
+        end_nee = true;
+      } else if (++tries == kNeeTries) {
+        end_nee = true;
+      } else {
+        D = hemisphere(N, seed);
+      }
+      if (end_nee) {
+        color = color + nee * att;
+        D = Lambertian::sample(Din, N, seed, rough);
+        O = P;
+        occl = false;
+        if (b == p.bounces) finish = true;
+      }
+    }
+
+    if (finish) {
+      accum = accum + color;
+      if (p.fp64) {
+        ad0 += (double)color.x;
+        ad1 += (double)color.y;
+        ad2 += (double)color.z;
+      }
+      if (kRecord) {
+        if (p.s_rgb) {
+          p.s_rgb[3 * (rec_base + s)] = color.x;
+          p.s_rgb[3 * (rec_base + s) + 1] = color.y;
+          p.s_rgb[3 * (rec_base + s) + 2] = color.z;
+        }
+        if (p.s_b) p.s_b[rec_base + s] = (int32_t)b;
+        if (p.s_o) p.s_o[rec_base + s] = occl_cnt;
+        if (p.s_h) p.s_h[rec_base + s] = hash;
+      }
+      ++s;
+      need_new = true;
+    }
+  }
+
+  // shader.cu:156-166
+  F3 out;
+  if (p.fp64) {
+    const double inv = 1.0 / (double)p.spp;
+    out = mk((float)(ad0 * inv), (float)(ad1 * inv), (float)(ad2 * inv));
+  } else {
+    const float inv = 1.0f / (float)p.spp;
+    out = accum * inv;
+  }
+  if (p.subframe > 0) {
+    const float a = 1.0f / (float)(p.subframe + 1);
+    out = lerp(xyz(p.accum_prev[item]), out, a);
+  }
+  p.accum[item] = make_float4(out.x, out.y, out.z, 1.0f);
+
+  if (kCount) {
+    atomicAdd(&p.counters[0], (unsigned long long)c_rad);
+    atomicAdd(&p.counters[1], (unsigned long long)c_occ);
+    atomicAdd(&p.counters[2], (unsigned long long)tc.visits);
+    atomicAdd(&p.counters[3], (unsigned long long)tc.tests);
+    atomicAdd(&p.counters[4], (unsigned long long)c_rhit);
+    atomicAdd(&p.counters[5], (unsigned long long)p.spp);
+    atomicAdd(&p.counters[6], (unsigned long long)c_inv);
+    atomicAdd(&p.counters[7], 1ull);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LBVH build (Karras, HPG 2012).
+// ---------------------------------------------------------------------------
+constexpr int kBuildBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBuildBlock * kSortItems;  // 4096 keys per tile
+
+__device__ __forceinline__ float centroid(const float* __restrict__ v, size_t k, int a) {
+  const float* t = v + 9 * k;
+  return (t[a] + t[3 + a] + t[6 + a]) / 3.0f;
+}
+
+// Per-block min/max of triangle centroids (6 floats per block).
+__global__ __launch_bounds__(kBuildBlock) void k_centroid_partial(const float* __restrict__ v,
+                                                                  int32_t T, float* partial) {
+  __shared__ float s[6][kBuildBlock];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int32_t k = blockIdx.x * kBuildBlock + threadIdx.x; k < T; k += gridDim.x * kBuildBlock) {
+    for (int a = 0; a < 3; ++a) {
+      const float c = centroid(v, (size_t)k, a);
+      lo[a] = fminf(lo[a], c);
+      hi[a] = fmaxf(hi[a], c);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    s[a][threadIdx.x] = lo[a];
+    s[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int w = kBuildBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = fminf(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+        s[3 + a][threadIdx.x] = fmaxf(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + w]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_centroid_final(const float* partial, int32_t nb,
+                                                                float* bounds) {
+  __shared__ float s[6][kBuildBlock];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int32_t i = threadIdx.x; i < nb; i += kBuildBlock)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], partial[6 * i + a]);
+      hi[a] = fmaxf(hi[a], partial[6 * i + 3 + a]);
+    }
+  for (int a = 0; a < 3; ++a) {
+    s[a][threadIdx.x] = lo[a];
+    s[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int w = kBuildBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = fminf(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+        s[3 + a][threadIdx.x] = fmaxf(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) bounds[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// 10-bit spread by multiply-and-mask (every third bit).
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t quant10(float u) {
+  float q = u * 1024.0f;
+  q = fminf(fmaxf(q, 0.0f), 1023.0f);
+  return (uint32_t)q;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_morton(const float* __restrict__ v, int32_t T,
+                                                        const float* __restrict__ bounds,
+                                                        uint32_t* keys, uint32_t* vals) {
+  const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
+  if (k >= T) return;
+  uint32_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    const float lo = bounds[a], hi = bounds[3 + a];
+    const float ext = hi - lo;
+    const float c = centroid(v, (size_t)k, a);
+    const float u = ext > 0.0f ? (c - lo) / ext : 0.5f;
+    q[a] = quant10(u);
+  }
+  keys[k] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  vals[k] = (uint32_t)k;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t* s_tmp /*>=4*/,
+                                                    uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_tmp[wave] = x;
+  __syncthreads();
+  uint32_t wbase = 0;
+  for (int w = 0; w < wave; ++w) wbase += s_tmp[w];
+  total = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
+  __syncthreads();
+  return wbase + x - v;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_sort_hist(const uint32_t* __restrict__ keys,
+                                                           int32_t n, int shift, uint32_t* hist,
+                                                           int32_t n_tiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + r * kBuildBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// One block per digit: exclusive scan of that digit's per-tile counts.
+__global__ __launch_bounds__(kBuildBlock) void k_sort_scan(uint32_t* hist, int32_t n_tiles,
+                                                           uint32_t* totals) {
+  __shared__ uint32_t s_tmp[4];
+  uint32_t* row = hist + (size_t)blockIdx.x * n_tiles;
+  uint32_t running = 0;
+  for (int32_t start = 0; start < n_tiles; start += kBuildBlock) {
+    const int32_t i = start + threadIdx.x;
+    const uint32_t v = i < n_tiles ? row[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exscan256(v, s_tmp, tot);
+    if (i < n_tiles) row[i] = running + ex;
+    running += tot;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = running;
+}
+
+// Stable scatter: wave64 match via 8 ballots, ranks across the 4 waves of a
+// round and across the kSortItems rounds of a tile in LDS.
+__global__ __launch_bounds__(kBuildBlock) void k_sort_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* kout,
+    uint32_t* vout, int32_t n, int shift, const uint32_t* __restrict__ hist,
+    const uint32_t* __restrict__ totals, int32_t n_tiles) {
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_wcnt[4][256];
+  __shared__ uint32_t s_woff[4][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t tot;
+  const uint32_t dbase = block_exscan256(totals[tid], s_tmp, tot);
+  s_base[tid] = dbase + hist[(size_t)tid * n_tiles + blockIdx.x];
+  s_run[tid] = 0;
+  for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + r * kBuildBlock + tid;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t val = valid ? vin[i] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool set = (d >> bit) & 1u;
+      const uint64_t bm = __ballot(set);
+      peers &= set ? bm : ~bm;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && rank == 0) s_wcnt[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    uint32_t run = s_run[tid];
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t c = s_wcnt[w][tid];
+      s_woff[w][tid] = run;
+      run += c;
+      s_wcnt[w][tid] = 0;
+    }
+    s_run[tid] = run;
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = s_base[d] + s_woff[wave][d] + rank;
+      if (pos >= (uint32_t)n) continue;  // unreachable when the histogram is consistent
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_gather(const float* __restrict__ v,
+                                                        const float* __restrict__ nrm,
+                                                        const int32_t* __restrict__ mat,
+                                                        const uint32_t* __restrict__ order,
+                                                        int32_t T, float4* tris, float4* tn) {
+  const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
+  if (k >= T) return;
+  const uint32_t p = order[k];
+  if (p >= (uint32_t)T) return;
+  const float* a = v + 9 * (size_t)p;
+  const float* b = nrm + 9 * (size_t)p;
+  tris[3 * (size_t)k + 0] = make_float4(a[0], a[1], a[2], __uint_as_float(p));
+  tris[3 * (size_t)k + 1] = make_float4(a[3], a[4], a[5], __uint_as_float((uint32_t)mat[p]));
+  tris[3 * (size_t)k + 2] = make_float4(a[6], a[7], a[8], 0.0f);
+  tn[3 * (size_t)k + 0] = make_float4(b[0], b[1], b[2], 0.0f);
+  tn[3 * (size_t)k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
+  tn[3 * (size_t)k + 2] = make_float4(b[6], b[7], b[8], 0.0f);
+}
+
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t n, int64_t i,
+                                     int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = keys[i], b = keys[j];
+  if (a != b) return __clz((int)(a ^ b));
+  return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_karras(const uint32_t* __restrict__ keys,
+                                                        int32_t n, float4* nodes, int32_t* parent,
+                                                        int32_t* leaf_parent) {
+  const int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int64_t lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int64_t j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int64_t s = 0, t = 0, div = 2;
+  do {
+    t = (l + div - 1) / div;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    div *= 2;
+  } while (t > 1);
+  const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+  if (g < 0 || g + 1 >= n) return;  // unreachable for sorted keys
+  const bool left_leaf = (i < j ? i : j) == g;
+  const bool right_leaf = (i > j ? i : j) == g + 1;
+  const uint32_t left = left_leaf ? (kLeaf | (uint32_t)g) : (uint32_t)g;
+  const uint32_t right = right_leaf ? (kLeaf | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
+  nodes[4 * i + 3] = make_float4(__uint_as_float(left), __uint_as_float(right), 0.0f, 0.0f);
+  if (left_leaf)
+    leaf_parent[g] = (int32_t)i;
+  else
+    parent[g] = (int32_t)i;
+  if (right_leaf)
+    leaf_parent[g + 1] = (int32_t)i;
+  else
+    parent[g + 1] = (int32_t)i;
+}
+
+__device__ __forceinline__ void store_slot(float4* nodes, int32_t node, int slot, const float* b) {
+  float* f = reinterpret_cast<float*>(nodes) + 16 * (size_t)node + 6 * slot;
+  for (int c = 0; c < 6; ++c) f[c] = b[c];
+}
+
+// Bottom-up refit: one work-item per leaf climbs while it is the second of
+// its parent's two children to arrive (agent-scope acq_rel counter per node).
+__global__ __launch_bounds__(kBuildBlock) void k_refit(const float4* __restrict__ tris, int32_t n,
+                                                       float4* nodes,
+                                                       const int32_t* __restrict__ parent,
+                                                       const int32_t* __restrict__ leaf_parent,
+                                                       uint32_t* flags) {
+  const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
+  if (k >= n) return;
+  float b[6];
+  {
+    const float4 a = tris[3 * (size_t)k], c = tris[3 * (size_t)k + 1], e = tris[3 * (size_t)k + 2];
+    b[0] = fminf(fminf(a.x, c.x), e.x);
+    b[1] = fminf(fminf(a.y, c.y), e.y);
+    b[2] = fminf(fminf(a.z, c.z), e.z);
+    b[3] = fmaxf(fmaxf(a.x, c.x), e.x);
+    b[4] = fmaxf(fmaxf(a.y, c.y), e.y);
+    b[5] = fmaxf(fmaxf(a.z, c.z), e.z);
+  }
+  int32_t p = leaf_parent[k];
+  if (p < 0 || p >= n - 1) return;
+  {
+    const uint32_t r0 = __float_as_uint(nodes[4 * (size_t)p + 3].x);
+    store_slot(nodes, p, r0 == (kLeaf | (uint32_t)k) ? 0 : 1, b);
+  }
+  for (int level = 0; level < 128; ++level) {
+    const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    const float* f = reinterpret_cast<const float*>(nodes) + 16 * (size_t)p;
+    float u[12];
+    for (int c = 0; c < 12; ++c)
+      u[c] = __hip_atomic_load(&f[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b[0] = fminf(u[0], u[6]);
+    b[1] = fminf(u[1], u[7]);
+    b[2] = fminf(u[2], u[8]);
+    b[3] = fmaxf(u[3], u[9]);
+    b[4] = fmaxf(u[4], u[10]);
+    b[5] = fmaxf(u[5], u[11]);
+    if (p == 0) return;
+    const int32_t q = parent[p];
+    if (q < 0 || q >= n - 1) return;
+    const uint32_t r0 = __float_as_uint(nodes[4 * (size_t)q + 3].x);
+    store_slot(nodes, q, r0 == (uint32_t)p ? 0 : 1, b);
+    p = q;
+  }
+}
+
+// T == 1: a root with the single leaf as child 0 and an empty child 1.
+__global__ void k_single_root(const float4* __restrict__ tris, float4* nodes) {
+  const float4 a = tris[0], c = tris[1], e = tris[2];
+  nodes[0] = make_float4(fminf(fminf(a.x, c.x), e.x), fminf(fminf(a.y, c.y), e.y),
+                         fminf(fminf(a.z, c.z), e.z), fmaxf(fmaxf(a.x, c.x), e.x));
+  nodes[1] = make_float4(fmaxf(fmaxf(a.y, c.y), e.y), fmaxf(fmaxf(a.z, c.z), e.z), 0.0f, 0.0f);
+  nodes[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  nodes[3] = make_float4(__uint_as_float(kLeaf), __uint_as_float(kEmpty), 0.0f, 0.0f);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+struct lisa_ctx {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  int32_t T = 0, nI = 0, M = 0;
+  float4* d_nodes = nullptr;
+  float4* d_tris = nullptr;
+  float4* d_nrm = nullptr;
+  DevMat* d_mats = nullptr;
+  uint32_t* d_morton = nullptr;
+  uint32_t* d_prim = nullptr;
+  bool has_scene = false;
+  float4* d_accum = nullptr;
+  size_t accum_px = 0;
+  unsigned long long* d_counters = nullptr;
+  uint64_t counters[LISA_NCOUNTERS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float render_ms = 0.0f, bvh_ms = 0.0f;
+  uint64_t scene_bytes = 0, accum_bytes = 0;
+};
+
+namespace {
+
+lisa_status fail(lisa_ctx* ctx, lisa_status code, const std::string& msg) {
+  if (ctx)
+    ctx->err = msg;
+  else
+    lisa::set_global_error(msg);
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail((ctx), LISA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void free_scene(lisa_ctx* c) {
+  (void)hipFree(c->d_nodes);
+  (void)hipFree(c->d_tris);
+  (void)hipFree(c->d_nrm);
+  (void)hipFree(c->d_mats);
+  (void)hipFree(c->d_morton);
+  (void)hipFree(c->d_prim);
+  c->d_nodes = nullptr;
+  c->d_tris = c->d_nrm = nullptr;
+  c->d_mats = nullptr;
+  c->d_morton = c->d_prim = nullptr;
+  c->T = c->nI = c->M = 0;
+  c->has_scene = false;
+  c->scene_bytes = 0;
+}
+
+SceneView view(const lisa_ctx* c) {
+  return SceneView{c->d_nodes, c->d_tris, c->d_nrm, c->d_mats, c->T};
+}
+
+// Device scratch freed at scope exit.
+struct Scratch {
+  std::vector<void*> ptrs;
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  hipError_t alloc(T** p, size_t count) {
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) ptrs.push_back(*p);
+    return e;
+  }
+};
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+lisa_status check_params(lisa_ctx* ctx, const lisa_render_params* p) {
+  if (!p) return fail(ctx, LISA_ERR_INVALID_ARG, "render params is NULL");
+  if (p->width == 0 || p->height == 0) return fail(ctx, LISA_ERR_INVALID_ARG, "width/height must be > 0");
+  if ((uint64_t)p->width * p->height > (1ull << 31))
+    return fail(ctx, LISA_ERR_INVALID_ARG, "image larger than 2^31 pixels");
+  if (p->spp == 0) return fail(ctx, LISA_ERR_INVALID_ARG, "spp must be > 0");
+  if (p->mode != LISA_MODE_PLAIN) return fail(ctx, LISA_ERR_UNSUPPORTED, "unknown render mode");
+  if (!ctx->has_scene) return fail(ctx, LISA_ERR_NO_SCENE, "lisa_set_scene has not been called");
+  return LISA_OK;
+}
+
+PathArgs make_args(const lisa_render_params* p) {
+  PathArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.width = p->width;
+  a.height = p->height;
+  a.spp = p->spp;
+  a.bounces = p->bounces;
+  a.subframe = p->subframe;
+  a.eye = mk(p->eye[0], p->eye[1], p->eye[2]);
+  a.U = mk(p->U[0], p->U[1], p->U[2]);
+  a.V = mk(p->V[0], p->V[1], p->V[2]);
+  a.W = mk(p->W[0], p->W[1], p->W[2]);
+  a.fp64 = (p->flags & LISA_FLAG_FP64_ACCUM) ? 1 : 0;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+lisa_status lisa_create(lisa_ctx** out, int32_t device) {
+  if (!out) return fail(nullptr, LISA_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return fail(nullptr, LISA_ERR_HIP, std::string("no HIP device: ") + hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(nullptr, LISA_ERR_INVALID_ARG, "device index out of range");
+  lisa_ctx* c = new lisa_ctx;
+  c->device = device;
+  if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&c->stream)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->d_counters),
+                     LISA_NCOUNTERS * sizeof(unsigned long long))) != hipSuccess) {
+    const std::string msg = std::string("lisa_create: ") + hipGetErrorString(e);
+    lisa_destroy(c);
+    return fail(nullptr, LISA_ERR_HIP, msg);
+  }
+  *out = c;
+  return LISA_OK;
+}
+
+void lisa_destroy(lisa_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_scene(c);
+  (void)hipFree(c->d_accum);
+  (void)hipFree(c->d_counters);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* lisa_last_error(const lisa_ctx* c) { return c ? c->err.c_str() : lisa::global_error(); }
+
+lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* normals,
+                           const int32_t* mat_idx, int32_t T, const lisa_material* mats,
+                           int32_t M) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (T < 0 || M < 0) return fail(ctx, LISA_ERR_INVALID_ARG, "negative T or M");
+  if (T > 0 && (!verts || !normals || !mat_idx))
+    return fail(ctx, LISA_ERR_INVALID_ARG, "verts/normals/mat_idx NULL with T > 0");
+  if (T > 0 && (M <= 0 || !mats)) return fail(ctx, LISA_ERR_INVALID_ARG, "no materials");
+  if (T >= (1 << 30)) return fail(ctx, LISA_ERR_INVALID_ARG, "too many triangles (>= 2^30)");
+  for (int32_t k = 0; k < T; ++k)
+    if (mat_idx[k] < 0 || mat_idx[k] >= M)
+      return fail(ctx, LISA_ERR_INVALID_ARG, "material index out of range at triangle " + std::to_string(k));
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  free_scene(ctx);
+  ctx->T = T;
+  ctx->M = M;
+  ctx->nI = T > 1 ? T - 1 : 1;
+  ctx->bvh_ms = 0.0f;
+  std::vector<DevMat> hm(std::max(M, 1));
+  for (int32_t i = 0; i < M; ++i) {
+    const lisa_material& m = mats[i];
+    hm[i].diffuse_alpha = make_float4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.alpha);
+    hm[i].emission_emit = make_float4(m.emission[0], m.emission[1], m.emission[2], m.emit ? 1.0f : 0.0f);
+    hm[i].rough_n = make_float4(m.roughness, m.n, 0.0f, 0.0f);
+  }
+  const size_t nodes_b = 4 * sizeof(float4) * (size_t)ctx->nI;
+  const size_t tris_b = 3 * sizeof(float4) * (size_t)std::max(T, 1);
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mats), sizeof(DevMat) * hm.size()));
+  HIP_TRY(ctx, hipMemcpy(ctx->d_mats, hm.data(), sizeof(DevMat) * hm.size(), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_nodes), nodes_b));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_tris), tris_b));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_nrm), tris_b));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_morton), sizeof(uint32_t) * std::max(T, 1)));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_prim), sizeof(uint32_t) * std::max(T, 1)));
+  ctx->scene_bytes = sizeof(DevMat) * hm.size() + nodes_b + 2 * tris_b + 8 * (size_t)std::max(T, 1);
+  HIP_TRY(ctx, hipMemsetAsync(ctx->d_nodes, 0, nodes_b, ctx->stream));
+  if (T == 0) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->has_scene = true;
+    return LISA_OK;
+  }
+  Scratch S;
+  float *d_v = nullptr, *d_n = nullptr, *d_part = nullptr, *d_bounds = nullptr;
+  int32_t *d_mat = nullptr, *d_parent = nullptr, *d_leafp = nullptr;
+  uint32_t *d_k2 = nullptr, *d_v2 = nullptr, *d_hist = nullptr, *d_tot = nullptr, *d_flags = nullptr;
+  const int32_t n_tiles = (T + kSortTile - 1) / kSortTile;
+  const int32_t n_part = std::min<int32_t>(1024, (T + kBuildBlock - 1) / kBuildBlock);
+  HIP_TRY(ctx, S.alloc(&d_v, 9 * (size_t)T));
+  HIP_TRY(ctx, S.alloc(&d_n, 9 * (size_t)T));
+  HIP_TRY(ctx, S.alloc(&d_mat, (size_t)T));
+  HIP_TRY(ctx, S.alloc(&d_part, 6 * (size_t)n_part));
+  HIP_TRY(ctx, S.alloc(&d_bounds, 6));
+  HIP_TRY(ctx, S.alloc(&d_k2, (size_t)T));
+  HIP_TRY(ctx, S.alloc(&d_v2, (size_t)T));
+  HIP_TRY(ctx, S.alloc(&d_hist, 256 * (size_t)n_tiles));
+  HIP_TRY(ctx, S.alloc(&d_tot, 256));
+  HIP_TRY(ctx, S.alloc(&d_parent, (size_t)T));
+  HIP_TRY(ctx, S.alloc(&d_leafp, (size_t)T));
+  HIP_TRY(ctx, S.alloc(&d_flags, (size_t)T));
+  HIP_TRY(ctx, hipMemcpy(d_v, verts, 9 * sizeof(float) * (size_t)T, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(d_n, normals, 9 * sizeof(float) * (size_t)T, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(d_mat, mat_idx, sizeof(int32_t) * (size_t)T, hipMemcpyHostToDevice));
+
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  k_centroid_partial<<<n_part, kBuildBlock, 0, st>>>(d_v, T, d_part);
+  k_centroid_final<<<1, kBuildBlock, 0, st>>>(d_part, n_part, d_bounds);
+  k_morton<<<grid_for(T, kBuildBlock), kBuildBlock, 0, st>>>(d_v, T, d_bounds, ctx->d_morton, ctx->d_prim);
+  uint32_t *ka = ctx->d_morton, *va = ctx->d_prim, *kb = d_k2, *vb = d_v2;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 8 * pass;
+    k_sort_hist<<<n_tiles, kBuildBlock, 0, st>>>(ka, T, shift, d_hist, n_tiles);
+    k_sort_scan<<<256, kBuildBlock, 0, st>>>(d_hist, n_tiles, d_tot);
+    k_sort_scatter<<<n_tiles, kBuildBlock, 0, st>>>(ka, va, kb, vb, T, shift, d_hist, d_tot, n_tiles);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  // four passes: the sorted data is back in ctx->d_morton / ctx->d_prim
+  k_gather<<<grid_for(T, kBuildBlock), kBuildBlock, 0, st>>>(d_v, d_n, d_mat, ctx->d_prim, T,
+                                                             ctx->d_tris, ctx->d_nrm);
+  if (T == 1) {
+    k_single_root<<<1, 1, 0, st>>>(ctx->d_tris, ctx->d_nodes);
+  } else {
+    HIP_TRY(ctx, hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * (size_t)T, st));
+    HIP_TRY(ctx, hipMemsetAsync(d_parent, 0xFF, sizeof(int32_t) * (size_t)T, st));
+    HIP_TRY(ctx, hipMemsetAsync(d_leafp, 0xFF, sizeof(int32_t) * (size_t)T, st));
+    k_karras<<<grid_for(T - 1, kBuildBlock), kBuildBlock, 0, st>>>(ctx->d_morton, T, ctx->d_nodes,
+                                                                  d_parent, d_leafp);
+    k_refit<<<grid_for(T, kBuildBlock), kBuildBlock, 0, st>>>(ctx->d_tris, T, ctx->d_nodes,
+                                                             d_parent, d_leafp, d_flags);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
+  ctx->has_scene = true;
+  return LISA_OK;
+}
+
+lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum_rgba_out,
+                        uint8_t* rgba8_out) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  lisa_status rc = check_params(ctx, p);
+  if (rc != LISA_OK) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t npx = (size_t)p->width * p->height;
+  if (ctx->accum_px != npx) {
+    if (p->subframe > 0)
+      return fail(ctx, LISA_ERR_INVALID_ARG, "subframe > 0 but the accum buffer has another size");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->d_accum);
+    ctx->d_accum = nullptr;
+    ctx->accum_px = 0;
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_accum), npx * sizeof(float4)));
+    ctx->accum_px = npx;
+    ctx->accum_bytes = npx * sizeof(float4);
+  }
+  PathArgs a = make_args(p);
+  a.tiles_x = (p->width + kTileW - 1) / kTileW;
+  const uint32_t tiles_y = (p->height + kTileH - 1) / kTileH;
+  a.n_items = (int32_t)npx;
+  a.accum = ctx->d_accum;
+  a.accum_prev = ctx->d_accum;
+  a.counters = ctx->d_counters;
+  const bool count = (p->flags & LISA_FLAG_COUNT) != 0;
+  hipStream_t st = ctx->stream;
+  if (count)
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters, 0, LISA_NCOUNTERS * sizeof(unsigned long long), st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  const unsigned grid = a.tiles_x * tiles_y;
+  if (count)
+    k_path<false, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
+  else
+    k_path<false, false><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  if (accum_rgba_out || rgba8_out) {
+    std::vector<float> tmp;
+    float* dst = accum_rgba_out;
+    if (!dst) {
+      tmp.resize(npx * 4);
+      dst = tmp.data();
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(dst, ctx->d_accum, npx * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (rgba8_out) lisa_srgb_quantize(dst, (int64_t)npx, rgba8_out);
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  HIP_TRY(ctx, hipEventElapsedTime(&ctx->render_ms, ctx->ev0, ctx->ev1));
+  if (count) {
+    unsigned long long h[LISA_NCOUNTERS];
+    HIP_TRY(ctx, hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+    for (int i = 0; i < LISA_NCOUNTERS; ++i) ctx->counters[i] = h[i];
+  }
+  return LISA_OK;
+}
+
+lisa_status lisa_trace_subset(lisa_ctx* ctx, const lisa_render_params* p, const int32_t* pixels,
+                              int32_t n_pixels, float* accum_rgba_out, float* sample_rgb,
+                              int32_t* sample_bounces, int32_t* sample_occl,
+                              uint32_t* sample_hash, int32_t* rad_hits, int32_t record_bounces) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  lisa_status rc = check_params(ctx, p);
+  if (rc != LISA_OK) return rc;
+  if (n_pixels <= 0 || !pixels || !accum_rgba_out)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "pixels / accum_rgba_out required, n_pixels > 0");
+  if (p->subframe > 0) return fail(ctx, LISA_ERR_UNSUPPORTED, "lisa_trace_subset runs subframe 0 only");
+  if (record_bounces < 0) return fail(ctx, LISA_ERR_INVALID_ARG, "record_bounces < 0");
+  const int64_t npx = (int64_t)p->width * p->height;
+  for (int32_t i = 0; i < n_pixels; ++i)
+    if (pixels[i] < 0 || pixels[i] >= npx)
+      return fail(ctx, LISA_ERR_INVALID_ARG, "pixel id out of range");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t ns = (size_t)n_pixels * p->spp;
+  Scratch S;
+  int32_t* d_px = nullptr;
+  float4* d_acc = nullptr;
+  float* d_rgb = nullptr;
+  int32_t *d_b = nullptr, *d_o = nullptr, *d_rh = nullptr;
+  uint32_t* d_h = nullptr;
+  HIP_TRY(ctx, S.alloc(&d_px, (size_t)n_pixels));
+  HIP_TRY(ctx, S.alloc(&d_acc, (size_t)n_pixels));
+  if (sample_rgb) HIP_TRY(ctx, S.alloc(&d_rgb, 3 * ns));
+  if (sample_bounces) HIP_TRY(ctx, S.alloc(&d_b, ns));
+  if (sample_occl) HIP_TRY(ctx, S.alloc(&d_o, ns));
+  if (sample_hash) HIP_TRY(ctx, S.alloc(&d_h, ns));
+  if (rad_hits && record_bounces > 0) HIP_TRY(ctx, S.alloc(&d_rh, ns * record_bounces));
+  HIP_TRY(ctx, hipMemcpy(d_px, pixels, sizeof(int32_t) * n_pixels, hipMemcpyHostToDevice));
+  PathArgs a = make_args(p);
+  a.pixels = d_px;
+  a.n_items = n_pixels;
+  a.accum = d_acc;
+  a.accum_prev = nullptr;
+  a.s_rgb = d_rgb;
+  a.s_b = d_b;
+  a.s_o = d_o;
+  a.s_h = d_h;
+  a.rad_hits = d_rh;
+  a.rb = d_rh ? record_bounces : 0;
+  a.counters = ctx->d_counters;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters, 0, LISA_NCOUNTERS * sizeof(unsigned long long), st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  k_path<true, true><<<grid_for(n_pixels, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  HIP_TRY(ctx, hipEventElapsedTime(&ctx->render_ms, ctx->ev0, ctx->ev1));
+  HIP_TRY(ctx, hipMemcpy(accum_rgba_out, d_acc, sizeof(float4) * n_pixels, hipMemcpyDeviceToHost));
+  if (d_rgb) HIP_TRY(ctx, hipMemcpy(sample_rgb, d_rgb, sizeof(float) * 3 * ns, hipMemcpyDeviceToHost));
+  if (d_b) HIP_TRY(ctx, hipMemcpy(sample_bounces, d_b, sizeof(int32_t) * ns, hipMemcpyDeviceToHost));
+  if (d_o) HIP_TRY(ctx, hipMemcpy(sample_occl, d_o, sizeof(int32_t) * ns, hipMemcpyDeviceToHost));
+  if (d_h) HIP_TRY(ctx, hipMemcpy(sample_hash, d_h, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+  if (d_rh)
+    HIP_TRY(ctx, hipMemcpy(rad_hits, d_rh, sizeof(int32_t) * ns * record_bounces, hipMemcpyDeviceToHost));
+  unsigned long long h[LISA_NCOUNTERS];
+  HIP_TRY(ctx, hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+  for (int i = 0; i < LISA_NCOUNTERS; ++i) ctx->counters[i] = h[i];
+  return LISA_OK;
+}
+
+lisa_status lisa_trace_rays(lisa_ctx* ctx, const float* O, const float* D, int32_t n, float tmin,
+                            float tmax, int32_t* prim_out, float* t_out) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!ctx->has_scene) return fail(ctx, LISA_ERR_NO_SCENE, "lisa_set_scene has not been called");
+  if (n < 0 || (n > 0 && (!O || !D || !prim_out || !t_out)))
+    return fail(ctx, LISA_ERR_INVALID_ARG, "bad ray arrays");
+  if (n == 0) return LISA_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  Scratch S;
+  float *d_o = nullptr, *d_d = nullptr, *d_t = nullptr;
+  int32_t* d_p = nullptr;
+  HIP_TRY(ctx, S.alloc(&d_o, 3 * (size_t)n));
+  HIP_TRY(ctx, S.alloc(&d_d, 3 * (size_t)n));
+  HIP_TRY(ctx, S.alloc(&d_t, (size_t)n));
+  HIP_TRY(ctx, S.alloc(&d_p, (size_t)n));
+  HIP_TRY(ctx, hipMemcpy(d_o, O, 12 * (size_t)n, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(d_d, D, 12 * (size_t)n, hipMemcpyHostToDevice));
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters, 0, LISA_NCOUNTERS * sizeof(unsigned long long), st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  k_trace_rays<true><<<grid_for(n, kRayBlock), kRayBlock, 0, st>>>(view(ctx), d_o, d_d, n, tmin, tmax,
+                                                                   d_p, d_t, ctx->d_counters);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  HIP_TRY(ctx, hipEventElapsedTime(&ctx->render_ms, ctx->ev0, ctx->ev1));
+  HIP_TRY(ctx, hipMemcpy(prim_out, d_p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(t_out, d_t, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  unsigned long long h[LISA_NCOUNTERS];
+  HIP_TRY(ctx, hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+  for (int i = 0; i < LISA_NCOUNTERS; ++i) ctx->counters[i] = h[i];
+  return LISA_OK;
+}
+
+lisa_status lisa_get_bvh(lisa_ctx* ctx, uint32_t* morton, uint32_t* prim, uint32_t* child_refs,
+                         float* child_boxes) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!ctx->has_scene) return fail(ctx, LISA_ERR_NO_SCENE, "lisa_set_scene has not been called");
+  if (ctx->T == 0) return LISA_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (morton) HIP_TRY(ctx, hipMemcpy(morton, ctx->d_morton, 4 * (size_t)ctx->T, hipMemcpyDeviceToHost));
+  if (prim) HIP_TRY(ctx, hipMemcpy(prim, ctx->d_prim, 4 * (size_t)ctx->T, hipMemcpyDeviceToHost));
+  if (child_refs || child_boxes) {
+    std::vector<float> nodes(16 * (size_t)ctx->nI);
+    HIP_TRY(ctx, hipMemcpy(nodes.data(), ctx->d_nodes, 64 * (size_t)ctx->nI, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < ctx->nI; ++i) {
+      if (child_boxes) std::memcpy(child_boxes + 12 * (size_t)i, &nodes[16 * (size_t)i], 48);
+      if (child_refs) std::memcpy(child_refs + 2 * (size_t)i, &nodes[16 * (size_t)i + 12], 8);
+    }
+  }
+  return LISA_OK;
+}
+
+lisa_status lisa_copy_accum(lisa_ctx* ctx, float* rgba_out, int64_t n_pixels) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!rgba_out || n_pixels < 0 || (size_t)n_pixels != ctx->accum_px)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "rgba_out NULL or n_pixels != pixels of the last render");
+  if (n_pixels == 0) return LISA_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpyAsync(rgba_out, ctx->d_accum, (size_t)n_pixels * sizeof(float4),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LISA_OK;
+}
+
+lisa_status lisa_synchronize(lisa_ctx* ctx) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LISA_OK;
+}
+
+lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out) {
+  if (!ctx || !out) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL argument");
+  out->render_ms = ctx->render_ms;
+  out->bvh_build_ms = ctx->bvh_ms;
+  out->num_tris = ctx->T;
+  out->num_nodes = ctx->nI;
+  out->device_bytes = ctx->scene_bytes + ctx->accum_bytes;
+  return LISA_OK;
+}
+
+lisa_status lisa_get_counters(const lisa_ctx* ctx, uint64_t* counters) {
+  if (!ctx || !counters) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL argument");
+  for (int i = 0; i < LISA_NCOUNTERS; ++i) counters[i] = ctx->counters[i];
+  return LISA_OK;
+}
+
+void* lisa_get_stream(const lisa_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,773 @@
+// CPU oracle for the LiSA path-tracing hot path.  TEST INFRASTRUCTURE ONLY
+// (see lisa_oracle.h): the product library never links or calls this file.
+//
+// Built with -O3 -ffp-contract=off and no fast-math, so every FP32 operation
+// below is one IEEE-rounded operation in the order written.  The HIP kernels
+// are written to the same operation order; parity tests compare the two.
+#include "lisa_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// float3 algebra, op order as src/sutil/vec_math.h (dot 521, cross 527,
+// normalize 539-543, reflect 552-555, faceforward 561-564, lerp 494-497,
+// clamp 113, float3 / s 477-481).
+// ---------------------------------------------------------------------------
+struct V3 {
+  float x, y, z;
+};
+inline V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator-(V3 a) { return mk(-a.x, -a.y, -a.z); }
+inline V3 operator*(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+inline V3 operator*(float s, V3 a) { return mk(a.x * s, a.y * s, a.z * s); }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline V3 normalize(V3 v) {
+  const float inv = 1.0f / std::sqrt(dot(v, v));
+  return v * inv;
+}
+inline V3 reflect(V3 i, V3 n) { return i - (2.0f * n) * dot(n, i); }
+inline V3 faceforward(V3 n, V3 i, V3 nref) { return n * std::copysign(1.0f, dot(i, nref)); }
+inline V3 lerp(V3 a, V3 b, float t) { return a + t * (b - a); }
+inline float clampf(float f, float a, float b) { return std::fmax(a, std::fmin(f, b)); }
+inline float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+inline V3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+inline void st3(float* p, V3 v) {
+  p[0] = v.x;
+  p[1] = v.y;
+  p[2] = v.z;
+}
+
+// ---------------------------------------------------------------------------
+// RNG: src/cuda/random.h:31-46 (tea<16>: TEA, Wheeler & Needham 1994, as a
+// seed hash per Zafar, Olano & Curtis, HPG 2010), :49-55 (lcg, Numerical
+// Recipes constants), :64-67 (rnd);  src/LiSA/src/maths.cu:6-8 (rng).
+// ---------------------------------------------------------------------------
+inline uint32_t tea16(uint32_t v0, uint32_t v1) {
+  uint32_t s0 = 0;
+  for (int n = 0; n < 16; ++n) {
+    s0 += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+inline uint32_t lcg(uint32_t& s) {
+  s = 1664525u * s + 1013904223u;
+  return s & 0x00FFFFFFu;
+}
+inline float rnd(uint32_t& s) { return (float)lcg(s) / (float)0x01000000; }
+inline float rng(uint32_t& s) { return rnd(s) * 2.0f - 1.0f; }
+
+// maths.cu:10-15 -- x, then y, then z.
+inline V3 hemisphere(V3 N, uint32_t& s) {
+  const float x = rng(s);
+  const float y = rng(s);
+  const float z = rng(s);
+  const V3 r = normalize(mk(x, y, z));
+  return faceforward(r, N, r);
+}
+// maths.cu:17-20 (Schlick); pow(r,2) = r*r, pow(x,5) = ((x*x)*(x*x))*x.
+inline float fresnel(float cosT, float eta) {
+  const float r = (1.0f - eta) / (1.0f + eta);
+  const float R0 = r * r;
+  const float x = 1.0f - cosT;
+  const float x2 = x * x;
+  const float x5 = (x2 * x2) * x;
+  return R0 + (1.0f - R0) * x5;
+}
+// maths.cu:22-30: zero vector on total internal reflection.
+inline V3 refract(float cosI, V3 d, V3 N, float eta) {
+  const float cost2 = 1.0f - eta * eta * (1.0f - cosI * cosI);
+  const V3 t = eta * d + ((eta * cosI - std::sqrt(std::fabs(cost2))) * N);
+  return t * (float)(cost2 > 0.0f);
+}
+// maths.cu:33-56.
+inline V3 bary_normal(V3 P, V3 n1, V3 n2, V3 n3, V3 v1, V3 v2, V3 v3) {
+  const V3 e1 = v2 - v1;
+  const V3 e2 = v3 - v1;
+  const V3 i = P - v1;
+  const float d00 = dot(e1, e1);
+  const float d01 = dot(e1, e2);
+  const float d11 = dot(e2, e2);
+  const float d20 = dot(i, e1);
+  const float d21 = dot(i, e2);
+  const float denom = d00 * d11 - d01 * d01;
+  const float w = (d00 * d21 - d01 * d20) / denom;
+  const float v = (d11 * d20 - d01 * d21) / denom;
+  const float u = 1.0f - v - w;
+  return normalize(u * n1 + v * n2 + w * n3);
+}
+// lambertian.cu:7-13 (sample), :15-22 (eval: (double)c / M_PI then float), :24-27.
+inline V3 bsdf_bounce(V3 d, V3 N, uint32_t& s, float roughness) {
+  const V3 r = reflect(d, N);
+  const V3 h = hemisphere(N, s);
+  return lerp(r, h, roughness);
+}
+inline float bsdf_eval(V3 N, V3 L) {
+  const float NdotL = clampf(dot(N, L), 0.0f, 1.0f);
+  const float p = (float)((double)NdotL / 3.14159265358979323846);
+  return NdotL * p;
+}
+
+// src/cuda/helpers.h:35-66 on the host.
+inline float to_srgb(float c) {
+  const float p = std::pow(c, 1.0f / 2.4f);
+  return c < 0.0031308f ? 12.92f * c : 1.055f * p - 0.055f;
+}
+inline uint8_t quant8(float x) {
+  x = clampf(x, 0.0f, 1.0f);
+  const unsigned v = (unsigned)(x * 256.0f);
+  return (uint8_t)std::min(v, 255u);
+}
+
+// ---------------------------------------------------------------------------
+// LBVH (Karras, "Maximizing Parallelism in the Construction of BVHs,
+// Octrees, and k-d Trees", HPG 2012) over the triangle soup.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT 2013): widens the
+// far slab distance so rounding never culls a box the ray touches.
+constexpr float kRobust = 0x1.000006p+0f;
+
+// Spread the low 10 bits of v to every third bit (bit-by-bit form).
+inline uint32_t spread10(uint32_t v) {
+  uint32_t r = 0;
+  for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
+  return r;
+}
+inline uint32_t quant10(float u) {
+  float q = u * 1024.0f;
+  q = std::fmin(std::fmax(q, 0.0f), 1023.0f);
+  return (uint32_t)q;
+}
+inline uint32_t morton3(float x, float y, float z) {
+  return (spread10(quant10(x)) << 2) | (spread10(quant10(y)) << 1) | spread10(quant10(z));
+}
+
+struct Scene {
+  int32_t T = 0;
+  std::vector<float> v, n;     // 9T each, original order
+  std::vector<int32_t> mat;    // T
+  std::vector<orc_material> mats;
+  std::vector<uint32_t> morton, prim;  // sorted
+  std::vector<uint32_t> refs;          // 2 per internal node
+  std::vector<float> boxes;            // 12 per internal node
+  std::vector<float> tv;               // 9T, sorted order (traversal copy)
+};
+
+inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
+
+void build_bvh(Scene& s) {
+  const int64_t n = s.T;
+  if (n == 0) return;
+  // Centroids and their bounds.
+  std::vector<float> c(3 * n);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t k = 0; k < n; ++k) {
+    const float* t = &s.v[9 * k];
+    for (int a = 0; a < 3; ++a) {
+      const float ca = (t[a] + t[3 + a] + t[6 + a]) / 3.0f;
+      c[3 * k + a] = ca;
+      lo[a] = std::fmin(lo[a], ca);
+      hi[a] = std::fmax(hi[a], ca);
+    }
+  }
+  std::vector<uint32_t> code(n);
+  for (int64_t k = 0; k < n; ++k) {
+    float u[3];
+    for (int a = 0; a < 3; ++a) {
+      const float ext = hi[a] - lo[a];
+      u[a] = ext > 0.0f ? (c[3 * k + a] - lo[a]) / ext : 0.5f;
+    }
+    code[k] = morton3(u[0], u[1], u[2]);
+  }
+  s.prim.resize(n);
+  for (int64_t k = 0; k < n; ++k) s.prim[k] = (uint32_t)k;
+  std::stable_sort(s.prim.begin(), s.prim.end(),
+                   [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+  s.morton.resize(n);
+  for (int64_t k = 0; k < n; ++k) s.morton[k] = code[s.prim[k]];
+  s.tv.resize(9 * n);
+  for (int64_t k = 0; k < n; ++k)
+    std::memcpy(&s.tv[9 * k], &s.v[9 * (int64_t)s.prim[k]], 9 * sizeof(float));
+
+  const int64_t nI = std::max<int64_t>(n - 1, 1);
+  s.refs.assign(2 * nI, kEmpty);
+  s.boxes.assign(12 * nI, 0.0f);
+  auto leaf_box = [&](int64_t k, float* b) {
+    const float* t = &s.tv[9 * k];
+    for (int a = 0; a < 3; ++a) {
+      b[a] = std::fmin(std::fmin(t[a], t[3 + a]), t[6 + a]);
+      b[3 + a] = std::fmax(std::fmax(t[a], t[3 + a]), t[6 + a]);
+    }
+  };
+  if (n == 1) {
+    s.refs[0] = kLeaf | 0u;
+    leaf_box(0, &s.boxes[0]);
+    return;
+  }
+  // delta(i, j): common prefix length of index-augmented keys, -1 outside.
+  auto delta = [&](int64_t i, int64_t j) -> int {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = s.morton[i], b = s.morton[j];
+    if (a != b) return clz32(a ^ b);
+    return 32 + clz32((uint32_t)i ^ (uint32_t)j);
+  };
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const int d = (delta(i, i + 1) - delta(i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(i, i - d);
+    int64_t lmax = 2;
+    while (delta(i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+      if (delta(i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta(i, j);
+    int64_t sp = 0, t = 0, div = 2;
+    do {
+      t = (l + div - 1) / div;
+      if (delta(i, i + (sp + t) * d) > dnode) sp += t;
+      div *= 2;
+    } while (t > 1);
+    const int64_t g = i + sp * d + std::min(d, 0);
+    s.refs[2 * i + 0] = (std::min(i, j) == g) ? (kLeaf | (uint32_t)g) : (uint32_t)g;
+    s.refs[2 * i + 1] = (std::max(i, j) == g + 1) ? (kLeaf | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
+  }
+  // Refit: post-order, child boxes stored in the parent.
+  struct Frame {
+    uint32_t node;
+    int stage;
+  };
+  std::vector<Frame> st;
+  st.push_back({0, 0});
+  while (!st.empty()) {
+    Frame& f = st.back();
+    const uint32_t nd = f.node;
+    if (f.stage < 2) {
+      const uint32_t r = s.refs[2 * nd + f.stage];
+      f.stage++;
+      if (!(r & kLeaf)) st.push_back({r, 0});
+      continue;
+    }
+    for (int ch = 0; ch < 2; ++ch) {
+      const uint32_t r = s.refs[2 * nd + ch];
+      float* b = &s.boxes[12 * nd + 6 * ch];
+      if (r & kLeaf) {
+        leaf_box(r & ~kLeaf, b);
+      } else {
+        const float* cb = &s.boxes[12 * r];
+        for (int a = 0; a < 3; ++a) {
+          b[a] = std::fmin(cb[a], cb[6 + a]);
+          b[3 + a] = std::fmax(cb[3 + a], cb[9 + a]);
+        }
+      }
+    }
+    st.pop_back();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Traversal + watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2013).
+// ---------------------------------------------------------------------------
+struct Ray {
+  V3 O, D, inv;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+  bool valid;
+};
+
+inline Ray make_ray(V3 O, V3 D) {
+  Ray r;
+  r.O = O;
+  r.D = D;
+  r.valid = std::isfinite(D.x) && std::isfinite(D.y) && std::isfinite(D.z) &&
+            (D.x != 0.0f || D.y != 0.0f || D.z != 0.0f);
+  r.inv = mk(1.0f / D.x, 1.0f / D.y, 1.0f / D.z);
+  const float ax = std::fabs(D.x), ay = std::fabs(D.y), az = std::fabs(D.z);
+  int kz = 0;
+  float m = ax;
+  if (ay > m) {
+    kz = 1;
+    m = ay;
+  }
+  if (az > m) kz = 2;
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  if (comp(D, kz) < 0.0f) std::swap(kx, ky);
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  r.Sx = comp(D, kx) / comp(D, kz);
+  r.Sy = comp(D, ky) / comp(D, kz);
+  r.Sz = 1.0f / comp(D, kz);
+  return r;
+}
+
+inline bool watertight(const Ray& r, const float* tri, float& t) {
+  const V3 A = ld3(tri) - r.O, B = ld3(tri + 3) - r.O, C = ld3(tri + 6) - r.O;
+  const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+  const float Ax = comp(A, r.kx) - r.Sx * Akz;
+  const float Ay = comp(A, r.ky) - r.Sy * Akz;
+  const float Bx = comp(B, r.kx) - r.Sx * Bkz;
+  const float By = comp(B, r.ky) - r.Sy * Bkz;
+  const float Cx = comp(C, r.kx) - r.Sx * Ckz;
+  const float Cy = comp(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  const float T = U * Az + V * Bz + W * Cz;
+  t = T / det;
+  return true;
+}
+
+inline bool slab(const float* b, const Ray& r, float tmin, float tmax, float& tn_out) {
+  const float tx0 = (b[0] - r.O.x) * r.inv.x, tx1 = (b[3] - r.O.x) * r.inv.x;
+  const float ty0 = (b[1] - r.O.y) * r.inv.y, ty1 = (b[4] - r.O.y) * r.inv.y;
+  const float tz0 = (b[2] - r.O.z) * r.inv.z, tz1 = (b[5] - r.O.z) * r.inv.z;
+  const float tn = std::fmax(std::fmax(std::fmin(tx0, tx1), std::fmin(ty0, ty1)),
+                             std::fmax(std::fmin(tz0, tz1), tmin));
+  const float tfb = std::fmin(std::fmin(std::fmax(tx0, tx1), std::fmax(ty0, ty1)),
+                              std::fmax(tz0, tz1)) * kRobust;
+  const float tf = std::fmin(tfb, tmax);
+  tn_out = tn;
+  return tn <= tf;
+}
+
+struct Counters {
+  uint64_t c[ORC_NCOUNTERS] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Closest hit: returns the original triangle id (-1 on miss).  Accepts
+// t in (tmin, tmax]; equal t resolves to the lower triangle id.
+int32_t closest(const Scene& s, const Ray& r, float tmin, float tmax, float& t_out, Counters& cnt) {
+  if (!r.valid || s.T == 0) return -1;
+  float best_t = tmax;
+  uint32_t best = 0xFFFFFFFFu;
+  uint32_t stack[64];
+  int sp = 0;
+  uint32_t node = 0;
+  auto tri_test = [&](uint32_t k) {
+    cnt.c[3]++;
+    float t;
+    const uint32_t p = s.prim[k];
+    if (watertight(r, &s.tv[9 * (size_t)k], t) && t > tmin &&
+        (t < best_t || (t == best_t && p < best))) {
+      best_t = t;
+      best = p;
+    }
+  };
+  for (;;) {
+    cnt.c[2]++;
+    const float* bx = &s.boxes[12 * (size_t)node];
+    const uint32_t r0 = s.refs[2 * (size_t)node], r1 = s.refs[2 * (size_t)node + 1];
+    float tn0 = 0.0f, tn1 = 0.0f;
+    bool h0 = r0 != kEmpty && slab(bx, r, tmin, best_t, tn0);
+    bool h1 = r1 != kEmpty && slab(bx + 6, r, tmin, best_t, tn1);
+    if (h0 && (r0 & kLeaf)) {
+      tri_test(r0 & ~kLeaf);
+      h0 = false;
+    }
+    if (h1 && (r1 & kLeaf)) {
+      tri_test(r1 & ~kLeaf);
+      h1 = false;
+    }
+    if (h0 && h1) {
+      uint32_t nr = r0, fr = r1;
+      if (tn1 < tn0) {
+        nr = r1;
+        fr = r0;
+      }
+      stack[sp++] = fr;
+      node = nr;
+    } else if (h0) {
+      node = r0;
+    } else if (h1) {
+      node = r1;
+    } else {
+      if (sp == 0) break;
+      node = stack[--sp];
+    }
+  }
+  if (best == 0xFFFFFFFFu) return -1;
+  t_out = best_t;
+  return (int32_t)best;
+}
+
+int32_t closest_brute(const Scene& s, const Ray& r, float tmin, float tmax, float& t_out) {
+  if (!r.valid || s.T == 0) return -1;
+  float best_t = tmax;
+  uint32_t best = 0xFFFFFFFFu;
+  for (int32_t p = 0; p < s.T; ++p) {
+    float t;
+    if (watertight(r, &s.v[9 * (size_t)p], t) && t > tmin &&
+        (t < best_t || (t == best_t && (uint32_t)p < best))) {
+      best_t = t;
+      best = (uint32_t)p;
+    }
+  }
+  if (best == 0xFFFFFFFFu) return -1;
+  t_out = best_t;
+  return (int32_t)best;
+}
+
+// ---------------------------------------------------------------------------
+// Path loop: src/LiSA/src/shader.cu.
+// ---------------------------------------------------------------------------
+constexpr float kTmin = 1e-4f;  // shader.cu:114, :202
+constexpr float kTmax = 1e16f;  // shader.cu:115, :202
+constexpr int kNeeTries = 30;   // shader.cu:199
+
+struct RayState {  // shader.cu:30-42
+  int32_t light_mat = -1;
+  V3 normal{0, 0, 0};
+  V3 origin{0, 0, 0};
+  V3 att{1.0f, 1.0f, 1.0f};
+  V3 color{0.0f, 0.0f, 0.0f};
+  V3 direction{0, 0, 0};
+  bool hit = false;
+  bool done = false;
+};
+
+struct SampleRec {
+  int32_t bounces = 0;
+  int32_t occl = 0;
+  uint32_t hash = 2166136261u;
+  int32_t* rad_hits = nullptr;
+  int32_t rb = 0;
+  void ray(int32_t prim, bool occlusion) {
+    const uint32_t code = (uint32_t)prim ^ (occlusion ? 0x5bd1e995u : 0u);
+    hash = (hash ^ code) * 16777619u;
+  }
+};
+
+struct Tracer {
+  const Scene& s;
+  Counters& cnt;
+  uint32_t& seed;
+  SampleRec& rec;
+
+  const orc_material& mat_of(int32_t prim) const { return s.mats[s.mat[prim]]; }
+
+  int32_t trace_ray(V3 O, V3 D, float& t) {
+    const Ray r = make_ray(O, D);
+    if (!r.valid) {
+      cnt.c[6]++;
+      return -2;
+    }
+    return closest(s, r, kTmin, kTmax, t, cnt);
+  }
+
+  // shader.cu:196-209 (+ occlusion CH/miss 172-184).
+  V3 shoot_ray_to_light(RayState& st) {
+    for (int i = 0; i < kNeeTries; ++i) {
+      const V3 dir = hemisphere(st.normal, seed);
+      cnt.c[1]++;
+      rec.occl++;
+      float t;
+      const int32_t p = trace_ray(st.origin, dir, t);
+      rec.ray(p, true);
+      if (p < 0) {
+        st.hit = false;
+      } else if (mat_of(p).emit) {
+        st.light_mat = s.mat[p];
+        st.hit = true;
+      }
+      if (st.hit) return ld3(s.mats[st.light_mat].emission) * bsdf_eval(st.normal, dir);
+    }
+    return mk(0.0f, 0.0f, 0.0f);
+  }
+
+  // One radiance trace: shader.cu:77-98 + closest-hit 211-255 / miss 189-194.
+  void radiance(RayState& st, V3 O, V3 D) {
+    cnt.c[0]++;
+    float t = 0.0f;
+    const int32_t p = trace_ray(O, D, t);
+    rec.ray(p, false);
+    if (rec.rad_hits && rec.bounces < rec.rb) rec.rad_hits[rec.bounces] = p;
+    rec.bounces++;
+    if (p < 0) {
+      st.color = st.color + mk(0.0f, 0.0f, 0.0f);
+      st.done = true;
+      return;
+    }
+    const orc_material& m = mat_of(p);
+    if (m.emit) {
+      st.color = st.color + ld3(m.emission) * st.att;
+      st.done = true;
+      return;
+    }
+    cnt.c[4]++;
+    st.origin = O + t * D;
+    st.direction = D;
+    {
+      const float* v = &s.v[9 * (size_t)p];
+      const float* n = &s.n[9 * (size_t)p];
+      st.normal = bary_normal(st.origin, ld3(n), ld3(n + 3), ld3(n + 6), ld3(v), ld3(v + 3), ld3(v + 6));
+    }
+    if (m.alpha < 1.0f) {
+      float cosI = dot(D, st.normal);
+      float eta;
+      V3 nn;
+      if (cosI < 0.0f) {
+        cosI = -cosI;
+        eta = 1.0f / m.n;
+        nn = st.normal;
+      } else {
+        st.att = st.att * ld3(m.diffuse);
+        eta = m.n;
+        nn = -st.normal;
+      }
+      if (eta == 1.0f) {
+        st.direction = D;
+      } else if (rnd(seed) <= fresnel(cosI, eta)) {
+        st.direction = reflect(D, nn);
+      } else {
+        st.direction = refract(cosI, D, nn, eta);
+      }
+    } else {
+      st.att = st.att * ld3(m.diffuse);
+      const V3 nee = shoot_ray_to_light(st);
+      st.color = st.color + nee * st.att;
+      st.direction = bsdf_bounce(st.direction, st.normal, seed, m.roughness);
+    }
+  }
+
+  // shader.cu:102-124.
+  V3 trace(V3 O, V3 D, int bounces) {
+    RayState st;
+    for (int i = 0; i < bounces; ++i) {
+      radiance(st, O, D);
+      D = st.direction;
+      O = st.origin;
+      if (st.done) break;
+    }
+    return st.color;
+  }
+};
+
+// shader.cu:126-167 for one pixel.
+void render_pixel(const Scene& s, const orc_render_params& p, uint32_t pix, float* out4,
+                  const float* prev4, float* srgb, int32_t* sb, int32_t* so, uint32_t* sh,
+                  int32_t* rh, Counters& cnt) {
+  const uint32_t x = pix % p.width, y = pix / p.width;
+  uint32_t seed = tea16(y * p.width + x, p.subframe);
+  const float sx = (float)p.width, sy = (float)p.height;
+  const float ix = (float)x, iy = (float)y;
+  const V3 eye = ld3(p.eye), U = ld3(p.U), V = ld3(p.V), W = ld3(p.W);
+  V3 accum = mk(0.0f, 0.0f, 0.0f);
+  for (uint32_t i = 0; i < p.spp; ++i) {
+    const float jx = rng(seed);
+    const float jy = rng(seed);
+    const float dx = (2.0f * ix + jx) / sx - 1.0f;
+    const float dy = (2.0f * iy + jy) / sy - 1.0f;
+    const V3 D = normalize(dx * U + dy * V + W);
+    SampleRec rec;
+    rec.rb = p.record_bounces;
+    if (rh) {
+      rec.rad_hits = rh + (size_t)i * p.record_bounces;
+      for (int b = 0; b < p.record_bounces; ++b) rec.rad_hits[b] = -3;
+    }
+    Tracer tr{s, cnt, seed, rec};
+    const V3 c = tr.trace(eye, D, (int)p.bounces);
+    cnt.c[5]++;
+    if (srgb) st3(srgb + 3 * (size_t)i, c);
+    if (sb) sb[i] = rec.bounces;
+    if (so) so[i] = rec.occl;
+    if (sh) sh[i] = rec.hash;
+    accum = accum + c;
+  }
+  const float inv = 1.0f / (float)p.spp;
+  accum = accum * inv;
+  if (p.subframe > 0) {
+    const float a = 1.0f / (float)(p.subframe + 1);
+    accum = lerp(ld3(prev4), accum, a);
+  }
+  st3(out4, accum);
+  out4[3] = 1.0f;
+  cnt.c[7]++;
+}
+
+}  // namespace
+
+struct orc_scene {
+  Scene s;
+};
+
+extern "C" {
+
+uint32_t orc_tea16(uint32_t v0, uint32_t v1) { return tea16(v0, v1); }
+uint32_t orc_lcg(uint32_t* state) { return lcg(*state); }
+float orc_rnd(uint32_t* state) { return rnd(*state); }
+float orc_rng(uint32_t* state) { return rng(*state); }
+void orc_hemisphere(const float* N, uint32_t* state, float* out3) { st3(out3, hemisphere(ld3(N), *state)); }
+float orc_fresnel(float cosT, float eta) { return fresnel(cosT, eta); }
+void orc_refract(float cosI, const float* d, const float* N, float eta, float* out3) {
+  st3(out3, refract(cosI, ld3(d), ld3(N), eta));
+}
+void orc_reflect(const float* i, const float* n, float* out3) { st3(out3, reflect(ld3(i), ld3(n))); }
+float orc_brdf(const float* N, const float* L) { return bsdf_eval(ld3(N), ld3(L)); }
+void orc_bounce(const float* d, const float* N, uint32_t* state, float roughness, float* out3) {
+  st3(out3, bsdf_bounce(ld3(d), ld3(N), *state, roughness));
+}
+void orc_bary_normal(const float* P, const float* n1, const float* n2, const float* n3,
+                     const float* v1, const float* v2, const float* v3, float* out3) {
+  st3(out3, bary_normal(ld3(P), ld3(n1), ld3(n2), ld3(n3), ld3(v1), ld3(v2), ld3(v3)));
+}
+// src/sutil/Camera.cpp:34-45 with up = (0,1,0) (optix_wrapper.cc:434-442).
+void orc_camera_uvw(const float* eye, const float* lookat, float fov, float aspect, float* U,
+                    float* V, float* W) {
+  const V3 e = ld3(eye), l = ld3(lookat), up = mk(0.0f, 1.0f, 0.0f);
+  const V3 w = l - e;
+  const float wlen = std::sqrt(dot(w, w));
+  V3 u = normalize(cross(w, up));
+  V3 v = normalize(cross(u, w));
+  const float vlen = wlen * std::tan(0.5f * fov * 3.14159265358979323846f / 180.0f);
+  v = v * vlen;
+  const float ulen = vlen * aspect;
+  u = u * ulen;
+  st3(U, u);
+  st3(V, v);
+  st3(W, w);
+}
+void orc_srgb_quantize(const float* rgba, int n_pixels, uint8_t* rgba8) {
+  for (int i = 0; i < n_pixels; ++i) {
+    for (int c = 0; c < 3; ++c) rgba8[4 * i + c] = quant8(to_srgb(clampf(rgba[4 * i + c], 0.0f, 1.0f)));
+    rgba8[4 * i + 3] = 255;
+  }
+}
+// src/sutil/sutil.cpp:97-117 (P6 header) and :541-553 (row flip).
+int orc_save_ppm(const char* path, const uint8_t* rgba8, int width, int height) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return -1;
+  std::fprintf(f, "P6\n%d %d\n255\n", width, height);
+  std::vector<uint8_t> row(3 * (size_t)width);
+  for (int j = height - 1; j >= 0; --j) {
+    for (int i = 0; i < width; ++i)
+      for (int c = 0; c < 3; ++c) row[3 * i + c] = rgba8[4 * ((size_t)j * width + i) + c];
+    std::fwrite(row.data(), 1, row.size(), f);
+  }
+  std::fclose(f);
+  return 0;
+}
+uint32_t orc_morton3(float x, float y, float z) { return morton3(x, y, z); }
+
+orc_scene* orc_scene_create(const float* verts, const float* normals, const int32_t* mat_idx,
+                            int32_t T, const orc_material* mats, int32_t M) {
+  orc_scene* o = new orc_scene;
+  Scene& s = o->s;
+  s.T = T;
+  s.v.assign(verts, verts + 9 * (size_t)T);
+  s.n.assign(normals, normals + 9 * (size_t)T);
+  s.mat.assign(mat_idx, mat_idx + T);
+  s.mats.assign(mats, mats + M);
+  build_bvh(s);
+  return o;
+}
+void orc_scene_destroy(orc_scene* s) { delete s; }
+int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
+
+int orc_get_bvh(const orc_scene* o, uint32_t* morton_sorted, uint32_t* prim_sorted,
+                uint32_t* child_refs, float* child_boxes) {
+  const Scene& s = o->s;
+  if (s.T == 0) return 0;
+  std::copy(s.morton.begin(), s.morton.end(), morton_sorted);
+  std::copy(s.prim.begin(), s.prim.end(), prim_sorted);
+  std::copy(s.refs.begin(), s.refs.end(), child_refs);
+  std::copy(s.boxes.begin(), s.boxes.end(), child_boxes);
+  return 0;
+}
+
+void orc_closest_hit(const orc_scene* o, const float* O, const float* D, int32_t n, float tmin,
+                     float tmax, int32_t* prim_out, float* t_out, uint64_t* counters, int threads) {
+  const Scene& s = o->s;
+  Counters total;
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+  {
+    Counters cnt;
+#pragma omp for schedule(dynamic, 256)
+    for (int32_t i = 0; i < n; ++i) {
+      const Ray r = make_ray(ld3(O + 3 * (size_t)i), ld3(D + 3 * (size_t)i));
+      float t = 0.0f;
+      const int32_t p = closest(s, r, tmin, tmax, t, cnt);
+      prim_out[i] = p;
+      t_out[i] = p >= 0 ? t : 0.0f;
+    }
+#pragma omp critical
+    for (int k = 0; k < ORC_NCOUNTERS; ++k) total.c[k] += cnt.c[k];
+  }
+  if (counters)
+    for (int k = 0; k < ORC_NCOUNTERS; ++k) counters[k] = total.c[k];
+}
+
+void orc_closest_hit_brute(const orc_scene* o, const float* O, const float* D, int32_t n,
+                           float tmin, float tmax, int32_t* prim_out, float* t_out) {
+  const Scene& s = o->s;
+  for (int32_t i = 0; i < n; ++i) {
+    const Ray r = make_ray(ld3(O + 3 * (size_t)i), ld3(D + 3 * (size_t)i));
+    float t = 0.0f;
+    const int32_t p = closest_brute(s, r, tmin, tmax, t);
+    prim_out[i] = p;
+    t_out[i] = p >= 0 ? t : 0.0f;
+  }
+}
+
+int orc_render(const orc_scene* o, const orc_render_params* p, const int32_t* pixel_ids,
+               int32_t n_pixels, float* accum_out, const float* accum_prev, float* sample_rgb,
+               int32_t* sample_bounces, int32_t* sample_occl, uint32_t* sample_hash,
+               int32_t* rad_hits, uint64_t* counters, int threads) {
+  const Scene& s = o->s;
+  if (p->width == 0 || p->height == 0 || p->spp == 0) return -1;
+  if (p->subframe > 0 && !accum_prev) return -3;
+  const int64_t npx = (int64_t)p->width * p->height;
+  const int64_t n = pixel_ids ? n_pixels : npx;
+  for (int64_t i = 0; i < n; ++i)
+    if (pixel_ids && (pixel_ids[i] < 0 || pixel_ids[i] >= npx)) return -2;
+  const size_t spp = p->spp, rb = (size_t)std::max(p->record_bounces, 0);
+  Counters total;
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+  {
+    Counters cnt;
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t i = 0; i < n; ++i) {
+      const uint32_t pix = pixel_ids ? (uint32_t)pixel_ids[i] : (uint32_t)i;
+      render_pixel(s, *p, pix, accum_out + 4 * i, accum_prev ? accum_prev + 4 * i : nullptr,
+                   sample_rgb ? sample_rgb + 3 * spp * i : nullptr,
+                   sample_bounces ? sample_bounces + spp * i : nullptr,
+                   sample_occl ? sample_occl + spp * i : nullptr,
+                   sample_hash ? sample_hash + spp * i : nullptr,
+                   (rad_hits && rb) ? rad_hits + spp * rb * i : nullptr, cnt);
+    }
+#pragma omp critical
+    for (int k = 0; k < ORC_NCOUNTERS; ++k) total.c[k] += cnt.c[k];
+  }
+  if (counters)
+    for (int k = 0; k < ORC_NCOUNTERS; ++k) counters[k] = total.c[k];
+  return 0;
+}
+
+}  // extern "C"

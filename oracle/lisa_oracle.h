@@ -1,0 +1,113 @@
+/*
+ * CPU oracle for the LiSA path-tracing hot path.  TEST INFRASTRUCTURE ONLY:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker; the product (liblisa.so)
+ * never links or calls it.
+ *
+ * A scalar C++ restatement of
+ *   src/LiSA/src/shader.cu            (raygen / closest-hit / miss path loop, NEE)
+ *   src/LiSA/src/maths.cu             (rng remap, hemisphere, fresnel, refract, bary normal)
+ *   src/LiSA/src/bsdfs/lambertian.cu  (bounce / BRDF / BTDF)
+ *   src/cuda/random.h                 (tea<16>, lcg, rnd)
+ *   src/sutil/vec_math.h              (float3 algebra, exact op order)
+ *   src/cuda/helpers.h                (sRGB + 8-bit quantisation)
+ *   src/sutil/Camera.cpp:34-45        (UVW frame)
+ *   src/sutil/sutil.cpp:97-117,523-600 (PPM writer, row flip)
+ * plus the accelerator the reference gets from OptiX (closed): an LBVH
+ * (30-bit Morton codes, stable sort, Karras 2012 hierarchy, refit) and a
+ * stack traversal with the Woop/Benthin/Wald (JCGT 2013) watertight test.
+ *
+ * Decisions where the reference is unspecified (shared with the HIP kernels):
+ *  - occlusion rays report the CLOSEST hit (ties -> lowest triangle id), not
+ *    OptiX's first-found hit (shader.cu:69);
+ *  - a ray whose direction is zero (total internal reflection, maths.cu:29)
+ *    or not finite counts as a ray but misses without traversal;
+ *  - make_float3(rng,rng,rng) / make_float2(rng,rng) draw x, then y, then z;
+ *  - pow() in fresnel is r*r and ((x*x)*(x*x))*x;
+ *  - IEEE FP32 everywhere (no FMA contraction, correctly rounded / and sqrt),
+ *    the reference's --use_fast_math rounding is not reproduced.
+ * Parity status of every function is listed in the header of each test file.
+ */
+#ifndef LISA_ORACLE_H
+#define LISA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same byte layout as lisa_material in include/lisa.h (structs.hh:16-23). */
+typedef struct orc_material {
+  float roughness;
+  float alpha;
+  float n;
+  float diffuse[3];
+  int32_t emit;
+  float emission[3];
+} orc_material;
+
+typedef struct orc_render_params {
+  uint32_t width, height, spp, bounces, subframe;
+  float eye[3], U[3], V[3], W[3];
+  int32_t record_bounces; /* max bounces stored per sample in rad_hits (0 = none) */
+} orc_render_params;
+
+/* counters[0..7]: radiance rays, occlusion rays, node visits, triangle tests,
+ * radiance hits (shading gathers), samples, zero/invalid-direction rays, pixels */
+enum { ORC_NCOUNTERS = 8 };
+
+typedef struct orc_scene orc_scene;
+
+/* --- random.h / maths.cu / lambertian.cu restatements (exposed for KATs) --- */
+uint32_t orc_tea16(uint32_t v0, uint32_t v1);
+uint32_t orc_lcg(uint32_t* state);
+float orc_rnd(uint32_t* state);
+float orc_rng(uint32_t* state);
+void orc_hemisphere(const float* N, uint32_t* state, float* out3);
+float orc_fresnel(float cosT, float eta);
+void orc_refract(float cosI, const float* d, const float* N, float eta, float* out3);
+void orc_reflect(const float* i, const float* n, float* out3);
+float orc_brdf(const float* N, const float* L);
+void orc_bounce(const float* d, const float* N, uint32_t* state, float roughness, float* out3);
+void orc_bary_normal(const float* P, const float* n1, const float* n2, const float* n3,
+                     const float* v1, const float* v2, const float* v3, float* out3);
+void orc_camera_uvw(const float* eye, const float* lookat, float fov, float aspect,
+                    float* U, float* V, float* W);
+void orc_srgb_quantize(const float* rgba, int n_pixels, uint8_t* rgba8);
+int orc_save_ppm(const char* path, const uint8_t* rgba8, int width, int height);
+uint32_t orc_morton3(float x, float y, float z); /* x,y,z in [0,1] */
+
+/* --- scene / LBVH --- */
+orc_scene* orc_scene_create(const float* verts /*9T*/, const float* normals /*9T*/,
+                            const int32_t* mat_idx /*T*/, int32_t T,
+                            const orc_material* mats, int32_t M);
+void orc_scene_destroy(orc_scene* s);
+int32_t orc_num_tris(const orc_scene* s);
+/* morton_sorted[T], prim_sorted[T], child refs [2*max(T-1,1)],
+ * child boxes [12*max(T-1,1)] as (c0.lo.xyz, c0.hi.xyz, c1.lo.xyz, c1.hi.xyz) */
+int orc_get_bvh(const orc_scene* s, uint32_t* morton_sorted, uint32_t* prim_sorted,
+                uint32_t* child_refs, float* child_boxes);
+
+/* Closest hit of n rays; prim_out = -1 on miss; counters optional (ORC_NCOUNTERS). */
+void orc_closest_hit(const orc_scene* s, const float* O, const float* D, int32_t n,
+                     float tmin, float tmax, int32_t* prim_out, float* t_out,
+                     uint64_t* counters, int threads);
+void orc_closest_hit_brute(const orc_scene* s, const float* O, const float* D, int32_t n,
+                           float tmin, float tmax, int32_t* prim_out, float* t_out);
+
+/* Render pixels (all if pixel_ids == NULL, else n_pixels listed ids).
+ * accum_out: float4 per listed pixel (rgb, 1).  accum_prev (same layout) is read
+ * when subframe > 0.  Per-sample records are optional (NULL to skip):
+ *   sample_rgb[n*spp*3], sample_bounces[n*spp], sample_occl[n*spp],
+ *   sample_hash[n*spp], rad_hits[n*spp*record_bounces]. */
+int orc_render(const orc_scene* s, const orc_render_params* p,
+               const int32_t* pixel_ids, int32_t n_pixels,
+               float* accum_out, const float* accum_prev,
+               float* sample_rgb, int32_t* sample_bounces, int32_t* sample_occl,
+               uint32_t* sample_hash, int32_t* rad_hits,
+               uint64_t* counters, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
