@@ -1,0 +1,140 @@
+"""Throughput of the plain path-tracing megakernel on one MI355X.
+
+A step is one full render (subframe 0, every sample in one launch, as
+render.cc:133-148) of the 65k-triangle Cornell box at 1024x1024, 256 spp,
+7 bounces, with the scene and LBVH already resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--spp S] [--no-cpu]
+
+Prints one JSON line.  `value` is rays traced (radiance + occlusion, exact
+counts from the counting kernel, which is bit-identical in work to the
+timed one) per second of wall time over the K timed steps.  `roofline` uses
+the average kernel duration from HIP events recorded on the context's own
+stream; `cpu_baseline` times the CPU oracle on a seeded pixel subset of the
+same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s; time to RMSE≤1e-2 vs 16k-spp ref, cache vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes: one 64-byte node line per BVH node visit, one 48-byte
+# triangle record per triangle test, 96 bytes (vertices + normals) per
+# shaded radiance hit, one float4 accum store per pixel.
+S_NODE, S_TRI, S_SHADE, S_PIXEL = 64, 48, 96, 16
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-pixels", type=int, default=65536)
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
+    recorded in profiles/k_path_cornell65k_pmc.json, or None."""
+    path = os.path.join(ROOT, "profiles", "k_path_cornell65k_pmc.json")
+    try:
+        with open(path) as fh:
+            return float(json.load(fh)["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the multi-GPU design (pixel-tile sharding) is described in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+    from lisa_amd import capi, scenes
+
+    sc = scenes.cornell_65k()
+    W, H, B = sc["width"], sc["height"], sc["bounces"]
+    spp = a.spp
+    cam = sc["camera"]
+    U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+    ctx = capi.Context(0)
+    ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    bvh_ms = ctx.stats()["bvh_build_ms"]
+    print(f"[bench] scene {sc['name']}: {ctx.T} triangles, LBVH build {bvh_ms:.2f} ms",
+          file=sys.stderr, flush=True)
+
+    # exact work per step (counting instantiation of the same kernel)
+    p = capi.render_params(W, H, spp, B, cam["eye"], U, V, Wv, flags=capi.FLAG_COUNT)
+    ctx.render(p, want_accum=False)
+    cnt = [int(x) for x in ctx.counters()]
+    counts = dict(zip(capi.COUNTER_NAMES, cnt))
+    rays = counts["rays_radiance"] + counts["rays_occlusion"]
+    print(f"[bench] counted step: {counts}", file=sys.stderr, flush=True)
+
+    p.flags = 0
+    for _ in range(a.warmup):
+        ctx.render(p, want_accum=False)
+    ctx.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ctx.render(p, want_accum=False)
+        kernel_ms.append(ctx.stats()["render_ms"])
+        print(f"[bench] step {i}: kernel {kernel_ms[-1]:.1f} ms", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / a.steps
+    value = rays * a.steps / wall / 1e6
+    avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
+
+    alg_bytes = (S_NODE * counts["node_visits"] + S_TRI * counts["tri_tests"]
+                 + S_SHADE * counts["radiance_hits"] + S_PIXEL * W * H)
+    achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profile(),
+                "kernel": "k_path<false,false>", "alg_bytes_per_launch": alg_bytes,
+                "avg_kernel_ms": round(avg_kernel_ms, 3)}
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle
+        threads = len(os.sched_getaffinity(0))
+        o = oracle.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        px = scenes.seeded_pixels(W, H, a.cpu_pixels, seed=7)
+        t = time.perf_counter()
+        r = o.render(W, H, spp, B, cam["eye"], U, V, Wv, pixels=px, threads=threads)
+        dt = time.perf_counter() - t
+        crays = int(r["counters"][0] + r["counters"][1])
+        cpu = {"value": round(crays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{a.cpu_pixels} seeded pixels of the same 1024x1024 render at "
+                         f"{spp} spp x {B} bounces ({crays} rays, {dt:.1f} s, OpenMP over pixels)"}
+        print(f"[bench] cpu oracle: {cpu}", file=sys.stderr, flush=True)
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "cornell_65k plain megakernel", "width": W, "height": H,
+                      "spp": spp, "bounces": B, "triangles": ctx.T, "parallelism": "single GPU"},
+           "samples_per_s": round(W * H * spp * a.steps / wall, 1),
+           "rays_per_step": rays, "bvh_build_ms": round(bvh_ms, 3),
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

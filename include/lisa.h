@@ -73,6 +73,7 @@ typedef struct lisa_stats {
   float bvh_build_ms; /* last lisa_set_scene device build time */
   int32_t num_tris;
   int32_t num_nodes;  /* internal BVH nodes (64 B each) */
+  int32_t num_emitters; /* triangles whose material emits (the NEE light BVH) */
   uint64_t device_bytes; /* device memory held by the context */
 } lisa_stats;
 
@@ -117,11 +118,12 @@ void* lisa_get_stream(const lisa_ctx* ctx);
 
 /* ---- debug / parity hooks (no counterpart in the reference: OptiX hides these) ---- */
 
-/* LBVH as built: Morton codes and triangle ids in sorted order (T each),
- * child refs (2 per internal node; bit 31 = leaf, low bits = sorted index;
+/* An LBVH as built (which = 0: all T triangles; 1: the N emitting ones):
+ * Morton codes and global triangle ids in sorted order (N each), child refs
+ * (2 per internal node; bit 31 = leaf, low bits = sorted index;
  * 0xFFFFFFFF = empty) and child boxes (12 floats per internal node:
- * c0.lo.xyz c0.hi.xyz c1.lo.xyz c1.hi.xyz).  Internal nodes = max(T-1, 1). */
-lisa_status lisa_get_bvh(lisa_ctx* ctx, uint32_t* morton, uint32_t* prim,
+ * c0.lo.xyz c0.hi.xyz c1.lo.xyz c1.hi.xyz).  Internal nodes = max(N-1, 1). */
+lisa_status lisa_get_bvh(lisa_ctx* ctx, int32_t which, uint32_t* morton, uint32_t* prim,
                          uint32_t* child_refs, float* child_boxes);
 
 /* Closest hit of n rays (O, D: 3n floats): prim_out = triangle id or -1,
@@ -140,6 +142,12 @@ lisa_status lisa_trace_subset(lisa_ctx* ctx, const lisa_render_params* p,
                               float* sample_rgb, int32_t* sample_bounces,
                               int32_t* sample_occl, uint32_t* sample_hash,
                               int32_t* rad_hits, int32_t record_bounces);
+
+/* Streams `bytes` through a plain device copy kernel (read + write, float4,
+ * grid-stride) `iters` times and returns the average kernel time; the
+ * known-size reference for HBM counters (FETCH_SIZE / WRITE_SIZE) and the
+ * achievable-bandwidth figure.  No counterpart in the reference. */
+lisa_status lisa_copy_probe(lisa_ctx* ctx, int64_t bytes, int32_t iters, float* avg_ms);
 
 /* ---- host-side pieces of the drop-in (no device needed) ---- */
 

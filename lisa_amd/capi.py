@@ -35,7 +35,7 @@ COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
 EXPORTS = (
     "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_render",
     "lisa_copy_accum", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
-    "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset",
+    "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset", "lisa_copy_probe",
     "lisa_camera_uvw", "lisa_srgb_quantize", "lisa_save_ppm", "lisa_scene_file_load",
     "lisa_scene_file_free", "lisa_scene_file_info", "lisa_scene_file_vertices",
     "lisa_scene_file_normals", "lisa_scene_file_mat_indices", "lisa_scene_file_materials",
@@ -63,7 +63,8 @@ class RenderParams(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("render_ms", C.c_float), ("bvh_build_ms", C.c_float), ("num_tris", C.c_int32),
-                ("num_nodes", C.c_int32), ("device_bytes", C.c_uint64)]
+                ("num_nodes", C.c_int32), ("num_emitters", C.c_int32),
+                ("device_bytes", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):
@@ -98,10 +99,11 @@ def lib():
         "lisa_get_stats": (st, [vp, C.POINTER(Stats)]),
         "lisa_get_counters": (st, [vp, u64p]),
         "lisa_get_stream": (vp, [vp]),
-        "lisa_get_bvh": (st, [vp, u32p, u32p, u32p, f32p]),
+        "lisa_get_bvh": (st, [vp, C.c_int32, u32p, u32p, u32p, f32p]),
         "lisa_trace_rays": (st, [vp, f32p, f32p, C.c_int32, C.c_float, C.c_float, i32p, f32p]),
         "lisa_trace_subset": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, f32p, f32p,
                                    i32p, i32p, u32p, i32p, C.c_int32]),
+        "lisa_copy_probe": (st, [vp, C.c_int64, C.c_int32, f32p]),
         "lisa_camera_uvw": (st, [f32p, f32p, C.c_float, C.c_float, f32p, f32p, f32p]),
         "lisa_srgb_quantize": (st, [f32p, C.c_int64, u8p]),
         "lisa_save_ppm": (st, [C.c_char_p, u8p, C.c_int32, C.c_int32]),
@@ -271,24 +273,29 @@ class Context:
         self._ck(lib().lisa_get_stats(self.h, C.byref(s)))
         return {"render_ms": s.render_ms, "bvh_build_ms": s.bvh_build_ms,
                 "num_tris": s.num_tris, "num_nodes": s.num_nodes,
-                "device_bytes": s.device_bytes}
+                "num_emitters": s.num_emitters, "device_bytes": s.device_bytes}
 
     def counters(self):
         c = np.zeros(NCOUNTERS, np.uint64)
         self._ck(lib().lisa_get_counters(self.h, _p(c, C.c_uint64)))
         return c
 
+    def copy_probe(self, nbytes, iters=10):
+        ms = C.c_float()
+        self._ck(lib().lisa_copy_probe(self.h, int(nbytes), int(iters), C.byref(ms)))
+        return ms.value
+
     def stream(self):
         return lib().lisa_get_stream(self.h)
 
-    def bvh(self):
-        T = self.T
+    def bvh(self, which=0):
+        T = self.T if which == 0 else self.stats()["num_emitters"]
         nI = max(T - 1, 1)
         m = np.zeros(max(T, 1), np.uint32)
         p = np.zeros(max(T, 1), np.uint32)
         r = np.zeros(2 * nI, np.uint32)
         b = np.zeros(12 * nI, np.float32)
-        self._ck(lib().lisa_get_bvh(self.h, _p(m, C.c_uint32), _p(p, C.c_uint32),
+        self._ck(lib().lisa_get_bvh(self.h, which, _p(m, C.c_uint32), _p(p, C.c_uint32),
                                     _p(r, C.c_uint32), _p(b, C.c_float)))
         return {"morton": m[:T], "prim": p[:T], "refs": r.reshape(nI, 2),
                 "boxes": b.reshape(nI, 12)}

@@ -32,6 +32,11 @@ constexpr uint32_t kLeaf = 0x80000000u;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT 2013).
 constexpr float kRobust = 0x1.000006p+0f;
+// Boxes are culled against the best hit distance widened by 2^-16, so a
+// triangle whose hit lies on its box boundary (axis-aligned or coplanar
+// geometry, where the slab entry can round a few ulp above the triangle's t)
+// is still tested and equal-t hits resolve to the lower triangle id.
+constexpr float kWiden = 0x1.0001p+0f;
 constexpr float kTmin = 1e-4f;  // shader.cu:114, :202
 constexpr float kTmax = 1e16f;  // shader.cu:115, :202
 constexpr int kNeeTries = 30;   // shader.cu:199
@@ -50,12 +55,18 @@ struct DevMat {
   float4 rough_n;        // roughness, n, -, -
 };
 
-struct SceneView {
+// One device LBVH: 64-byte nodes + 48-byte Morton-ordered triangle records.
+struct BvhView {
   const float4* __restrict__ nodes;
   const float4* __restrict__ tris;
-  const float4* __restrict__ nrm;
-  const DevMat* __restrict__ mats;
   int32_t T;
+};
+
+struct SceneView {
+  BvhView all;   // every triangle
+  BvhView emit;  // emitting triangles only (NEE light queries)
+  const float4* __restrict__ nrm;  // vertex normals, Morton order of `all`
+  const DevMat* __restrict__ mats;
 };
 
 // ---------------------------------------------------------------------------
@@ -147,30 +158,36 @@ struct TravCount {
   uint64_t visits = 0, tests = 0;
 };
 
-// Returns the sorted index of the closest triangle (-1 on miss); t_out its t.
+// Closest hit (any = false) or any hit (any = true) over `bv`: returns the
+// sorted index of the accepted triangle (-1 when none); t_out its t.  A
+// triangle is accepted when t > tmin and (t < best_t or (t == best_t and
+// id < best)), starting from best_t = tlim, best = lim_prim: the closest hit
+// resolves equal t to the lower triangle id; the any-hit query answers "is
+// any triangle strictly before (tlim, lim_prim)" and stops at the first.
 // `stk` points at this lane's first LDS stack slot; consecutive entries are
 // kStride words apart, so a wave's 64 lanes hit 64 distinct banks.
 template <bool kCount, int kStride>
-__device__ __forceinline__ int32_t closest(const SceneView& sc, const RayT& r, float tmin,
-                                           float tmax, float& t_out, uint32_t* stk,
-                                           TravCount& cnt) {
+__device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, float tmin,
+                                           float tlim, uint32_t lim_prim, bool any, float& t_out,
+                                           uint32_t* stk, TravCount& cnt) {
   uint32_t spill[kStackMax - kStackLds];
-  float best_t = tmax;
-  uint32_t best_prim = 0xFFFFFFFFu;
+  float best_t = tlim;
+  float cull_t = tlim * kWiden;
+  uint32_t best_prim = lim_prim;
   int32_t best_k = -1;
   int sp = 0;
   uint32_t node = 0;
   // A tree visits each internal node at most once; the bound only guards
   // against a corrupt hierarchy (it never triggers on a valid one).
-  const uint32_t max_visits = (uint32_t)sc.T + 1u;
+  const uint32_t max_visits = (uint32_t)bv.T + 1u;
   for (uint32_t it = 0; it < max_visits; ++it) {
     if (kCount) cnt.visits++;
-    const float4* np = sc.nodes + 4 * (size_t)node;
+    const float4* np = bv.nodes + 4 * (size_t)node;
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
     const uint32_t r0 = __float_as_uint(d.x), r1 = __float_as_uint(d.y);
     float tn0 = 0.0f, tn1 = 0.0f;
-    bool h0 = (r0 != kEmpty) && slab(a.x, a.y, a.z, a.w, b.x, b.y, r, tmin, best_t, tn0);
-    bool h1 = (r1 != kEmpty) && slab(b.z, b.w, c.x, c.y, c.z, c.w, r, tmin, best_t, tn1);
+    bool h0 = (r0 != kEmpty) && slab(a.x, a.y, a.z, a.w, b.x, b.y, r, tmin, cull_t, tn0);
+    bool h1 = (r1 != kEmpty) && slab(b.z, b.w, c.x, c.y, c.z, c.w, r, tmin, cull_t, tn1);
 #pragma unroll
     for (int ch = 0; ch < 2; ++ch) {
       const bool hit = ch == 0 ? h0 : h1;
@@ -178,13 +195,14 @@ __device__ __forceinline__ int32_t closest(const SceneView& sc, const RayT& r, f
       if (hit && (ref & kLeaf)) {
         if (kCount) cnt.tests++;
         const uint32_t k = ref & ~kLeaf;
-        const float4* tp = sc.tris + 3 * (size_t)k;
+        const float4* tp = bv.tris + 3 * (size_t)k;
         const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
         const uint32_t prim = __float_as_uint(t0.w);
         float t;
         if (watertight(r, xyz(t0), xyz(t1), xyz(t2), t) && t > tmin &&
             (t < best_t || (t == best_t && prim < best_prim))) {
           best_t = t;
+          cull_t = t * kWiden;
           best_prim = prim;
           best_k = (int32_t)k;
         }
@@ -194,6 +212,7 @@ __device__ __forceinline__ int32_t closest(const SceneView& sc, const RayT& r, f
           h1 = false;
       }
     }
+    if (any && best_k >= 0) break;
     if (h0 && h1) {
       uint32_t nr = r0, fr = r1;
       if (tn1 < tn0) {
@@ -239,10 +258,11 @@ __global__ __launch_bounds__(kRayBlock) void k_trace_rays(SceneView sc, const fl
   TravCount cnt;
   int32_t prim = -1;
   float t = 0.0f;
-  if (sc.T > 0 && ray_valid(d)) {
+  if (sc.all.T > 0 && ray_valid(d)) {
     const RayT r = make_ray(o, d);
-    const int32_t k = closest<kCount, kRayBlock>(sc, r, tmin, tmax, t, stk + threadIdx.x, cnt);
-    if (k >= 0) prim = (int32_t)__float_as_uint(sc.tris[3 * (size_t)k].w);
+    const int32_t k = closest<kCount, kRayBlock>(sc.all, r, tmin, tmax, 0xFFFFFFFFu, false, t,
+                                                 stk + threadIdx.x, cnt);
+    if (k >= 0) prim = (int32_t)__float_as_uint(sc.all.tris[3 * (size_t)k].w);
   }
   prim_out[i] = prim;
   t_out[i] = prim >= 0 ? t : 0.0f;
@@ -310,8 +330,11 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
   // Per-lane path state (RayState, shader.cu:30-42, plus the NEE loop index).
   F3 O, D, color, att, N = mk(0.0f, 0.0f, 0.0f), P = mk(0.0f, 0.0f, 0.0f);
   F3 Din = mk(0.0f, 0.0f, 0.0f);
-  bool occl = false, hitf = false, need_new = true;
-  int32_t light_mat = 0, tries = 0;
+  bool hitf = false, need_new = true;
+  int32_t qstage = 0;  // 0 radiance, 1 NEE emitter query, 2 NEE any-hit query
+  int32_t light_mat = 0, tries = 0, matE = -1;
+  uint32_t primE = 0xFFFFFFFFu;
+  float tE = kTmax;
   float rough = 0.0f;
   uint32_t b = 0, s = 0;
   uint32_t hash = 0;
@@ -330,7 +353,7 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
       color = mk(0.0f, 0.0f, 0.0f);
       att = mk(1.0f, 1.0f, 1.0f);
       hitf = false;
-      occl = false;
+      qstage = 0;
       b = 0;
       need_new = false;
       if (kRecord) {
@@ -362,31 +385,46 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
       }
     }
 
-    // ---- trace one ray (radiance or occlusion) ----
+    // ---- one BVH query per iteration ----
+    // qstage 0: radiance ray, closest hit over all triangles.
+    // qstage 1: NEE occlusion ray, closest emitter (emitter BVH).
+    // qstage 2: the same occlusion ray, any triangle before (tE, primE) --
+    //           or before tmax when no emitter was hit and the hit flag is stale.
+    // A miss and a non-emitting closest hit both leave a false hit flag false,
+    // so with the flag false an occlusion ray that hits no emitter is
+    // resolved after qstage 1 (shader.cu:172-184, 204-206).
     int32_t k = -1;
     float t = 0.0f;
     const bool valid = ray_valid(D);
-    if (valid && sc.T > 0) {
-      const RayT r = make_ray(O, D);
-      k = closest<kCount, kPathBlock>(sc, r, kTmin, kTmax, t, my_stk, tc);
+    {
+      const BvhView& bv = qstage == 1 ? sc.emit : sc.all;
+      if (valid && bv.T > 0) {
+        const RayT r = make_ray(O, D);
+        const bool any = qstage == 2;
+        k = closest<kCount, kPathBlock>(bv, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
+                                        any, t, my_stk, tc);
+      }
     }
-    if (kCount && !valid) c_inv++;
-    float4 t0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), t1 = t0;
-    int32_t prim = valid ? -1 : -2;
-    int32_t mat = -1;
-    if (k >= 0) {
-      t0 = sc.tris[3 * (size_t)k];
-      t1 = sc.tris[3 * (size_t)k + 1];
-      prim = (int32_t)__float_as_uint(t0.w);
-      mat = (int32_t)__float_as_uint(t1.w);
-    }
-    if (kRecord) hash = (hash ^ ((uint32_t)prim ^ (occl ? 0x5bd1e995u : 0u))) * 16777619u;
 
     bool finish = false;
-    if (!occl) {
+    if (qstage == 0) {
       // radiance: miss (189-194) / closest hit (211-255)
-      if (kCount) c_rad++;
-      if (kRecord && p.rad_hits && (int32_t)b < p.rb) p.rad_hits[(rec_base + s) * p.rb + b] = prim;
+      if (kCount) {
+        c_rad++;
+        if (!valid) c_inv++;
+      }
+      float4 t0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), t1 = t0;
+      int32_t prim = valid ? -1 : -2, mat = -1;
+      if (k >= 0) {
+        t0 = sc.all.tris[3 * (size_t)k];
+        t1 = sc.all.tris[3 * (size_t)k + 1];
+        prim = (int32_t)__float_as_uint(t0.w);
+        mat = (int32_t)__float_as_uint(t1.w);
+      }
+      if (kRecord) {
+        hash = (hash ^ (uint32_t)prim) * 16777619u;
+        if (p.rad_hits && (int32_t)b < p.rb) p.rad_hits[(rec_base + s) * p.rb + b] = prim;
+      }
       ++b;
       if (k < 0) {
         color = color + mk(0.0f, 0.0f, 0.0f);
@@ -401,7 +439,7 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
           P = O + t * D;
           Din = D;
           {
-            const float4 t2 = sc.tris[3 * (size_t)k + 2];
+            const float4 t2 = sc.all.tris[3 * (size_t)k + 2];
             const float4* nq = sc.nrm + 3 * (size_t)k;
             N = bary_normal(P, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
           }
@@ -412,7 +450,7 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
           } else {
             att = att * xyz(m.diffuse_alpha);
             rough = m.rough_n.x;
-            occl = true;
+            qstage = 1;
             tries = 0;
             O = P;
             D = hemisphere(N, seed);
@@ -421,30 +459,71 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
       }
     } else {
       // occlusion: miss (172-175) / closest hit (177-184), then the NEE loop (200-207)
-      if (kCount) c_occ++;
-      if (kRecord) occl_cnt++;
-      if (k < 0) {
-        hitf = false;
-      } else if (sc.mats[mat].emission_emit.w != 0.0f) {
-        light_mat = mat;
-        hitf = true;
+      bool resolved = false;
+      int32_t code = 0;
+      if (qstage == 1) {
+        if (kCount) {
+          c_occ++;
+          if (!valid) c_inv++;
+        }
+        if (kRecord) occl_cnt++;
+        if (!valid) {  // counts as a ray, misses without traversal
+          code = hitf ? -1 : -5;
+          hitf = false;
+          resolved = true;
+        } else if (k >= 0) {
+          const float4 e0 = sc.emit.tris[3 * (size_t)k], e1 = sc.emit.tris[3 * (size_t)k + 1];
+          tE = t;
+          primE = __float_as_uint(e0.w);
+          matE = (int32_t)__float_as_uint(e1.w);
+          qstage = 2;
+        } else if (!hitf) {
+          code = -5;
+          resolved = true;
+        } else {
+          tE = kTmax;
+          primE = 0xFFFFFFFFu;
+          matE = -1;
+          qstage = 2;
+        }
+      } else {  // qstage 2
+        const bool blocked = k >= 0;
+        if (matE >= 0) {
+          if (!blocked) {
+            light_mat = matE;
+            hitf = true;
+            code = (int32_t)primE;
+          } else {
+            code = hitf ? -4 : -5;
+          }
+        } else if (blocked) {
+          code = -4;
+        } else {
+          code = -1;
+          hitf = false;
+        }
+        resolved = true;
       }
-      bool end_nee = false;
-      F3 nee = mk(0.0f, 0.0f, 0.0f);
-      if (hitf) {
-        nee = xyz(sc.mats[light_mat].emission_emit) * Lambertian::eval(N, D);
-        end_nee = true;
-      } else if (++tries == kNeeTries) {
-        end_nee = true;
-      } else {
-        D = hemisphere(N, seed);
-      }
-      if (end_nee) {
-        color = color + nee * att;
-        D = Lambertian::sample(Din, N, seed, rough);
-        O = P;
-        occl = false;
-        if (b == p.bounces) finish = true;
+      if (resolved) {
+        if (kRecord) hash = (hash ^ ((uint32_t)code ^ 0x5bd1e995u)) * 16777619u;
+        bool end_nee = false;
+        F3 nee = mk(0.0f, 0.0f, 0.0f);
+        if (hitf) {
+          nee = xyz(sc.mats[light_mat].emission_emit) * Lambertian::eval(N, D);
+          end_nee = true;
+        } else if (++tries == kNeeTries) {
+          end_nee = true;
+        } else {
+          D = hemisphere(N, seed);
+          qstage = 1;
+        }
+        if (end_nee) {
+          color = color + nee * att;
+          D = Lambertian::sample(Din, N, seed, rough);
+          O = P;
+          qstage = 0;
+          if (b == p.bounces) finish = true;
+        }
       }
     }
 
@@ -504,19 +583,26 @@ constexpr int kBuildBlock = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kBuildBlock * kSortItems;  // 4096 keys per tile
 
-__device__ __forceinline__ float centroid(const float* __restrict__ v, size_t k, int a) {
-  const float* t = v + 9 * k;
+// Triangle k of a build: global triangle gid[k] (k itself when gid is null).
+__device__ __forceinline__ uint32_t global_id(const uint32_t* __restrict__ gid, uint32_t k) {
+  return gid ? gid[k] : k;
+}
+
+__device__ __forceinline__ float centroid(const float* __restrict__ v, size_t g, int a) {
+  const float* t = v + 9 * g;
   return (t[a] + t[3 + a] + t[6 + a]) / 3.0f;
 }
 
 // Per-block min/max of triangle centroids (6 floats per block).
 __global__ __launch_bounds__(kBuildBlock) void k_centroid_partial(const float* __restrict__ v,
+                                                                  const uint32_t* __restrict__ gid,
                                                                   int32_t T, float* partial) {
   __shared__ float s[6][kBuildBlock];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int32_t k = blockIdx.x * kBuildBlock + threadIdx.x; k < T; k += gridDim.x * kBuildBlock) {
+    const uint32_t g = global_id(gid, (uint32_t)k);
     for (int a = 0; a < 3; ++a) {
-      const float c = centroid(v, (size_t)k, a);
+      const float c = centroid(v, (size_t)g, a);
       lo[a] = fminf(lo[a], c);
       hi[a] = fmaxf(hi[a], c);
     }
@@ -577,16 +663,18 @@ __device__ __forceinline__ uint32_t quant10(float u) {
   return (uint32_t)q;
 }
 
-__global__ __launch_bounds__(kBuildBlock) void k_morton(const float* __restrict__ v, int32_t T,
+__global__ __launch_bounds__(kBuildBlock) void k_morton(const float* __restrict__ v,
+                                                        const uint32_t* __restrict__ gid, int32_t T,
                                                         const float* __restrict__ bounds,
                                                         uint32_t* keys, uint32_t* vals) {
   const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
   if (k >= T) return;
+  const uint32_t g = global_id(gid, (uint32_t)k);
   uint32_t q[3];
   for (int a = 0; a < 3; ++a) {
     const float lo = bounds[a], hi = bounds[3 + a];
     const float ext = hi - lo;
-    const float c = centroid(v, (size_t)k, a);
+    const float c = centroid(v, (size_t)g, a);
     const float u = ext > 0.0f ? (c - lo) / ext : 0.5f;
     q[a] = quant10(u);
   }
@@ -699,23 +787,30 @@ __global__ __launch_bounds__(kBuildBlock) void k_sort_scatter(
   }
 }
 
+// Sorted position k: local index order[k] -> global triangle p.  Writes the
+// triangle (and normal) records and replaces order[k] by p in place.
 __global__ __launch_bounds__(kBuildBlock) void k_gather(const float* __restrict__ v,
                                                         const float* __restrict__ nrm,
                                                         const int32_t* __restrict__ mat,
-                                                        const uint32_t* __restrict__ order,
-                                                        int32_t T, float4* tris, float4* tn) {
+                                                        const uint32_t* __restrict__ gid,
+                                                        uint32_t* order, int32_t T, float4* tris,
+                                                        float4* tn) {
   const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
   if (k >= T) return;
-  const uint32_t p = order[k];
-  if (p >= (uint32_t)T) return;
+  const uint32_t local = order[k];
+  if (local >= (uint32_t)T) return;
+  const uint32_t p = global_id(gid, local);
+  order[k] = p;
   const float* a = v + 9 * (size_t)p;
-  const float* b = nrm + 9 * (size_t)p;
   tris[3 * (size_t)k + 0] = make_float4(a[0], a[1], a[2], __uint_as_float(p));
   tris[3 * (size_t)k + 1] = make_float4(a[3], a[4], a[5], __uint_as_float((uint32_t)mat[p]));
   tris[3 * (size_t)k + 2] = make_float4(a[6], a[7], a[8], 0.0f);
-  tn[3 * (size_t)k + 0] = make_float4(b[0], b[1], b[2], 0.0f);
-  tn[3 * (size_t)k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
-  tn[3 * (size_t)k + 2] = make_float4(b[6], b[7], b[8], 0.0f);
+  if (tn) {
+    const float* b = nrm + 9 * (size_t)p;
+    tn[3 * (size_t)k + 0] = make_float4(b[0], b[1], b[2], 0.0f);
+    tn[3 * (size_t)k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
+    tn[3 * (size_t)k + 2] = make_float4(b[6], b[7], b[8], 0.0f);
+  }
 }
 
 __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t n, int64_t i,
@@ -763,6 +858,19 @@ __global__ __launch_bounds__(kBuildBlock) void k_karras(const uint32_t* __restri
     parent[g + 1] = (int32_t)i;
 }
 
+// Triangle bounds widened outward by one ulp: an origin exactly on a face
+// plane of the tight box with a zero direction component would make the
+// slab test evaluate 0 * inf = NaN; inside the widened box it cannot.
+__device__ __forceinline__ void leaf_box(const float4* __restrict__ tris, int32_t k, float* b) {
+  const float4 a = tris[3 * (size_t)k], c = tris[3 * (size_t)k + 1], e = tris[3 * (size_t)k + 2];
+  b[0] = nextafterf(fminf(fminf(a.x, c.x), e.x), -INFINITY);
+  b[1] = nextafterf(fminf(fminf(a.y, c.y), e.y), -INFINITY);
+  b[2] = nextafterf(fminf(fminf(a.z, c.z), e.z), -INFINITY);
+  b[3] = nextafterf(fmaxf(fmaxf(a.x, c.x), e.x), INFINITY);
+  b[4] = nextafterf(fmaxf(fmaxf(a.y, c.y), e.y), INFINITY);
+  b[5] = nextafterf(fmaxf(fmaxf(a.z, c.z), e.z), INFINITY);
+}
+
 __device__ __forceinline__ void store_slot(float4* nodes, int32_t node, int slot, const float* b) {
   float* f = reinterpret_cast<float*>(nodes) + 16 * (size_t)node + 6 * slot;
   for (int c = 0; c < 6; ++c) f[c] = b[c];
@@ -778,15 +886,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_refit(const float4* __restrict_
   const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
   if (k >= n) return;
   float b[6];
-  {
-    const float4 a = tris[3 * (size_t)k], c = tris[3 * (size_t)k + 1], e = tris[3 * (size_t)k + 2];
-    b[0] = fminf(fminf(a.x, c.x), e.x);
-    b[1] = fminf(fminf(a.y, c.y), e.y);
-    b[2] = fminf(fminf(a.z, c.z), e.z);
-    b[3] = fmaxf(fmaxf(a.x, c.x), e.x);
-    b[4] = fmaxf(fmaxf(a.y, c.y), e.y);
-    b[5] = fmaxf(fmaxf(a.z, c.z), e.z);
-  }
+  leaf_box(tris, k, b);
   int32_t p = leaf_parent[k];
   if (p < 0 || p >= n - 1) return;
   {
@@ -816,12 +916,19 @@ __global__ __launch_bounds__(kBuildBlock) void k_refit(const float4* __restrict_
   }
 }
 
+// Plain copy (HBM calibration): float4 grid-stride.
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src,
+                                                    float4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
 // T == 1: a root with the single leaf as child 0 and an empty child 1.
 __global__ void k_single_root(const float4* __restrict__ tris, float4* nodes) {
-  const float4 a = tris[0], c = tris[1], e = tris[2];
-  nodes[0] = make_float4(fminf(fminf(a.x, c.x), e.x), fminf(fminf(a.y, c.y), e.y),
-                         fminf(fminf(a.z, c.z), e.z), fmaxf(fmaxf(a.x, c.x), e.x));
-  nodes[1] = make_float4(fmaxf(fmaxf(a.y, c.y), e.y), fmaxf(fmaxf(a.z, c.z), e.z), 0.0f, 0.0f);
+  float b[6];
+  leaf_box(tris, 0, b);
+  nodes[0] = make_float4(b[0], b[1], b[2], b[3]);
+  nodes[1] = make_float4(b[4], b[5], 0.0f, 0.0f);
   nodes[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   nodes[3] = make_float4(__uint_as_float(kLeaf), __uint_as_float(kEmpty), 0.0f, 0.0f);
 }
@@ -831,18 +938,25 @@ __global__ void k_single_root(const float4* __restrict__ tris, float4* nodes) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// A device LBVH over T triangles of the scene (see the layout at the top).
+struct DevBvh {
+  int32_t T = 0, nI = 0;
+  float4* nodes = nullptr;
+  float4* tris = nullptr;
+  uint32_t* morton = nullptr;
+  uint32_t* prim = nullptr;  // global triangle ids, Morton order
+  uint64_t bytes = 0;
+};
+
 struct lisa_ctx {
   int32_t device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
-  int32_t T = 0, nI = 0, M = 0;
-  float4* d_nodes = nullptr;
-  float4* d_tris = nullptr;
+  int32_t T = 0, M = 0;
+  DevBvh all, emit;  // over every triangle / over the emitting triangles
   float4* d_nrm = nullptr;
   DevMat* d_mats = nullptr;
-  uint32_t* d_morton = nullptr;
-  uint32_t* d_prim = nullptr;
   bool has_scene = false;
   float4* d_accum = nullptr;
   size_t accum_px = 0;
@@ -869,24 +983,29 @@ lisa_status fail(lisa_ctx* ctx, lisa_status code, const std::string& msg) {
       return fail((ctx), LISA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+void free_bvh(DevBvh& b) {
+  (void)hipFree(b.nodes);
+  (void)hipFree(b.tris);
+  (void)hipFree(b.morton);
+  (void)hipFree(b.prim);
+  b = DevBvh();
+}
+
 void free_scene(lisa_ctx* c) {
-  (void)hipFree(c->d_nodes);
-  (void)hipFree(c->d_tris);
+  free_bvh(c->all);
+  free_bvh(c->emit);
   (void)hipFree(c->d_nrm);
   (void)hipFree(c->d_mats);
-  (void)hipFree(c->d_morton);
-  (void)hipFree(c->d_prim);
-  c->d_nodes = nullptr;
-  c->d_tris = c->d_nrm = nullptr;
+  c->d_nrm = nullptr;
   c->d_mats = nullptr;
-  c->d_morton = c->d_prim = nullptr;
-  c->T = c->nI = c->M = 0;
+  c->T = c->M = 0;
   c->has_scene = false;
   c->scene_bytes = 0;
 }
 
 SceneView view(const lisa_ctx* c) {
-  return SceneView{c->d_nodes, c->d_tris, c->d_nrm, c->d_mats, c->T};
+  return SceneView{BvhView{c->all.nodes, c->all.tris, c->all.T},
+                   BvhView{c->emit.nodes, c->emit.tris, c->emit.T}, c->d_nrm, c->d_mats};
 }
 
 // Device scratch freed at scope exit.
@@ -930,6 +1049,70 @@ PathArgs make_args(const lisa_render_params* p) {
   a.W = mk(p->W[0], p->W[1], p->W[2]);
   a.fp64 = (p->flags & LISA_FLAG_FP64_ACCUM) ? 1 : 0;
   return a;
+}
+
+// LBVH over the n triangles gid[0..n) (all T when gid is null) of the soup
+// already on the device: centroid bounds -> 30-bit Morton codes -> 4-pass LDS
+// radix sort -> Morton-ordered triangle records -> Karras hierarchy -> refit.
+// Enqueued on ctx->stream; `nrm_out` (optional) receives the normal records.
+lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d_n,
+                      const int32_t* d_mat, const uint32_t* d_gid, int32_t n, float4* nrm_out) {
+  hipStream_t st = ctx->stream;
+  b.T = n;
+  b.nI = n > 1 ? n - 1 : 1;
+  const size_t nodes_b = 64 * (size_t)b.nI, tris_b = 48 * (size_t)std::max(n, 1);
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.nodes), nodes_b));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.tris), tris_b));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.morton), 4 * (size_t)std::max(n, 1)));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.prim), 4 * (size_t)std::max(n, 1)));
+  b.bytes = nodes_b + tris_b + 8 * (size_t)std::max(n, 1);
+  HIP_TRY(ctx, hipMemsetAsync(b.nodes, 0, nodes_b, st));
+  if (n == 0) return LISA_OK;
+  Scratch S;
+  float *d_part = nullptr, *d_bounds = nullptr;
+  int32_t *d_parent = nullptr, *d_leafp = nullptr;
+  uint32_t *d_k2 = nullptr, *d_v2 = nullptr, *d_hist = nullptr, *d_tot = nullptr, *d_flags = nullptr;
+  const int32_t n_tiles = (n + kSortTile - 1) / kSortTile;
+  const int32_t n_part = std::min<int32_t>(1024, (n + kBuildBlock - 1) / kBuildBlock);
+  HIP_TRY(ctx, S.alloc(&d_part, 6 * (size_t)n_part));
+  HIP_TRY(ctx, S.alloc(&d_bounds, 6));
+  HIP_TRY(ctx, S.alloc(&d_k2, (size_t)n));
+  HIP_TRY(ctx, S.alloc(&d_v2, (size_t)n));
+  HIP_TRY(ctx, S.alloc(&d_hist, 256 * (size_t)n_tiles));
+  HIP_TRY(ctx, S.alloc(&d_tot, 256));
+  HIP_TRY(ctx, S.alloc(&d_parent, (size_t)n));
+  HIP_TRY(ctx, S.alloc(&d_leafp, (size_t)n));
+  HIP_TRY(ctx, S.alloc(&d_flags, (size_t)n));
+  k_centroid_partial<<<n_part, kBuildBlock, 0, st>>>(d_v, d_gid, n, d_part);
+  k_centroid_final<<<1, kBuildBlock, 0, st>>>(d_part, n_part, d_bounds);
+  k_morton<<<grid_for(n, kBuildBlock), kBuildBlock, 0, st>>>(d_v, d_gid, n, d_bounds, b.morton, b.prim);
+  uint32_t *ka = b.morton, *va = b.prim, *kb = d_k2, *vb = d_v2;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 8 * pass;
+    k_sort_hist<<<n_tiles, kBuildBlock, 0, st>>>(ka, n, shift, d_hist, n_tiles);
+    k_sort_scan<<<256, kBuildBlock, 0, st>>>(d_hist, n_tiles, d_tot);
+    k_sort_scatter<<<n_tiles, kBuildBlock, 0, st>>>(ka, va, kb, vb, n, shift, d_hist, d_tot, n_tiles);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  // four passes: the sorted data is back in b.morton / b.prim
+  k_gather<<<grid_for(n, kBuildBlock), kBuildBlock, 0, st>>>(d_v, d_n, d_mat, d_gid, b.prim, n,
+                                                             b.tris, nrm_out);
+  if (n == 1) {
+    k_single_root<<<1, 1, 0, st>>>(b.tris, b.nodes);
+  } else {
+    HIP_TRY(ctx, hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * (size_t)n, st));
+    HIP_TRY(ctx, hipMemsetAsync(d_parent, 0xFF, sizeof(int32_t) * (size_t)n, st));
+    HIP_TRY(ctx, hipMemsetAsync(d_leafp, 0xFF, sizeof(int32_t) * (size_t)n, st));
+    k_karras<<<grid_for(n - 1, kBuildBlock), kBuildBlock, 0, st>>>(b.morton, n, b.nodes, d_parent,
+                                                                  d_leafp);
+    k_refit<<<grid_for(n, kBuildBlock), kBuildBlock, 0, st>>>(b.tris, n, b.nodes, d_parent, d_leafp,
+                                                             d_flags);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  // scratch is freed on return: wait for the kernels that use it
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return LISA_OK;
 }
 
 }  // namespace
@@ -990,7 +1173,6 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
   free_scene(ctx);
   ctx->T = T;
   ctx->M = M;
-  ctx->nI = T > 1 ? T - 1 : 1;
   ctx->bvh_ms = 0.0f;
   std::vector<DevMat> hm(std::max(M, 1));
   for (int32_t i = 0; i < M; ++i) {
@@ -999,76 +1181,39 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
     hm[i].emission_emit = make_float4(m.emission[0], m.emission[1], m.emission[2], m.emit ? 1.0f : 0.0f);
     hm[i].rough_n = make_float4(m.roughness, m.n, 0.0f, 0.0f);
   }
-  const size_t nodes_b = 4 * sizeof(float4) * (size_t)ctx->nI;
-  const size_t tris_b = 3 * sizeof(float4) * (size_t)std::max(T, 1);
+  std::vector<uint32_t> emit_ids;
+  for (int32_t k = 0; k < T; ++k)
+    if (mats[mat_idx[k]].emit) emit_ids.push_back((uint32_t)k);
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mats), sizeof(DevMat) * hm.size()));
   HIP_TRY(ctx, hipMemcpy(ctx->d_mats, hm.data(), sizeof(DevMat) * hm.size(), hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_nodes), nodes_b));
-  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_tris), tris_b));
-  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_nrm), tris_b));
-  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_morton), sizeof(uint32_t) * std::max(T, 1)));
-  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_prim), sizeof(uint32_t) * std::max(T, 1)));
-  ctx->scene_bytes = sizeof(DevMat) * hm.size() + nodes_b + 2 * tris_b + 8 * (size_t)std::max(T, 1);
-  HIP_TRY(ctx, hipMemsetAsync(ctx->d_nodes, 0, nodes_b, ctx->stream));
-  if (T == 0) {
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->has_scene = true;
-    return LISA_OK;
-  }
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_nrm), 48 * (size_t)std::max(T, 1)));
+  ctx->scene_bytes = sizeof(DevMat) * hm.size() + 48 * (size_t)std::max(T, 1);
   Scratch S;
-  float *d_v = nullptr, *d_n = nullptr, *d_part = nullptr, *d_bounds = nullptr;
-  int32_t *d_mat = nullptr, *d_parent = nullptr, *d_leafp = nullptr;
-  uint32_t *d_k2 = nullptr, *d_v2 = nullptr, *d_hist = nullptr, *d_tot = nullptr, *d_flags = nullptr;
-  const int32_t n_tiles = (T + kSortTile - 1) / kSortTile;
-  const int32_t n_part = std::min<int32_t>(1024, (T + kBuildBlock - 1) / kBuildBlock);
-  HIP_TRY(ctx, S.alloc(&d_v, 9 * (size_t)T));
-  HIP_TRY(ctx, S.alloc(&d_n, 9 * (size_t)T));
-  HIP_TRY(ctx, S.alloc(&d_mat, (size_t)T));
-  HIP_TRY(ctx, S.alloc(&d_part, 6 * (size_t)n_part));
-  HIP_TRY(ctx, S.alloc(&d_bounds, 6));
-  HIP_TRY(ctx, S.alloc(&d_k2, (size_t)T));
-  HIP_TRY(ctx, S.alloc(&d_v2, (size_t)T));
-  HIP_TRY(ctx, S.alloc(&d_hist, 256 * (size_t)n_tiles));
-  HIP_TRY(ctx, S.alloc(&d_tot, 256));
-  HIP_TRY(ctx, S.alloc(&d_parent, (size_t)T));
-  HIP_TRY(ctx, S.alloc(&d_leafp, (size_t)T));
-  HIP_TRY(ctx, S.alloc(&d_flags, (size_t)T));
-  HIP_TRY(ctx, hipMemcpy(d_v, verts, 9 * sizeof(float) * (size_t)T, hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(d_n, normals, 9 * sizeof(float) * (size_t)T, hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(d_mat, mat_idx, sizeof(int32_t) * (size_t)T, hipMemcpyHostToDevice));
-
+  float *d_v = nullptr, *d_n = nullptr;
+  int32_t* d_mat = nullptr;
+  uint32_t* d_gid = nullptr;
+  if (T > 0) {
+    HIP_TRY(ctx, S.alloc(&d_v, 9 * (size_t)T));
+    HIP_TRY(ctx, S.alloc(&d_n, 9 * (size_t)T));
+    HIP_TRY(ctx, S.alloc(&d_mat, (size_t)T));
+    HIP_TRY(ctx, hipMemcpy(d_v, verts, 9 * sizeof(float) * (size_t)T, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(d_n, normals, 9 * sizeof(float) * (size_t)T, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(d_mat, mat_idx, sizeof(int32_t) * (size_t)T, hipMemcpyHostToDevice));
+  }
+  if (!emit_ids.empty()) {
+    HIP_TRY(ctx, S.alloc(&d_gid, emit_ids.size()));
+    HIP_TRY(ctx, hipMemcpy(d_gid, emit_ids.data(), 4 * emit_ids.size(), hipMemcpyHostToDevice));
+  }
   hipStream_t st = ctx->stream;
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  k_centroid_partial<<<n_part, kBuildBlock, 0, st>>>(d_v, T, d_part);
-  k_centroid_final<<<1, kBuildBlock, 0, st>>>(d_part, n_part, d_bounds);
-  k_morton<<<grid_for(T, kBuildBlock), kBuildBlock, 0, st>>>(d_v, T, d_bounds, ctx->d_morton, ctx->d_prim);
-  uint32_t *ka = ctx->d_morton, *va = ctx->d_prim, *kb = d_k2, *vb = d_v2;
-  for (int pass = 0; pass < 4; ++pass) {
-    const int shift = 8 * pass;
-    k_sort_hist<<<n_tiles, kBuildBlock, 0, st>>>(ka, T, shift, d_hist, n_tiles);
-    k_sort_scan<<<256, kBuildBlock, 0, st>>>(d_hist, n_tiles, d_tot);
-    k_sort_scatter<<<n_tiles, kBuildBlock, 0, st>>>(ka, va, kb, vb, T, shift, d_hist, d_tot, n_tiles);
-    std::swap(ka, kb);
-    std::swap(va, vb);
-  }
-  // four passes: the sorted data is back in ctx->d_morton / ctx->d_prim
-  k_gather<<<grid_for(T, kBuildBlock), kBuildBlock, 0, st>>>(d_v, d_n, d_mat, ctx->d_prim, T,
-                                                             ctx->d_tris, ctx->d_nrm);
-  if (T == 1) {
-    k_single_root<<<1, 1, 0, st>>>(ctx->d_tris, ctx->d_nodes);
-  } else {
-    HIP_TRY(ctx, hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * (size_t)T, st));
-    HIP_TRY(ctx, hipMemsetAsync(d_parent, 0xFF, sizeof(int32_t) * (size_t)T, st));
-    HIP_TRY(ctx, hipMemsetAsync(d_leafp, 0xFF, sizeof(int32_t) * (size_t)T, st));
-    k_karras<<<grid_for(T - 1, kBuildBlock), kBuildBlock, 0, st>>>(ctx->d_morton, T, ctx->d_nodes,
-                                                                  d_parent, d_leafp);
-    k_refit<<<grid_for(T, kBuildBlock), kBuildBlock, 0, st>>>(ctx->d_tris, T, ctx->d_nodes,
-                                                             d_parent, d_leafp, d_flags);
-  }
-  HIP_TRY(ctx, hipGetLastError());
+  lisa_status rc = build_bvh(ctx, ctx->all, d_v, d_n, d_mat, nullptr, T, ctx->d_nrm);
+  if (rc == LISA_OK)
+    rc = build_bvh(ctx, ctx->emit, d_v, d_n, d_mat, d_gid, (int32_t)emit_ids.size(), nullptr);
+  if (rc != LISA_OK) return rc;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
+  ctx->scene_bytes += ctx->all.bytes + ctx->emit.bytes;
   ctx->has_scene = true;
   return LISA_OK;
 }
@@ -1229,19 +1374,21 @@ lisa_status lisa_trace_rays(lisa_ctx* ctx, const float* O, const float* D, int32
   return LISA_OK;
 }
 
-lisa_status lisa_get_bvh(lisa_ctx* ctx, uint32_t* morton, uint32_t* prim, uint32_t* child_refs,
-                         float* child_boxes) {
+lisa_status lisa_get_bvh(lisa_ctx* ctx, int32_t which, uint32_t* morton, uint32_t* prim,
+                         uint32_t* child_refs, float* child_boxes) {
   if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
   if (!ctx->has_scene) return fail(ctx, LISA_ERR_NO_SCENE, "lisa_set_scene has not been called");
-  if (ctx->T == 0) return LISA_OK;
+  if (which != 0 && which != 1) return fail(ctx, LISA_ERR_INVALID_ARG, "which must be 0 or 1");
+  const DevBvh& b = which == 0 ? ctx->all : ctx->emit;
+  if (b.T == 0) return LISA_OK;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (morton) HIP_TRY(ctx, hipMemcpy(morton, ctx->d_morton, 4 * (size_t)ctx->T, hipMemcpyDeviceToHost));
-  if (prim) HIP_TRY(ctx, hipMemcpy(prim, ctx->d_prim, 4 * (size_t)ctx->T, hipMemcpyDeviceToHost));
+  if (morton) HIP_TRY(ctx, hipMemcpy(morton, b.morton, 4 * (size_t)b.T, hipMemcpyDeviceToHost));
+  if (prim) HIP_TRY(ctx, hipMemcpy(prim, b.prim, 4 * (size_t)b.T, hipMemcpyDeviceToHost));
   if (child_refs || child_boxes) {
-    std::vector<float> nodes(16 * (size_t)ctx->nI);
-    HIP_TRY(ctx, hipMemcpy(nodes.data(), ctx->d_nodes, 64 * (size_t)ctx->nI, hipMemcpyDeviceToHost));
-    for (int32_t i = 0; i < ctx->nI; ++i) {
+    std::vector<float> nodes(16 * (size_t)b.nI);
+    HIP_TRY(ctx, hipMemcpy(nodes.data(), b.nodes, 64 * (size_t)b.nI, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < b.nI; ++i) {
       if (child_boxes) std::memcpy(child_boxes + 12 * (size_t)i, &nodes[16 * (size_t)i], 48);
       if (child_refs) std::memcpy(child_refs + 2 * (size_t)i, &nodes[16 * (size_t)i + 12], 8);
     }
@@ -1261,6 +1408,29 @@ lisa_status lisa_copy_accum(lisa_ctx* ctx, float* rgba_out, int64_t n_pixels) {
   return LISA_OK;
 }
 
+lisa_status lisa_copy_probe(lisa_ctx* ctx, int64_t bytes, int32_t iters, float* avg_ms) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (bytes < 16 || iters < 1 || !avg_ms) return fail(ctx, LISA_ERR_INVALID_ARG, "bad copy probe args");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int64_t n = bytes / 16;
+  Scratch S;
+  float4 *a = nullptr, *b = nullptr;
+  HIP_TRY(ctx, S.alloc(&a, (size_t)n));
+  HIP_TRY(ctx, S.alloc(&b, (size_t)n));
+  HIP_TRY(ctx, hipMemsetAsync(a, 0, (size_t)n * 16, ctx->stream));
+  hipStream_t st = ctx->stream;
+  k_copy_probe<<<4096, 256, 0, st>>>(a, b, n);  // warm
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  for (int32_t i = 0; i < iters; ++i) k_copy_probe<<<4096, 256, 0, st>>>(a, b, n);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  float ms = 0.0f;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *avg_ms = ms / (float)iters;
+  return LISA_OK;
+}
+
 lisa_status lisa_synchronize(lisa_ctx* ctx) {
   if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1272,7 +1442,8 @@ lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out) {
   out->render_ms = ctx->render_ms;
   out->bvh_build_ms = ctx->bvh_ms;
   out->num_tris = ctx->T;
-  out->num_nodes = ctx->nI;
+  out->num_nodes = ctx->all.nI;
+  out->num_emitters = ctx->emit.T;
   out->device_bytes = ctx->scene_bytes + ctx->accum_bytes;
   return LISA_OK;
 }

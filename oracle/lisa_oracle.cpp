@@ -144,6 +144,11 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT 2013): widens the
 // far slab distance so rounding never culls a box the ray touches.
 constexpr float kRobust = 0x1.000006p+0f;
+// Boxes are culled against the best hit distance widened by 2^-16: a
+// triangle whose hit lies on its box boundary (axis-aligned or coplanar
+// geometry) can compute a t a few ulp below the box's slab entry, and must
+// still be tested so that equal-t hits resolve to the lower triangle id.
+constexpr float kWiden = 0x1.0001p+0f;
 
 // Spread the low 10 bits of v to every third bit (bit-by-bit form).
 inline uint32_t spread10(uint32_t v) {
@@ -160,27 +165,37 @@ inline uint32_t morton3(float x, float y, float z) {
   return (spread10(quant10(x)) << 2) | (spread10(quant10(y)) << 1) | spread10(quant10(z));
 }
 
+// One LBVH over a subset of the soup's triangles (all of them, or the
+// emitters).  prim holds global triangle ids in Morton order.
+struct Bvh {
+  int32_t T = 0;
+  std::vector<uint32_t> morton, prim;  // sorted
+  std::vector<uint32_t> refs;          // 2 per internal node
+  std::vector<float> boxes;            // 12 per internal node
+  std::vector<float> tv;               // 9 per sorted triangle (traversal copy)
+};
+
 struct Scene {
   int32_t T = 0;
   std::vector<float> v, n;     // 9T each, original order
   std::vector<int32_t> mat;    // T
   std::vector<orc_material> mats;
-  std::vector<uint32_t> morton, prim;  // sorted
-  std::vector<uint32_t> refs;          // 2 per internal node
-  std::vector<float> boxes;            // 12 per internal node
-  std::vector<float> tv;               // 9T, sorted order (traversal copy)
+  Bvh all;   // every triangle
+  Bvh emit;  // the triangles whose material emits (light queries of NEE)
 };
 
 inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
 
-void build_bvh(Scene& s) {
-  const int64_t n = s.T;
+// Builds `s` over the triangles gid[0..n) of the soup `v` (9 floats each).
+void build_bvh(Bvh& s, const std::vector<float>& v, const std::vector<uint32_t>& gid) {
+  const int64_t n = (int64_t)gid.size();
+  s.T = (int32_t)n;
   if (n == 0) return;
   // Centroids and their bounds.
   std::vector<float> c(3 * n);
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int64_t k = 0; k < n; ++k) {
-    const float* t = &s.v[9 * k];
+    const float* t = &v[9 * (size_t)gid[k]];
     for (int a = 0; a < 3; ++a) {
       const float ca = (t[a] + t[3 + a] + t[6 + a]) / 3.0f;
       c[3 * k + a] = ca;
@@ -197,24 +212,32 @@ void build_bvh(Scene& s) {
     }
     code[k] = morton3(u[0], u[1], u[2]);
   }
-  s.prim.resize(n);
-  for (int64_t k = 0; k < n; ++k) s.prim[k] = (uint32_t)k;
-  std::stable_sort(s.prim.begin(), s.prim.end(),
+  std::vector<uint32_t> order(n);
+  for (int64_t k = 0; k < n; ++k) order[k] = (uint32_t)k;
+  std::stable_sort(order.begin(), order.end(),
                    [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
   s.morton.resize(n);
-  for (int64_t k = 0; k < n; ++k) s.morton[k] = code[s.prim[k]];
+  s.prim.resize(n);
   s.tv.resize(9 * n);
-  for (int64_t k = 0; k < n; ++k)
-    std::memcpy(&s.tv[9 * k], &s.v[9 * (int64_t)s.prim[k]], 9 * sizeof(float));
+  for (int64_t k = 0; k < n; ++k) {
+    s.morton[k] = code[order[k]];
+    s.prim[k] = gid[order[k]];
+    std::memcpy(&s.tv[9 * k], &v[9 * (size_t)s.prim[k]], 9 * sizeof(float));
+  }
 
   const int64_t nI = std::max<int64_t>(n - 1, 1);
   s.refs.assign(2 * nI, kEmpty);
   s.boxes.assign(12 * nI, 0.0f);
+  // Leaf boxes are the triangle bounds widened outward by one ulp: a ray
+  // with a zero direction component whose origin lies exactly on a face
+  // plane of the tight box would evaluate 0 * inf = NaN in the slab test;
+  // with the widened box such an origin lies strictly inside, and an origin
+  // on the widened plane is outside all of the box's geometry.
   auto leaf_box = [&](int64_t k, float* b) {
     const float* t = &s.tv[9 * k];
     for (int a = 0; a < 3; ++a) {
-      b[a] = std::fmin(std::fmin(t[a], t[3 + a]), t[6 + a]);
-      b[3 + a] = std::fmax(std::fmax(t[a], t[3 + a]), t[6 + a]);
+      b[a] = std::nextafter(std::fmin(std::fmin(t[a], t[3 + a]), t[6 + a]), -INFINITY);
+      b[3 + a] = std::nextafter(std::fmax(std::fmax(t[a], t[3 + a]), t[6 + a]), INFINITY);
     }
   };
   if (n == 1) {
@@ -362,32 +385,42 @@ struct Counters {
   uint64_t c[ORC_NCOUNTERS] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
-// Closest hit: returns the original triangle id (-1 on miss).  Accepts
-// t in (tmin, tmax]; equal t resolves to the lower triangle id.
-int32_t closest(const Scene& s, const Ray& r, float tmin, float tmax, float& t_out, Counters& cnt) {
-  if (!r.valid || s.T == 0) return -1;
-  float best_t = tmax;
-  uint32_t best = 0xFFFFFFFFu;
+// Closest hit (kAny = false) or any hit (kAny = true) over `b`: returns the
+// global triangle id (-1 when none).  A triangle is accepted when
+// t > tmin and (t < best_t or (t == best_t and id < best)), starting from
+// best_t = tlim, best = lim_prim; the closest hit therefore resolves equal t
+// to the lower triangle id, and the any-hit query answers "is any triangle
+// strictly before (tlim, lim_prim) in that order".
+template <bool kAny>
+int32_t closest(const Bvh& b, const Ray& r, float tmin, float tlim, uint32_t lim_prim,
+                float& t_out, Counters& cnt) {
+  if (!r.valid || b.T == 0) return -1;
+  float best_t = tlim;
+  float cull_t = tlim * kWiden;
+  uint32_t best = lim_prim;
+  bool found = false;
   uint32_t stack[64];
   int sp = 0;
   uint32_t node = 0;
   auto tri_test = [&](uint32_t k) {
     cnt.c[3]++;
     float t;
-    const uint32_t p = s.prim[k];
-    if (watertight(r, &s.tv[9 * (size_t)k], t) && t > tmin &&
+    const uint32_t p = b.prim[k];
+    if (watertight(r, &b.tv[9 * (size_t)k], t) && t > tmin &&
         (t < best_t || (t == best_t && p < best))) {
       best_t = t;
+      cull_t = t * kWiden;
       best = p;
+      found = true;
     }
   };
   for (;;) {
     cnt.c[2]++;
-    const float* bx = &s.boxes[12 * (size_t)node];
-    const uint32_t r0 = s.refs[2 * (size_t)node], r1 = s.refs[2 * (size_t)node + 1];
+    const float* bx = &b.boxes[12 * (size_t)node];
+    const uint32_t r0 = b.refs[2 * (size_t)node], r1 = b.refs[2 * (size_t)node + 1];
     float tn0 = 0.0f, tn1 = 0.0f;
-    bool h0 = r0 != kEmpty && slab(bx, r, tmin, best_t, tn0);
-    bool h1 = r1 != kEmpty && slab(bx + 6, r, tmin, best_t, tn1);
+    bool h0 = r0 != kEmpty && slab(bx, r, tmin, cull_t, tn0);
+    bool h1 = r1 != kEmpty && slab(bx + 6, r, tmin, cull_t, tn1);
     if (h0 && (r0 & kLeaf)) {
       tri_test(r0 & ~kLeaf);
       h0 = false;
@@ -396,6 +429,7 @@ int32_t closest(const Scene& s, const Ray& r, float tmin, float tmax, float& t_o
       tri_test(r1 & ~kLeaf);
       h1 = false;
     }
+    if (kAny && found) break;
     if (h0 && h1) {
       uint32_t nr = r0, fr = r1;
       if (tn1 < tn0) {
@@ -413,7 +447,7 @@ int32_t closest(const Scene& s, const Ray& r, float tmin, float tmax, float& t_o
       node = stack[--sp];
     }
   }
-  if (best == 0xFFFFFFFFu) return -1;
+  if (!found) return -1;
   t_out = best_t;
   return (int32_t)best;
 }
@@ -453,6 +487,84 @@ struct RayState {  // shader.cu:30-42
   bool done = false;
 };
 
+// shader.cu:196-209 with the occlusion programs 172-184.  `occlude(dir, hit)`
+// returns the material id of the closest hit or -1 on a miss (given the
+// current hit flag it may answer a non-emitting id for a miss when the two
+// have the same effect).  The hit flag lives in RayState and is only
+// cleared by a miss (Q1: after one success, a later NEE returns on its first
+// non-missing ray with the stale light).  Returns true with the sampled
+// direction in dir_out when a light is found.
+template <class Sample, class Occlude, class Emits>
+bool nee_loop(RayState& st, Sample sample_dir, Occlude occlude, Emits emits, V3& dir_out,
+              int& tries) {
+  for (int i = 0; i < kNeeTries; ++i) {
+    const V3 dir = sample_dir();
+    ++tries;
+    const int32_t m = occlude(dir, st.hit);
+    if (m < 0) {
+      st.hit = false;
+    } else if (emits(m)) {
+      st.light_mat = m;
+      st.hit = true;
+    }
+    if (st.hit) {
+      dir_out = dir;
+      return true;
+    }
+  }
+  return false;
+}
+
+constexpr int32_t kNonEmitter = 0x7FFFFFFF;  // a material id that never emits
+
+// The occlusion query of NEE.  Returns the material of the closest hit when
+// it emits, -1 on a miss, kNonEmitter when the closest hit does not emit --
+// exactly the outcomes shader.cu:172-184 distinguish, computed as:
+//   1. closest emitter hit (t_E, E) from the emitter BVH;
+//   2. if found: any triangle before (t_E, E) in (t, id) order? -> blocked;
+//   3. if none and the hit flag is false, miss and non-emitter both leave
+//      it false, so no further query is made;
+//   4. otherwise any triangle at all? -> non-emitter, else miss.
+// `code` is the per-ray record: the emitter's triangle id, or -5 when the
+// flag was false and no light was found, -4 non-emitter, -1 miss.
+int32_t occlusion_query(const Scene& s, V3 O, V3 D, bool hit_before, int32_t& code,
+                        Counters& cnt) {
+  const Ray r = make_ray(O, D);
+  if (!r.valid) {
+    cnt.c[6]++;
+    code = hit_before ? -1 : -5;
+    return -1;
+  }
+  float tE = 0.0f, tb = 0.0f;
+  const int32_t e = closest<false>(s.emit, r, kTmin, kTmax, 0xFFFFFFFFu, tE, cnt);
+  if (e >= 0) {
+    if (closest<true>(s.all, r, kTmin, tE, (uint32_t)e, tb, cnt) < 0) {
+      code = e;
+      return s.mat[e];
+    }
+    code = hit_before ? -4 : -5;
+    return kNonEmitter;
+  }
+  if (!hit_before) {
+    code = -5;
+    return kNonEmitter;
+  }
+  const bool blocked = closest<true>(s.all, r, kTmin, kTmax, 0xFFFFFFFFu, tb, cnt) >= 0;
+  code = blocked ? -4 : -1;
+  return blocked ? kNonEmitter : -1;
+}
+
+// The same record code from one full closest-hit query (the reference
+// semantics of an occlusion ray, for checking occlusion_query).
+int32_t occlusion_reference(const Scene& s, V3 O, V3 D, bool hit_before, Counters& cnt) {
+  const Ray r = make_ray(O, D);
+  float t = 0.0f;
+  const int32_t p = closest<false>(s.all, r, kTmin, kTmax, 0xFFFFFFFFu, t, cnt);
+  if (p >= 0 && s.mats[s.mat[p]].emit) return p;
+  if (!hit_before) return -5;
+  return p < 0 ? -1 : -4;
+}
+
 struct SampleRec {
   int32_t bounces = 0;
   int32_t occl = 0;
@@ -479,26 +591,25 @@ struct Tracer {
       cnt.c[6]++;
       return -2;
     }
-    return closest(s, r, kTmin, kTmax, t, cnt);
+    return closest<false>(s.all, r, kTmin, kTmax, 0xFFFFFFFFu, t, cnt);
   }
 
   // shader.cu:196-209 (+ occlusion CH/miss 172-184).
   V3 shoot_ray_to_light(RayState& st) {
-    for (int i = 0; i < kNeeTries; ++i) {
-      const V3 dir = hemisphere(st.normal, seed);
-      cnt.c[1]++;
-      rec.occl++;
-      float t;
-      const int32_t p = trace_ray(st.origin, dir, t);
-      rec.ray(p, true);
-      if (p < 0) {
-        st.hit = false;
-      } else if (mat_of(p).emit) {
-        st.light_mat = s.mat[p];
-        st.hit = true;
-      }
-      if (st.hit) return ld3(s.mats[st.light_mat].emission) * bsdf_eval(st.normal, dir);
-    }
+    V3 dir;
+    int tries = 0;
+    const bool found = nee_loop(
+        st, [&]() { return hemisphere(st.normal, seed); },
+        [&](V3 d, bool hit_before) {
+          cnt.c[1]++;
+          rec.occl++;
+          int32_t code;
+          const int32_t m = occlusion_query(s, st.origin, d, hit_before, code, cnt);
+          rec.ray(code, true);
+          return m;
+        },
+        [&](int32_t m) { return m != kNonEmitter && s.mats[m].emit != 0; }, dir, tries);
+    if (found) return ld3(s.mats[st.light_mat].emission) * bsdf_eval(st.normal, dir);
     return mk(0.0f, 0.0f, 0.0f);
   }
 
@@ -676,6 +787,26 @@ int orc_save_ppm(const char* path, const uint8_t* rgba8, int width, int height) 
 }
 uint32_t orc_morton3(float x, float y, float z) { return morton3(x, y, z); }
 
+int32_t orc_nee_scripted(const int32_t* outcomes, int32_t n, int32_t hit_in, int32_t light_in,
+                         int32_t* hit_out, int32_t* light_out) {
+  RayState st;
+  st.hit = hit_in != 0;
+  st.light_mat = light_in;
+  int i = 0, tries = 0;
+  V3 dir;
+  const bool found = nee_loop(
+      st, [&]() { return mk((float)i, 0.0f, 0.0f); },
+      [&](V3, bool) {
+        const int32_t m = i < n ? outcomes[i] : 0;
+        ++i;
+        return m;
+      },
+      [&](int32_t m) { return m > 0; }, dir, tries);
+  *hit_out = st.hit ? 1 : 0;
+  *light_out = st.light_mat;
+  return found ? tries : -tries;
+}
+
 orc_scene* orc_scene_create(const float* verts, const float* normals, const int32_t* mat_idx,
                             int32_t T, const orc_material* mats, int32_t M) {
   orc_scene* o = new orc_scene;
@@ -685,15 +816,23 @@ orc_scene* orc_scene_create(const float* verts, const float* normals, const int3
   s.n.assign(normals, normals + 9 * (size_t)T);
   s.mat.assign(mat_idx, mat_idx + T);
   s.mats.assign(mats, mats + M);
-  build_bvh(s);
+  std::vector<uint32_t> gid_all(T), gid_emit;
+  for (int32_t k = 0; k < T; ++k) {
+    gid_all[k] = (uint32_t)k;
+    if (s.mats[s.mat[k]].emit) gid_emit.push_back((uint32_t)k);
+  }
+  build_bvh(s.all, s.v, gid_all);
+  build_bvh(s.emit, s.v, gid_emit);
   return o;
 }
 void orc_scene_destroy(orc_scene* s) { delete s; }
 int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
 
-int orc_get_bvh(const orc_scene* o, uint32_t* morton_sorted, uint32_t* prim_sorted,
+int32_t orc_num_emitters(const orc_scene* s) { return s->s.emit.T; }
+
+int orc_get_bvh(const orc_scene* o, int32_t which, uint32_t* morton_sorted, uint32_t* prim_sorted,
                 uint32_t* child_refs, float* child_boxes) {
-  const Scene& s = o->s;
+  const Bvh& s = which == 0 ? o->s.all : o->s.emit;
   if (s.T == 0) return 0;
   std::copy(s.morton.begin(), s.morton.end(), morton_sorted);
   std::copy(s.prim.begin(), s.prim.end(), prim_sorted);
@@ -713,7 +852,7 @@ void orc_closest_hit(const orc_scene* o, const float* O, const float* D, int32_t
     for (int32_t i = 0; i < n; ++i) {
       const Ray r = make_ray(ld3(O + 3 * (size_t)i), ld3(D + 3 * (size_t)i));
       float t = 0.0f;
-      const int32_t p = closest(s, r, tmin, tmax, t, cnt);
+      const int32_t p = closest<false>(s.all, r, tmin, tmax, 0xFFFFFFFFu, t, cnt);
       prim_out[i] = p;
       t_out[i] = p >= 0 ? t : 0.0f;
     }
@@ -722,6 +861,22 @@ void orc_closest_hit(const orc_scene* o, const float* O, const float* D, int32_t
   }
   if (counters)
     for (int k = 0; k < ORC_NCOUNTERS; ++k) counters[k] = total.c[k];
+}
+
+void orc_occlusion(const orc_scene* o, const float* O, const float* D, int32_t n,
+                   const int32_t* hit_before, int32_t* code_fast, int32_t* code_ref,
+                   uint64_t* counters) {
+  const Scene& s = o->s;
+  Counters cf, cr;
+  for (int32_t i = 0; i < n; ++i) {
+    const V3 Oi = ld3(O + 3 * (size_t)i), Di = ld3(D + 3 * (size_t)i);
+    int32_t code;
+    occlusion_query(s, Oi, Di, hit_before[i] != 0, code, cf);
+    code_fast[i] = code;
+    code_ref[i] = occlusion_reference(s, Oi, Di, hit_before[i] != 0, cr);
+  }
+  if (counters)
+    for (int k = 0; k < ORC_NCOUNTERS; ++k) counters[k] = cf.c[k];
 }
 
 void orc_closest_hit_brute(const orc_scene* o, const float* O, const float* D, int32_t n,

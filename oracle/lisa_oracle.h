@@ -76,6 +76,12 @@ void orc_camera_uvw(const float* eye, const float* lookat, float fov, float aspe
 void orc_srgb_quantize(const float* rgba, int n_pixels, uint8_t* rgba8);
 int orc_save_ppm(const char* path, const uint8_t* rgba8, int width, int height);
 uint32_t orc_morton3(float x, float y, float z); /* x,y,z in [0,1] */
+/* The NEE loop of shader.cu:196-209 driven by scripted occlusion outcomes
+ * (-1 miss, 0 non-emitter, m > 0 emitter material m; past n: non-emitter).
+ * Returns +tries when a light was found, -tries otherwise; the final hit
+ * flag and light material are returned through hit_out / light_out. */
+int32_t orc_nee_scripted(const int32_t* outcomes, int32_t n, int32_t hit_in, int32_t light_in,
+                         int32_t* hit_out, int32_t* light_out);
 
 /* --- scene / LBVH --- */
 orc_scene* orc_scene_create(const float* verts /*9T*/, const float* normals /*9T*/,
@@ -83,10 +89,18 @@ orc_scene* orc_scene_create(const float* verts /*9T*/, const float* normals /*9T
                             const orc_material* mats, int32_t M);
 void orc_scene_destroy(orc_scene* s);
 int32_t orc_num_tris(const orc_scene* s);
-/* morton_sorted[T], prim_sorted[T], child refs [2*max(T-1,1)],
- * child boxes [12*max(T-1,1)] as (c0.lo.xyz, c0.hi.xyz, c1.lo.xyz, c1.hi.xyz) */
-int orc_get_bvh(const orc_scene* s, uint32_t* morton_sorted, uint32_t* prim_sorted,
-                uint32_t* child_refs, float* child_boxes);
+int32_t orc_num_emitters(const orc_scene* s);
+/* which = 0: the BVH over all triangles, 1: over the emitting triangles (N of them).
+ * morton_sorted[N], prim_sorted[N] (global triangle ids), child refs [2*max(N-1,1)],
+ * child boxes [12*max(N-1,1)] as (c0.lo.xyz, c0.hi.xyz, c1.lo.xyz, c1.hi.xyz) */
+int orc_get_bvh(const orc_scene* s, int32_t which, uint32_t* morton_sorted,
+                uint32_t* prim_sorted, uint32_t* child_refs, float* child_boxes);
+/* NEE occlusion record codes of n rays with the given hit flags, from the
+ * emitter-BVH + any-hit query used by the path loop (code_fast) and from one
+ * full closest-hit query (code_ref).  Counters are those of the fast query. */
+void orc_occlusion(const orc_scene* s, const float* O, const float* D, int32_t n,
+                   const int32_t* hit_before, int32_t* code_fast, int32_t* code_ref,
+                   uint64_t* counters);
 
 /* Closest hit of n rays; prim_out = -1 on miss; counters optional (ORC_NCOUNTERS). */
 void orc_closest_hit(const orc_scene* s, const float* O, const float* D, int32_t n,

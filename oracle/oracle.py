@@ -71,13 +71,18 @@ def lib():
         L.orc_save_ppm.argtypes = [C.c_char_p, u8p, C.c_int, C.c_int]
         L.orc_morton3.restype = C.c_uint32
         L.orc_morton3.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.orc_nee_scripted.restype = C.c_int32
+        L.orc_nee_scripted.argtypes = [i32p, C.c_int32, C.c_int32, C.c_int32, i32p, i32p]
         L.orc_scene_create.restype = C.c_void_p
         L.orc_scene_create.argtypes = [f32p, f32p, i32p, C.c_int32, C.POINTER(Material), C.c_int32]
         L.orc_scene_destroy.argtypes = [C.c_void_p]
         L.orc_num_tris.restype = C.c_int32
         L.orc_num_tris.argtypes = [C.c_void_p]
         L.orc_get_bvh.restype = C.c_int
-        L.orc_get_bvh.argtypes = [C.c_void_p, u32p, u32p, u32p, f32p]
+        L.orc_get_bvh.argtypes = [C.c_void_p, C.c_int32, u32p, u32p, u32p, f32p]
+        L.orc_num_emitters.restype = C.c_int32
+        L.orc_num_emitters.argtypes = [C.c_void_p]
+        L.orc_occlusion.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, i32p, i32p, i32p, u64p]
         L.orc_closest_hit.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, C.c_float, C.c_float,
                                       i32p, f32p, u64p, C.c_int]
         L.orc_closest_hit_brute.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, C.c_float,
@@ -191,6 +196,14 @@ def morton3(x, y, z):
     return lib().orc_morton3(x, y, z)
 
 
+def nee_scripted(outcomes, hit_in=False, light_in=-1):
+    o = np.ascontiguousarray(outcomes, dtype=np.int32)
+    h, l = C.c_int32(), C.c_int32()
+    r = lib().orc_nee_scripted(_p(o, C.c_int32) if o.size else None, o.size, int(hit_in),
+                               light_in, C.byref(h), C.byref(l))
+    return r, bool(h.value), l.value
+
+
 def materials_array(mats):
     arr = (Material * max(len(mats), 1))()
     for i, m in enumerate(mats):
@@ -228,16 +241,28 @@ class Scene:
         except Exception:
             pass
 
-    def bvh(self):
-        T = self.T
+    def bvh(self, which=0):
+        T = self.T if which == 0 else lib().orc_num_emitters(self.h)
         nI = max(T - 1, 1)
-        m = np.zeros(T, np.uint32)
-        p = np.zeros(T, np.uint32)
+        m = np.zeros(max(T, 1), np.uint32)
+        p = np.zeros(max(T, 1), np.uint32)
         r = np.zeros(2 * nI, np.uint32)
         b = np.zeros(12 * nI, np.float32)
-        lib().orc_get_bvh(self.h, _p(m, C.c_uint32), _p(p, C.c_uint32), _p(r, C.c_uint32),
-                          _p(b, C.c_float))
-        return {"morton": m, "prim": p, "refs": r.reshape(nI, 2), "boxes": b.reshape(nI, 12)}
+        lib().orc_get_bvh(self.h, which, _p(m, C.c_uint32), _p(p, C.c_uint32),
+                          _p(r, C.c_uint32), _p(b, C.c_float))
+        return {"morton": m[:T], "prim": p[:T], "refs": r.reshape(nI, 2),
+                "boxes": b.reshape(nI, 12)}
+
+    def occlusion(self, O, D, hit_before):
+        O, D = f32(O).reshape(-1, 3), f32(D).reshape(-1, 3)
+        hb = np.ascontiguousarray(hit_before, dtype=np.int32)
+        n = O.shape[0]
+        fast = np.zeros(n, np.int32)
+        ref = np.zeros(n, np.int32)
+        cnt = np.zeros(NCOUNTERS, np.uint64)
+        lib().orc_occlusion(self.h, _p(O, C.c_float), _p(D, C.c_float), n, _p(hb, C.c_int32),
+                            _p(fast, C.c_int32), _p(ref, C.c_int32), _p(cnt, C.c_uint64))
+        return fast, ref, cnt
 
     def closest_hit(self, O, D, tmin=1e-4, tmax=1e16, threads=1, brute=False):
         O, D = f32(O).reshape(-1, 3), f32(D).reshape(-1, 3)

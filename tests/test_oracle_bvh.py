@@ -1,0 +1,208 @@
+"""Pins for the oracle's LBVH and traversal (no GPU).
+
+* Morton codes: an independent bit interleave of the quantised coordinates.
+* LBVH (Karras, HPG 2012): sorted keys non-decreasing, stable for equal keys,
+  every internal node spans a contiguous range of sorted leaves whose split
+  sits at the highest differing bit of the index-augmented keys, every leaf
+  reached exactly once, depth < 64, refit boxes = brute-force unions.
+* Traversal: the BVH closest hit equals a brute-force loop over every
+  triangle (same watertight test, same tie rule), bit for bit.
+* Watertightness (Woop, Benthin & Wald, JCGT 2013): rays through shared
+  edges and vertices of a triangle fan never fall through.
+"""
+import numpy as np
+import pytest
+
+from tests.scene_util import cornell, random_rays
+
+LEAF = 0x80000000
+EMPTY = 0xFFFFFFFF
+
+
+def interleave(qx, qy, qz):
+    code = 0
+    for b in range(10):
+        code |= ((qx >> b) & 1) << (3 * b + 2)
+        code |= ((qy >> b) & 1) << (3 * b + 1)
+        code |= ((qz >> b) & 1) << (3 * b)
+    return code
+
+
+def test_morton_interleave(oracle_mod):
+    rs = np.random.default_rng(0)
+    for _ in range(500):
+        u = rs.uniform(-0.1, 1.1, size=3).astype(np.float32)
+        q = [min(max(int(np.float32(x) * np.float32(1024.0)), 0), 1023) if x >= 0 else 0 for x in u]
+        assert oracle_mod.morton3(*map(float, u)) == interleave(*q)
+    assert oracle_mod.morton3(1.0, 1.0, 1.0) == (1 << 30) - 1
+    assert oracle_mod.morton3(0.0, 0.0, 0.0) == 0
+
+
+def _tri_boxes(sc, prim):
+    # triangle bounds widened outward by one ulp
+    v = sc["verts"].reshape(-1, 3, 3)[prim]
+    return (np.nextafter(v.min(axis=1), np.float32(-np.inf)),
+            np.nextafter(v.max(axis=1), np.float32(np.inf)))
+
+
+def _check_tree(sc, bvh):
+    T = sc["verts"].shape[0]
+    morton, prim, refs, boxes = bvh["morton"], bvh["prim"], bvh["refs"], bvh["boxes"]
+    assert np.all(np.diff(morton.astype(np.int64)) >= 0)
+    assert np.array_equal(np.sort(prim), np.arange(T))
+    same = np.diff(morton.astype(np.int64)) == 0
+    assert np.all(np.diff(prim.astype(np.int64))[same] > 0)  # stable
+    if T == 1:
+        assert refs[0, 0] == LEAF and refs[0, 1] == EMPTY
+        return
+    lo, hi = _tri_boxes(sc, prim)  # per sorted leaf
+
+    def aug(i):
+        return (int(morton[i]) << 32) | i
+
+    seen = np.zeros(T, np.int32)
+    stack = [(0, 0)]
+    ranges = {}
+    maxdepth = 0
+
+    def span(ref):
+        if ref & LEAF:
+            k = ref & ~LEAF
+            return k, k
+        return ranges[ref]
+
+    # post-order: compute ranges and boxes bottom up
+    order = []
+    while stack:
+        node, depth = stack.pop()
+        maxdepth = max(maxdepth, depth)
+        order.append(node)
+        for r in refs[node]:
+            r = int(r)
+            if r & LEAF:
+                seen[r & ~LEAF] += 1
+            else:
+                stack.append((r, depth + 1))
+    assert np.all(seen == 1)
+    assert maxdepth < 64
+    for node in reversed(order):
+        r0, r1 = int(refs[node, 0]), int(refs[node, 1])
+        a0, b0 = span(r0)
+        a1, b1 = span(r1)
+        assert b0 + 1 == a1  # contiguous, left range then right range
+        ranges[node] = (a0, b1)
+        # split at the highest differing bit of the augmented keys of the range
+        first, last = aug(a0), aug(b1)
+        if first != last:
+            bit = (first ^ last).bit_length() - 1
+            assert (aug(b0) >> bit) & 1 == 0 and (aug(a1) >> bit) & 1 == 1
+        for ch, (a, b) in enumerate(((a0, b0), (a1, b1))):
+            want_lo = lo[a:b + 1].min(axis=0)
+            want_hi = hi[a:b + 1].max(axis=0)
+            got = boxes[node, 6 * ch:6 * ch + 6]
+            assert np.array_equal(got[:3], want_lo) and np.array_equal(got[3:], want_hi)
+    assert ranges[0] == (0, T - 1)
+
+
+@pytest.mark.parametrize("seg,rings", [(32, 16), (64, 32)])
+def test_lbvh_invariants(oracle_mod, seg, rings):
+    sc = cornell(seg, rings, 8, 8, 1, 1)
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    _check_tree(sc, o.bvh())
+
+
+def test_lbvh_duplicate_keys_and_tiny(oracle_mod):
+    # many triangles with the same centroid: equal Morton keys, index-augmented split
+    rs = np.random.default_rng(3)
+    base = rs.uniform(-1, 1, size=(1, 9)).astype(np.float32)
+    v = np.repeat(base, 37, axis=0)
+    n = np.tile(np.float32([0, 1, 0]), (37, 3))
+    mats = [{"diffuse": (1, 1, 1), "alpha": 1.0, "roughness": 1.0}]
+    for T in (1, 2, 3, 37):
+        sc = {"verts": v[:T], "normals": n[:T], "mat_idx": np.zeros(T, np.int32),
+              "materials": mats}
+        o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], mats)
+        _check_tree(sc, o.bvh())
+
+
+@pytest.mark.parametrize("seg,rings,inside", [(32, 16, True), (32, 16, False), (64, 32, True)])
+def test_traversal_equals_brute_force(oracle_mod, seg, rings, inside):
+    sc = cornell(seg, rings, 8, 8, 1, 1)
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    O, D = random_rays(4000, seed=seg + inside, inside=inside)
+    # plus unnormalised and axis-parallel directions (dielectric / lerp paths)
+    D[:500] *= 3.7
+    D[500:600] = np.float32([0, -1, 0])
+    D[600:700] = np.float32([1, 0, 0])
+    p1, t1, cnt = o.closest_hit(O, D, threads=4)
+    p2, t2, _ = o.closest_hit(O, D, brute=True)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
+    assert cnt[2] > 0 and cnt[3] > 0
+
+
+def test_zero_and_nan_directions_miss(oracle_mod):
+    sc = cornell(32, 16, 8, 8, 1, 1)
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    O = np.zeros((3, 3), np.float32)
+    D = np.float32([[0, 0, 0], [np.nan, 0, 1], [np.inf, 0, 0]])
+    p, _, cnt = o.closest_hit(O, D)
+    assert np.all(p == -1) and cnt[2] == 0
+
+
+def test_watertight_fan(oracle_mod):
+    # a fan of 8 triangles around the origin in the z = 0 plane; rays straight
+    # down through every shared edge and the shared centre vertex must hit
+    k = 8
+    ang = np.linspace(0, 2 * np.pi, k + 1)
+    rim = np.stack([np.cos(ang), np.sin(ang), np.zeros_like(ang)], 1).astype(np.float32)
+    v = np.concatenate([np.zeros((k, 3), np.float32), rim[:-1], rim[1:]], axis=1)
+    n = np.tile(np.float32([0, 0, 1]), (k, 3))
+    mats = [{"diffuse": (1, 1, 1), "alpha": 1.0, "roughness": 1.0}]
+    o = oracle_mod.Scene(v, n, np.zeros(k, np.int32), mats)
+    pts = [np.float32([0, 0, 0])]
+    for i in range(k):
+        for s in np.linspace(0.05, 0.95, 19):
+            pts.append((rim[i] * np.float32(s)).astype(np.float32))  # on shared spokes
+    P = np.array(pts, np.float32)
+    O = P + np.float32([0, 0, 1])
+    for d in (np.float32([0, 0, -1]), np.float32([0.3, -0.2, -1]), np.float32([1e-3, 0, -1])):
+        D = np.tile(d, (len(P), 1))
+        Oo = P - d  # ray passes exactly through P at t = 1
+        prim, t, _ = o.closest_hit(Oo, D)
+        assert np.all(prim >= 0), np.nonzero(prim < 0)
+
+
+def test_emitter_bvh_invariants(oracle_mod):
+    sc = cornell(32, 16, 8, 8, 1, 1)
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    emits = np.array([m.get("emit", False) for m in sc["materials"]])
+    gids = np.nonzero(emits[sc["mat_idx"]])[0]
+    b = o.bvh(which=1)
+    assert np.array_equal(np.sort(b["prim"]), gids)
+    sub = {"verts": sc["verts"][b["prim"]]}
+    # the subset tree's leaves refer to sorted positions; check it with the subset soup
+    local = dict(b, prim=np.arange(len(gids), dtype=np.uint32))
+    _check_tree(sub, local)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_occlusion_query_equals_closest_hit(oracle_mod, seed):
+    """NEE's emitter-BVH + any-hit query gives the same outcome as one full
+    closest-hit query, for both values of the (stale) hit flag."""
+    sc = cornell(64, 32, 8, 8, 1, 1)
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    rs = np.random.default_rng(seed)
+    n = 20000
+    O = rs.uniform(-0.49, 0.49, size=(n, 3)).astype(np.float32)
+    # half the rays aimed at points of the light quad, half in random directions
+    L = np.stack([rs.uniform(-0.125, 0.125, n), np.full(n, 0.499), rs.uniform(-0.125, 0.125, n)], 1)
+    D = (L - O).astype(np.float32)
+    D[n // 2:] = rs.normal(size=(n - n // 2, 3)).astype(np.float32)
+    D[:100] = 0.0  # zero directions
+    for hb in (0, 1):
+        fast, ref, cnt = o.occlusion(O, D, np.full(n, hb))
+        assert np.array_equal(fast, ref)
+        assert (fast >= 0).sum() > 1000  # lights are found
+        if hb:
+            assert (fast == -4).sum() > 1000 and (fast == -1).sum() > 100

@@ -1,0 +1,116 @@
+"""Pins for the oracle's path loop (shader.cu:102-255) on analytic scenes (no GPU).
+
+* enclosing emitter: every sample is exactly Le (first hit emits, att = 1);
+* empty scene: every sample is 0 after one radiance ray (the miss program);
+* zero bounces: no rays at all, samples are 0;
+* "sky" scene (Lambertian floor under an all-emissive box, one bounce): the
+  first NEE ray always finds the light, so a sample is
+  albedo * Le * c * (c / pi) with c = dot(N, L); for the cube-distributed
+  directions of shoot_ray_hemisphere E[c^2] = 1/3 by symmetry, so the mean
+  is albedo * Le / (3 pi);
+* the NEE loop's stale hit flag (Q1, shader.cu:174-183, 204-206) on
+  scripted occlusion outcomes;
+* zero-direction radiance rays after total internal reflection (Q4)
+  terminate the sample.
+"""
+import math
+
+import numpy as np
+
+from tests.scene_util import cornell
+
+
+def _box(lo, hi, inward):
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    quads = [
+        ((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0), (-1, 0, 0)),
+        ((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1), (1, 0, 0)),
+        ((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -1, 0)),
+        ((x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (0, 1, 0)),
+        ((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0), (0, 0, -1)),
+        ((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1), (0, 0, 1)),
+    ]
+    v, n = [], []
+    for a, b, c, d, nn in quads:
+        nn = np.float32(nn) * (-1 if inward else 1)
+        v += [a + b + c, a + c + d]
+        n += [np.tile(nn, 3), np.tile(nn, 3)]
+    return np.float32(v), np.float32(n)
+
+
+def _render(oracle_mod, v, n, m, mats, eye, look, fov, W, H, spp, bounces, **kw):
+    s = oracle_mod.Scene(v, n, m, mats)
+    U, V, Wv = oracle_mod.camera_uvw(eye, look, fov, W / H)
+    return s.render(W, H, spp, bounces, eye, U, V, Wv, record=True, threads=4, **kw)
+
+
+def test_enclosing_emitter(oracle_mod):
+    v, n = _box((-1, -1, -1), (1, 1, 1), inward=True)
+    Le = (0.3, 0.5, 0.7)
+    mats = [{"emit": True, "emission": Le, "alpha": 1.0}]
+    r = _render(oracle_mod, v, n, np.zeros(12, np.int32), mats, (0, 0, 0), (0, 0, -1), 60,
+                16, 16, 4, 7)
+    assert np.array_equal(r["sample_rgb"], np.broadcast_to(np.float32(Le), r["sample_rgb"].shape))
+    assert np.all(r["sample_bounces"] == 1) and np.all(r["sample_occl"] == 0)
+    assert np.allclose(r["accum"][:, :3], Le, rtol=1e-6)
+
+
+def test_empty_scene_and_zero_bounces(oracle_mod):
+    v = np.zeros((0, 9), np.float32)
+    mats = [{"diffuse": (1, 1, 1), "alpha": 1.0, "roughness": 1.0}]
+    r = _render(oracle_mod, v, v, np.zeros(0, np.int32), mats, (0, 0, 0), (0, 0, -1), 40, 8, 8, 3, 5)
+    assert np.all(r["accum"][:, :3] == 0) and np.all(r["accum"][:, 3] == 1)
+    assert np.all(r["sample_bounces"] == 1) and np.all(r["rad_hits"][:, :, 0] == -1)
+    c = cornell(32, 16, 8, 8, 2, 0)
+    r0 = _render(oracle_mod, c["verts"], c["normals"], c["mat_idx"], c["materials"],
+                 c["camera"]["eye"], c["camera"]["look_at"], 40, 8, 8, 2, 0)
+    assert np.all(r0["sample_bounces"] == 0) and int(r0["counters"][0] + r0["counters"][1]) == 0
+    assert np.all(r0["accum"][:, :3] == 0)
+
+
+def test_sky_one_bounce_expectation(oracle_mod):
+    fv, fn = _box((-10, -0.001, -10), (10, 0, 10), inward=False)  # thin floor slab
+    sv, sn = _box((-20, -1, -20), (20, 20, 20), inward=True)        # emissive sky
+    v = np.concatenate([fv, sv])
+    n = np.concatenate([fn, sn])
+    m = np.int32([0] * 12 + [1] * 12)
+    albedo, Le = np.float32([0.5, 0.7, 0.9]), np.float32([2.0, 2.0, 2.0])
+    mats = [{"diffuse": tuple(albedo), "alpha": 1.0, "roughness": 1.0},
+            {"emit": True, "emission": tuple(Le), "alpha": 1.0}]
+    r = _render(oracle_mod, v, n, m, mats, (0, 1, 0), (0, 0, -1), 10, 64, 64, 32, 1)
+    assert np.all(r["sample_bounces"] == 1)
+    # the first NEE ray finds the sky, except for rare grazing rays whose origin
+    # rounds just below the floor and re-hit it beyond tmin (reference behaviour)
+    occl = r["sample_occl"]
+    assert np.all((occl >= 1) & (occl <= 2)) and (occl == 1).mean() > 0.995
+    x = r["sample_rgb"].reshape(-1, 3).astype(np.float64)
+    want = albedo * Le / (3 * math.pi)
+    err = np.abs(x.mean(0) - want)
+    assert np.all(err < 5 * x.std(0) / math.sqrt(x.shape[0])), (x.mean(0), want)
+    # per-sample value is albedo * Le * c^2 / pi with c in [0, 1]
+    assert np.all(x >= 0) and np.all(x <= albedo * Le / math.pi * (1 + 1e-6))
+
+
+def test_nee_stale_hit_scripted(oracle_mod):
+    f = oracle_mod.nee_scripted
+    assert f([0, 0, 5]) == (3, True, 5)                       # fresh: third ray finds light 5
+    assert f([-1] * 30) == (-30, False, -1)                   # 30 misses: no light
+    assert f([0] * 30) == (-30, False, -1)                    # 30 non-emitters: no light
+    assert f([0], hit_in=True, light_in=7) == (1, True, 7)    # stale: returns light 7 at once
+    assert f([3], hit_in=True, light_in=7) == (1, True, 3)    # a new emitter replaces it
+    assert f([-1, 0, 0, 9], hit_in=True, light_in=7) == (4, True, 9)  # a miss clears the flag
+    assert f([-1] + [0] * 29, hit_in=True, light_in=7) == (-30, False, 7)
+
+
+def test_tir_zero_direction_ends_sample(oracle_mod):
+    c = cornell(32, 16, 64, 64, 8, 7)
+    r = _render(oracle_mod, c["verts"], c["normals"], c["mat_idx"], c["materials"],
+                c["camera"]["eye"], c["camera"]["look_at"], 40, 64, 64, 8, 7)
+    rh, nb = r["rad_hits"], r["sample_bounces"]
+    assert int(r["counters"][6]) > 0  # zero-direction rays occur (glass box)
+    traced = (rh != -3).sum(-1)
+    assert np.array_equal(traced, nb)
+    idx = np.nonzero(rh == -2)
+    for p, s, b in zip(*idx):
+        assert b == nb[p, s] - 1  # the zero-direction ray is the sample's last

@@ -1,0 +1,143 @@
+"""The `.rto` / `.obj` reader of liblisa.so (host code, no GPU) against the
+behaviour of SceneParser (src/LiSA/src/scene_parser.cc) and parse_obj
+(src/LiSA/src/parse_obj.cc), including the quirks of its regular
+expressions: a space is required between `roughness` and `=`, a line-leading
+`n=` is rejected, `1e-3` reads as 1, colours above 1 are divided by 255, the
+first declaration of a duplicated material name wins, comments are removed.
+"""
+import numpy as np
+import pytest
+
+from lisa_amd import scenes
+
+OBJ = """v 0 0 0
+v 1 0 0
+v 0 1 0
+v 1 1 0
+vn 0 0 1
+vn 0 0 -1
+f 1//1 2//1 3//1
+f 2/7/2 4/8/2 3/9/2
+"""
+
+
+def _write(tmp_path, body, obj=OBJ):
+    (tmp_path / "m.obj").write_text(obj)
+    path = tmp_path / "s.rto"
+    path.write_text(body.replace("OBJ", str(tmp_path / "m.obj")))
+    return str(path)
+
+
+BASE = """/* a comment with material fake { color = (9,9,9,9) } */
+material white {
+  color = (1, 1, 1, 1)
+  roughness = 0.25
+}
+material glass {
+  color = (255, 128, 0, 0)
+   n = 1.33
+}
+material lamp {
+  emit = true
+  color = (2, 4, 8, 1)
+}
+material white {
+  color = (0, 0, 0, 1)
+  roughness = 1
+}
+mesh {
+  obj_file = OBJ
+  material = white
+}
+mesh {
+  obj_file = OBJ
+  material = glass
+}
+camera {
+  position = (0, -1.5, 2)
+  look_at = (+0.5, 0, 0)
+  fov = 1e-3
+}
+num_samples = 12
+num_bounces = 3
+width = 40
+height = 30
+output_image = out dir/x.ppm   
+"""
+
+
+def test_parse_base(capi, tmp_path):
+    f = capi.SceneFile(_write(tmp_path, BASE))
+    assert (f.width, f.height, f.num_samples, f.num_bounces) == (40, 30, 12, 3)
+    assert f.output_image == "out dir/x.ppm   "  # the path regex swallows trailing spaces
+    assert f.num_materials == 3  # duplicate `white` skipped
+    w, g, l = f.materials
+    assert w["roughness"] == pytest.approx(0.25) and w["alpha"] == 1.0 and w["diffuse"] == (1, 1, 1)
+    assert g["alpha"] == 0.0 and g["n"] == pytest.approx(1.33)
+    assert g["diffuse"] == pytest.approx((1.0, 128 / 255, 0.0))  # >1 => /255
+    assert l["emit"] and l["emission"] == pytest.approx((2 / 255, 4 / 255, 8 / 255))
+    assert f.fov == 1.0  # `1e-3` matches the float regex as `1`
+    assert np.array_equal(f.eye, np.float32([0, -1.5, 2]))
+    assert np.array_equal(f.look_at, np.float32([0.5, 0, 0]))
+    assert f.num_tris == 4
+    assert np.array_equal(f.mat_idx, [0, 0, 1, 1])
+    assert np.array_equal(f.verts[1], np.float32([1, 0, 0, 1, 1, 0, 0, 1, 0]))
+    assert np.array_equal(f.normals[1], np.float32([0, 0, -1] * 3))
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ("roughness = 0.25", "roughness=0.25"),  # no space before '=' -> rejected
+])
+def test_roughness_needs_space(capi, tmp_path, bad, msg):
+    body = BASE.replace(bad, msg)
+    with pytest.raises(capi.LisaError) as e:
+        capi.SceneFile(_write(tmp_path, body))
+    assert "roughness|refractive index" in str(e.value)
+
+
+def test_line_leading_n_rejected(capi, tmp_path):
+    body = BASE.replace("   n = 1.33", "n = 1.33")
+    with pytest.raises(capi.LisaError) as e:
+        capi.SceneFile(_write(tmp_path, body))
+    assert "material glass" in str(e.value)
+
+
+@pytest.mark.parametrize("drop,msg", [
+    ("width = 40", "Param width not found."),
+    ("output_image = out dir/x.ppm   ", "Param output_image not found."),
+])
+def test_missing_params(capi, tmp_path, drop, msg):
+    with pytest.raises(capi.LisaError) as e:
+        capi.SceneFile(_write(tmp_path, BASE.replace(drop, "")))
+    assert msg in str(e.value) and e.value.status == capi.ERR_PARSE
+
+
+def test_camera_and_mesh_errors(capi, tmp_path):
+    with pytest.raises(capi.LisaError) as e:
+        capi.SceneFile(_write(tmp_path, BASE.replace("fov = 1e-3", "")))
+    assert "Error in camera declaration: no valid fov provided." in str(e.value)
+    with pytest.raises(capi.LisaError) as e:
+        capi.SceneFile(_write(tmp_path, BASE.replace("material = glass", "material = steel")))
+    assert "steel material not found" in str(e.value)
+    with pytest.raises(capi.LisaError) as e:
+        capi.SceneFile(_write(tmp_path, BASE, obj="v 0 0 0\nf 1//1 1//1 1//1\n"))
+    assert e.value.status == capi.ERR_PARSE  # normal index out of range
+    with pytest.raises(capi.LisaError) as e:
+        capi.SceneFile(str(tmp_path / "missing.rto"))
+    assert e.value.status == capi.ERR_IO and "not found" in str(e.value)
+
+
+def test_generated_scene_round_trip(capi, tmp_path):
+    sc = scenes.cornell_small()
+    f = capi.SceneFile(scenes.write_rto(sc, str(tmp_path / "c1"), output_image="c1.ppm"))
+    assert np.array_equal(f.verts, sc["verts"])
+    assert np.array_equal(f.normals, sc["normals"])
+    assert np.array_equal(f.mat_idx, sc["mat_idx"])
+    assert (f.width, f.height, f.num_samples, f.num_bounces) == (256, 256, 16, 7)
+    for got, want in zip(f.materials, sc["materials"]):
+        if want.get("emit"):
+            assert got["emit"] and got["emission"] == pytest.approx(want["emission"])
+        else:
+            assert got["alpha"] == want["alpha"] and got["diffuse"] == pytest.approx(want["diffuse"])
+            key = "n" if want["alpha"] < 1 else "roughness"
+            assert got[key] == pytest.approx(want[key])
