@@ -35,20 +35,25 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose_resources=False):
-    os.makedirs(OUT, exist_ok=True)
-    dev_o = os.path.join(OUT, "lisa_device.o")
-    host_o = os.path.join(OUT, "lisa_host.o")
+def build(verbose_resources=False, defines=(), out_dir=OUT, cli=True):
+    """Compile liblisa.so (and the CLI) into out_dir; `defines` are extra
+    -D tuning macros (LISA_PATH_BLOCK, LISA_STACK_LDS, LISA_PATH_MIN_BLOCKS)."""
+    os.makedirs(out_dir, exist_ok=True)
+    dev_o = os.path.join(out_dir, "lisa_device.o")
+    host_o = os.path.join(out_dir, "lisa_host.o")
+    lib = os.path.join(out_dir, "liblisa.so")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+    extra += [f"-D{d}" for d in defines]
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, *FP_FLAGS, *HIP_FLAGS, *extra, "-c",
           os.path.join(SRC, "lisa_device.hip"), "-o", dev_o])
     _run(["g++", *COMMON, *FP_FLAGS, "-c", os.path.join(SRC, "lisa_host.cpp"), "-o", host_o])
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", dev_o, host_o, "-o", tmp])
-    os.replace(tmp, LIB)
-    _run(["g++", *COMMON, os.path.join(SRC, "lisa_cli.cpp"), "-o", CLI, "-L", OUT, "-llisa",
-          "-Wl,-rpath,$ORIGIN"])
-    return LIB
+    os.replace(tmp, lib)
+    if cli:
+        _run(["g++", *COMMON, os.path.join(SRC, "lisa_cli.cpp"), "-o",
+              os.path.join(out_dir, "lisa"), "-L", out_dir, "-llisa", "-Wl,-rpath,$ORIGIN"])
+    return lib
 
 
 if __name__ == "__main__":

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liblisa.so")
+# LISA_LIBRARY selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("LISA_LIBRARY") or os.path.join(HERE, "_build", "liblisa.so")
 CLI_PATH = os.path.join(HERE, "_build", "lisa")
 
 OK = 0

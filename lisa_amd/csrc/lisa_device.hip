@@ -43,11 +43,26 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 // Per-lane traversal stack: the first kStackLds entries in LDS, the rest in
 // a private spill array.  The depth of a Karras tree over index-augmented
 // 30-bit keys is below 64, so kStackMax entries always suffice.
-constexpr int kStackLds = 32;
+#ifndef LISA_STACK_LDS
+#define LISA_STACK_LDS 32
+#endif
+#ifndef LISA_PATH_BLOCK
+#define LISA_PATH_BLOCK 64
+#endif
+#ifndef LISA_PATH_MIN_BLOCKS
+#define LISA_PATH_MIN_BLOCKS 8
+#endif
+constexpr int kStackLds = LISA_STACK_LDS;
 constexpr int kStackMax = 64;
-constexpr int kPathBlock = 128;  // 2 wave64 per workgroup, 16x8 pixel tile
-constexpr int kTileW = 16, kTileH = 8;
+// Path workgroup: kPathBlock work-items on a kTileW x kTileH pixel tile
+// (64: 8x8, 128: 16x8, 256: 16x16), so a wave covers a compact 8-row or
+// 4-row block of pixels whose camera rays start coherent.
+constexpr int kPathBlock = LISA_PATH_BLOCK;
+constexpr int kTileW = kPathBlock >= 128 ? 16 : 8;
+constexpr int kTileH = kPathBlock / kTileW;
 constexpr int kRayBlock = 256;
+static_assert(kStackLds >= 1 && kStackLds <= kStackMax, "LDS stack depth");
+static_assert(kPathBlock == 64 || kPathBlock == 128 || kPathBlock == 256, "path block size");
 
 struct DevMat {
   float4 diffuse_alpha;  // diffuse.xyz, alpha
@@ -298,7 +313,7 @@ struct PathArgs {
 };
 
 template <bool kRecord, bool kCount>
-__global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
+__global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   uint32_t* my_stk = stk + threadIdx.x;
   int32_t item;
@@ -328,10 +343,9 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
   double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0;
 
   // Per-lane path state (RayState, shader.cu:30-42, plus the NEE loop index).
-  F3 O, D, color, att, N = mk(0.0f, 0.0f, 0.0f), P = mk(0.0f, 0.0f, 0.0f);
-  F3 Din = mk(0.0f, 0.0f, 0.0f);
-  bool hitf = false, need_new = true;
-  int32_t qstage = 0;  // 0 radiance, 1 NEE emitter query, 2 NEE any-hit query
+  F3 O = mk(0.0f, 0.0f, 0.0f), D = O, color = O, att = O, N = O, P = O, Din = O;
+  bool hitf = false;
+  int32_t qstage = 0;  // 0 radiance ray, 1 NEE emitter query, 2 NEE any-hit query
   int32_t light_mat = 0, tries = 0, matE = -1;
   uint32_t primE = 0xFFFFFFFFu;
   float tE = kTmax;
@@ -341,9 +355,30 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
   int32_t occl_cnt = 0;
   const size_t rec_base = (size_t)item * p.spp;
 
-  for (;;) {
-    if (need_new) {
-      if (s == p.spp) break;
+  // accum_color += trace(...) (shader.cu:154) and the per-sample records.
+  auto finish_sample = [&]() {
+    accum = accum + color;
+    if (p.fp64) {
+      ad0 += (double)color.x;
+      ad1 += (double)color.y;
+      ad2 += (double)color.z;
+    }
+    if (kRecord) {
+      if (p.s_rgb) {
+        p.s_rgb[3 * (rec_base + s)] = color.x;
+        p.s_rgb[3 * (rec_base + s) + 1] = color.y;
+        p.s_rgb[3 * (rec_base + s) + 2] = color.z;
+      }
+      if (p.s_b) p.s_b[rec_base + s] = (int32_t)b;
+      if (p.s_o) p.s_o[rec_base + s] = occl_cnt;
+      if (p.s_h) p.s_h[rec_base + s] = hash;
+    }
+    ++s;
+  };
+  // The camera ray of the next sample (shader.cu:147-152); samples with no
+  // bounce finish at once.  False when the pixel has no sample left.
+  auto begin_sample = [&]() -> bool {
+    while (s < p.spp) {
       const float jx = rng(seed);
       const float jy = rng(seed);
       const float dx = (2.0f * fx + jx) / sx - 1.0f;
@@ -355,58 +390,95 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
       hitf = false;
       qstage = 0;
       b = 0;
-      need_new = false;
       if (kRecord) {
         hash = 2166136261u;
         occl_cnt = 0;
         if (p.rad_hits)
           for (int32_t q = 0; q < p.rb; ++q) p.rad_hits[(rec_base + s) * p.rb + q] = -3;
       }
-      if (p.bounces == 0) {
-        need_new = true;
-        accum = accum + color;
-        if (p.fp64) {
-          ad0 += (double)color.x;
-          ad1 += (double)color.y;
-          ad2 += (double)color.z;
-        }
-        if (kRecord) {
-          if (p.s_rgb) {
-            p.s_rgb[3 * (rec_base + s)] = color.x;
-            p.s_rgb[3 * (rec_base + s) + 1] = color.y;
-            p.s_rgb[3 * (rec_base + s) + 2] = color.z;
-          }
-          if (p.s_b) p.s_b[rec_base + s] = 0;
-          if (p.s_o) p.s_o[rec_base + s] = 0;
-          if (p.s_h) p.s_h[rec_base + s] = hash;
-        }
-        ++s;
-        continue;
-      }
+      if (p.bounces > 0) return true;
+      finish_sample();
     }
+    return false;
+  };
+  // End of one shoot_ray_to_light: add its light, sample the BSDF for the
+  // next bounce (shader.cu:251-252), end the sample after the last bounce.
+  auto end_nee = [&](F3 nee) -> bool {
+    color = color + nee * att;
+    D = Lambertian::sample(Din, N, seed, rough);
+    O = P;
+    qstage = 0;
+    if (b == p.bounces) {
+      finish_sample();
+      return begin_sample();
+    }
+    return true;
+  };
+  // One occlusion ray resolved (record `code`): return the light, try again,
+  // or give up after kNeeTries (shader.cu:200-208).
+  auto nee_after = [&](int32_t code) -> bool {
+    if (kRecord) hash = (hash ^ ((uint32_t)code ^ 0x5bd1e995u)) * 16777619u;
+    if (hitf) return end_nee(xyz(sc.mats[light_mat].emission_emit) * Lambertian::eval(N, D));
+    if (++tries == kNeeTries) return end_nee(mk(0.0f, 0.0f, 0.0f));
+    D = hemisphere(N, seed);
+    qstage = 1;
+    return true;
+  };
 
-    // ---- one BVH query per iteration ----
-    // qstage 0: radiance ray, closest hit over all triangles.
-    // qstage 1: NEE occlusion ray, closest emitter (emitter BVH).
-    // qstage 2: the same occlusion ray, any triangle before (tE, primE) --
-    //           or before tmax when no emitter was hit and the hit flag is stale.
-    // A miss and a non-emitting closest hit both leave a false hit flag false,
-    // so with the flag false an occlusion ray that hits no emitter is
-    // resolved after qstage 1 (shader.cu:172-184, 204-206).
+  bool alive = begin_sample();
+  while (alive) {
+    // -- NEE occlusion rays, first query: the closest emitter (emitter BVH).
+    // A miss and a non-emitting closest hit both leave a false hit flag
+    // false (shader.cu:172-184), so with the flag false a ray that hits no
+    // emitter is resolved here; these cheap tries loop without waiting for
+    // the wave's full traversals.
+    while (qstage == 1) {
+      const bool valid = ray_valid(D);
+      if (kCount) {
+        c_occ++;
+        if (!valid) c_inv++;
+      }
+      if (kRecord) occl_cnt++;
+      int32_t k = -1;
+      float t = 0.0f;
+      if (valid && sc.emit.T > 0) {
+        const RayT r = make_ray(O, D);
+        k = closest<kCount, kPathBlock>(sc.emit, r, kTmin, kTmax, 0xFFFFFFFFu, false, t, my_stk, tc);
+      }
+      if (!valid) {  // counts as a ray, misses without traversal
+        const int32_t code = hitf ? -1 : -5;
+        hitf = false;
+        alive = nee_after(code);
+      } else if (k >= 0) {  // an emitter: is anything in front of it?
+        const float4 e0 = sc.emit.tris[3 * (size_t)k], e1 = sc.emit.tris[3 * (size_t)k + 1];
+        tE = t;
+        primE = __float_as_uint(e0.w);
+        matE = (int32_t)__float_as_uint(e1.w);
+        qstage = 2;
+      } else if (!hitf) {
+        alive = nee_after(-5);
+      } else {  // stale hit flag: a miss clears it, any other hit keeps it
+        tE = kTmax;
+        primE = 0xFFFFFFFFu;
+        matE = -1;
+        qstage = 2;
+      }
+      if (!alive) break;
+    }
+    if (!alive) break;
+
+    // -- one full traversal per iteration: the closest hit of a radiance ray
+    // (qstage 0) or the any-hit query of an occlusion ray (qstage 2).
+    const bool valid = ray_valid(D);
     int32_t k = -1;
     float t = 0.0f;
-    const bool valid = ray_valid(D);
-    {
-      const BvhView& bv = qstage == 1 ? sc.emit : sc.all;
-      if (valid && bv.T > 0) {
-        const RayT r = make_ray(O, D);
-        const bool any = qstage == 2;
-        k = closest<kCount, kPathBlock>(bv, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
-                                        any, t, my_stk, tc);
-      }
+    if (valid && sc.all.T > 0) {
+      const RayT r = make_ray(O, D);
+      const bool any = qstage == 2;
+      k = closest<kCount, kPathBlock>(sc.all, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
+                                      any, t, my_stk, tc);
     }
 
-    bool finish = false;
     if (qstage == 0) {
       // radiance: miss (189-194) / closest hit (211-255)
       if (kCount) {
@@ -426,6 +498,7 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
         if (p.rad_hits && (int32_t)b < p.rb) p.rad_hits[(rec_base + s) * p.rb + b] = prim;
       }
       ++b;
+      bool finish = false;
       if (k < 0) {
         color = color + mk(0.0f, 0.0f, 0.0f);
         finish = true;
@@ -457,95 +530,29 @@ __global__ __launch_bounds__(kPathBlock) void k_path(SceneView sc, PathArgs p) {
           }
         }
       }
+      if (finish) {
+        finish_sample();
+        alive = begin_sample();
+      }
     } else {
-      // occlusion: miss (172-175) / closest hit (177-184), then the NEE loop (200-207)
-      bool resolved = false;
-      int32_t code = 0;
-      if (qstage == 1) {
-        if (kCount) {
-          c_occ++;
-          if (!valid) c_inv++;
-        }
-        if (kRecord) occl_cnt++;
-        if (!valid) {  // counts as a ray, misses without traversal
-          code = hitf ? -1 : -5;
-          hitf = false;
-          resolved = true;
-        } else if (k >= 0) {
-          const float4 e0 = sc.emit.tris[3 * (size_t)k], e1 = sc.emit.tris[3 * (size_t)k + 1];
-          tE = t;
-          primE = __float_as_uint(e0.w);
-          matE = (int32_t)__float_as_uint(e1.w);
-          qstage = 2;
-        } else if (!hitf) {
-          code = -5;
-          resolved = true;
+      // occlusion, second query: blocked before (tE, primE)?
+      const bool blocked = k >= 0;
+      int32_t code;
+      if (matE >= 0) {
+        if (!blocked) {
+          light_mat = matE;
+          hitf = true;
+          code = (int32_t)primE;
         } else {
-          tE = kTmax;
-          primE = 0xFFFFFFFFu;
-          matE = -1;
-          qstage = 2;
+          code = hitf ? -4 : -5;
         }
-      } else {  // qstage 2
-        const bool blocked = k >= 0;
-        if (matE >= 0) {
-          if (!blocked) {
-            light_mat = matE;
-            hitf = true;
-            code = (int32_t)primE;
-          } else {
-            code = hitf ? -4 : -5;
-          }
-        } else if (blocked) {
-          code = -4;
-        } else {
-          code = -1;
-          hitf = false;
-        }
-        resolved = true;
+      } else if (blocked) {
+        code = -4;
+      } else {
+        code = -1;
+        hitf = false;
       }
-      if (resolved) {
-        if (kRecord) hash = (hash ^ ((uint32_t)code ^ 0x5bd1e995u)) * 16777619u;
-        bool end_nee = false;
-        F3 nee = mk(0.0f, 0.0f, 0.0f);
-        if (hitf) {
-          nee = xyz(sc.mats[light_mat].emission_emit) * Lambertian::eval(N, D);
-          end_nee = true;
-        } else if (++tries == kNeeTries) {
-          end_nee = true;
-        } else {
-          D = hemisphere(N, seed);
-          qstage = 1;
-        }
-        if (end_nee) {
-          color = color + nee * att;
-          D = Lambertian::sample(Din, N, seed, rough);
-          O = P;
-          qstage = 0;
-          if (b == p.bounces) finish = true;
-        }
-      }
-    }
-
-    if (finish) {
-      accum = accum + color;
-      if (p.fp64) {
-        ad0 += (double)color.x;
-        ad1 += (double)color.y;
-        ad2 += (double)color.z;
-      }
-      if (kRecord) {
-        if (p.s_rgb) {
-          p.s_rgb[3 * (rec_base + s)] = color.x;
-          p.s_rgb[3 * (rec_base + s) + 1] = color.y;
-          p.s_rgb[3 * (rec_base + s) + 2] = color.z;
-        }
-        if (p.s_b) p.s_b[rec_base + s] = (int32_t)b;
-        if (p.s_o) p.s_o[rec_base + s] = occl_cnt;
-        if (p.s_h) p.s_h[rec_base + s] = hash;
-      }
-      ++s;
-      need_new = true;
+      alive = nee_after(code);
     }
   }
 
