@@ -2,9 +2,11 @@
 
 A step is one full render (subframe 0, every sample in one launch, as
 render.cc:133-148) of the 65k-triangle Cornell box at 1024x1024, 256 spp,
-7 bounces, with the scene and LBVH already resident in HBM.
+7 bounces, with the scene and LBVH already resident in HBM.  The
+traversal-bound 5.2M-triangle displaced icosphere (1920x1080, 64 spp,
+12 bounces) is `--workload icosphere`.
 
-    python bench.py [--steps K] [--warmup W] [--spp S] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--spp S] [--no-cpu] [--workload W]
 
 Prints one JSON line.  `value` is rays traced (radiance + occlusion, exact
 counts from the counting kernel, which is bit-identical in work to the
@@ -32,24 +34,38 @@ HBM_PEAK_GBS = 8000.0
 S_NODE, S_TRI, S_SHADE, S_PIXEL = 64, 48, 96, 16
 
 
+# name -> (scene factory, default spp, default CPU-baseline pixel count)
+WORKLOADS = {
+    "cornell65k": ("cornell_65k", 256, 65536),
+    "icosphere": ("displaced_icosphere", 64, 2048),
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-pixels", type=int, default=65536)
+    ap.add_argument("--cpu-pixels", type=int, default=None)
+    ap.add_argument("--workload", default="cornell65k", choices=sorted(WORKLOADS),
+                    help="cornell65k: the headline configuration; icosphere: the "
+                         "traversal-bound 5.2M-triangle scene")
     return ap.parse_args()
 
 
-def traffic_from_profile():
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
-    recorded in profiles/k_path_cornell65k_pmc.json, or None."""
-    path = os.path.join(ROOT, "profiles", "k_path_cornell65k_pmc.json")
+def traffic_from_profile(workload, spp):
+    """HBM bytes per launch of the timed kernel measured by two rocprofv3 --pmc
+    runs (FETCH_SIZE, WRITE_SIZE; scaled by the copy-kernel calibration) and
+    recorded in profiles/k_path_<workload>_pmc.json for the same spp, or None."""
+    path = os.path.join(ROOT, "profiles", f"k_path_{workload}_pmc.json")
     try:
         with open(path) as fh:
-            return float(json.load(fh)["hbm_bytes_per_launch"])
+            rec = json.load(fh)
+        if int(rec["spp"]) != spp:
+            return None
+        return float(rec["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -63,9 +79,11 @@ def main():
         sys.exit(2)
     from lisa_amd import capi, scenes
 
-    sc = scenes.cornell_65k()
+    factory, def_spp, def_cpu_px = WORKLOADS[a.workload]
+    sc = getattr(scenes, factory)()
     W, H, B = sc["width"], sc["height"], sc["bounces"]
-    spp = a.spp
+    spp = a.spp or def_spp
+    cpu_pixels = a.cpu_pixels or def_cpu_px
     cam = sc["camera"]
     U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
     ctx = capi.Context(0)
@@ -103,7 +121,7 @@ def main():
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profile(),
+                "traffic": traffic_from_profile(a.workload, spp),
                 "kernel": "k_path<false,false>", "alg_bytes_per_launch": alg_bytes,
                 "avg_kernel_ms": round(avg_kernel_ms, 3)}
 
@@ -112,14 +130,14 @@ def main():
         from oracle import oracle
         threads = len(os.sched_getaffinity(0))
         o = oracle.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
-        px = scenes.seeded_pixels(W, H, a.cpu_pixels, seed=7)
+        px = scenes.seeded_pixels(W, H, cpu_pixels, seed=7)
         t = time.perf_counter()
         r = o.render(W, H, spp, B, cam["eye"], U, V, Wv, pixels=px, threads=threads)
         dt = time.perf_counter() - t
         crays = int(r["counters"][0] + r["counters"][1])
         cpu = {"value": round(crays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                "kind": "port",
-               "sample": f"{a.cpu_pixels} seeded pixels of the same 1024x1024 render at "
+               "sample": f"{cpu_pixels} seeded pixels of the same {W}x{H} render at "
                          f"{spp} spp x {B} bounces ({crays} rays, {dt:.1f} s, OpenMP over pixels)"}
         print(f"[bench] cpu oracle: {cpu}", file=sys.stderr, flush=True)
 
@@ -127,7 +145,7 @@ def main():
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 2),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
-           "config": {"workload": "cornell_65k plain megakernel", "width": W, "height": H,
+           "config": {"workload": f"{sc['name']} plain megakernel", "width": W, "height": H,
                       "spp": spp, "bounces": B, "triangles": ctx.T, "parallelism": "single GPU"},
            "samples_per_s": round(W * H * spp * a.steps / wall, 1),
            "rays_per_step": rays, "bvh_build_ms": round(bvh_ms, 3),
