@@ -96,6 +96,14 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
                            const int32_t* mat_idx, int32_t T,
                            const lisa_material* mats, int32_t M);
 
+/* New vertex positions (9*T floats) and optionally normals (NULL keeps them)
+ * for the T triangles of the current scene: triangle records are rewritten
+ * in the build's Morton order and both LBVHs are refit bottom-up, keeping
+ * topology and Morton codes (animated / re-posed geometry; cheaper than a
+ * rebuild, box quality degrades with large motion).  No counterpart in the
+ * reference, whose GAS is built once (optix_wrapper.cc:104-106). */
+lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float* normals);
+
 /* Replaces render() -> launchSubframe() -> optixLaunch(w, h, 1)
  * (src/LiSA/src/render.cc:35-58, 133-148).  The float4 accum buffer stays on
  * the device (as OptixParams::accum_buffer); accum_rgba_out (w*h*4 floats)

@@ -48,7 +48,8 @@ def build(verbose_resources=False, defines=(), out_dir=OUT, cli=True):
           os.path.join(SRC, "lisa_device.hip"), "-o", dev_o])
     _run(["g++", *COMMON, *FP_FLAGS, "-c", os.path.join(SRC, "lisa_host.cpp"), "-o", host_o])
     tmp = lib + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", dev_o, host_o, "-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", dev_o, host_o, "-o", tmp,
+          "-lpthread"])
     os.replace(tmp, lib)
     if cli:
         _run(["g++", *COMMON, os.path.join(SRC, "lisa_cli.cpp"), "-o",

@@ -34,7 +34,8 @@ COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
 
 # Every function declared in include/lisa.h.
 EXPORTS = (
-    "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_render",
+    "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_update_vertices",
+    "lisa_render",
     "lisa_copy_accum", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
     "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset", "lisa_copy_probe",
     "lisa_camera_uvw", "lisa_srgb_quantize", "lisa_save_ppm", "lisa_scene_file_load",
@@ -94,6 +95,7 @@ def lib():
         "lisa_destroy": (None, [vp]),
         "lisa_last_error": (C.c_char_p, [vp]),
         "lisa_set_scene": (st, [vp, f32p, f32p, i32p, C.c_int32, C.POINTER(Material), C.c_int32]),
+        "lisa_update_vertices": (st, [vp, f32p, f32p]),
         "lisa_render": (st, [vp, C.POINTER(RenderParams), f32p, u8p]),
         "lisa_copy_accum": (st, [vp, f32p, C.c_int64]),
         "lisa_synchronize": (st, [vp]),
@@ -252,6 +254,13 @@ class Context:
         self.T = int(v.shape[0])
         self._ck(lib().lisa_set_scene(self.h, _p(v, C.c_float), _p(n, C.c_float),
                                       _p(m, C.c_int32), self.T, mats, len(materials)))
+
+    def update_vertices(self, verts, normals=None):
+        v = f32(verts).reshape(-1, 9)
+        if v.shape[0] != self.T:
+            raise ValueError("update_vertices: triangle count differs from the scene")
+        n = None if normals is None else f32(normals).reshape(-1, 9)
+        self._ck(lib().lisa_update_vertices(self.h, _p(v, C.c_float), _p(n, C.c_float)))
 
     def render(self, params, want_accum=True, want_rgba8=False):
         npx = params.width * params.height

@@ -50,7 +50,7 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #define LISA_PATH_BLOCK 64
 #endif
 #ifndef LISA_PATH_MIN_BLOCKS
-#define LISA_PATH_MIN_BLOCKS 8
+#define LISA_PATH_MIN_BLOCKS 1
 #endif
 constexpr int kStackLds = LISA_STACK_LDS;
 constexpr int kStackMax = 64;
@@ -923,6 +923,30 @@ __global__ __launch_bounds__(kBuildBlock) void k_refit(const float4* __restrict_
   }
 }
 
+// New positions (and normals) for sorted triangle k = global prim[k]; the
+// ids in the .w lanes are kept.
+__global__ __launch_bounds__(kBuildBlock) void k_regather(const float* __restrict__ v,
+                                                          const float* __restrict__ nrm,
+                                                          const uint32_t* __restrict__ prim,
+                                                          int32_t T, float4* tris, float4* tn) {
+  const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
+  if (k >= T) return;
+  const uint32_t p = prim[k];
+  const float* a = v + 9 * (size_t)p;
+  for (int j = 0; j < 3; ++j) {
+    float4 r = tris[3 * (size_t)k + j];
+    r.x = a[3 * j];
+    r.y = a[3 * j + 1];
+    r.z = a[3 * j + 2];
+    tris[3 * (size_t)k + j] = r;
+  }
+  if (tn && nrm) {
+    const float* b = nrm + 9 * (size_t)p;
+    for (int j = 0; j < 3; ++j)
+      tn[3 * (size_t)k + j] = make_float4(b[3 * j], b[3 * j + 1], b[3 * j + 2], 0.0f);
+  }
+}
+
 // Plain copy (HBM calibration): float4 grid-stride.
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src,
                                                     float4* __restrict__ dst, int64_t n) {
@@ -951,7 +975,9 @@ struct DevBvh {
   float4* nodes = nullptr;
   float4* tris = nullptr;
   uint32_t* morton = nullptr;
-  uint32_t* prim = nullptr;  // global triangle ids, Morton order
+  uint32_t* prim = nullptr;       // global triangle ids, Morton order
+  int32_t* parent = nullptr;      // parent of each internal node (kept for refits)
+  int32_t* leaf_parent = nullptr; // parent of each leaf
   uint64_t bytes = 0;
 };
 
@@ -995,6 +1021,8 @@ void free_bvh(DevBvh& b) {
   (void)hipFree(b.tris);
   (void)hipFree(b.morton);
   (void)hipFree(b.prim);
+  (void)hipFree(b.parent);
+  (void)hipFree(b.leaf_parent);
   b = DevBvh();
 }
 
@@ -1072,12 +1100,14 @@ lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.tris), tris_b));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.morton), 4 * (size_t)std::max(n, 1)));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.prim), 4 * (size_t)std::max(n, 1)));
-  b.bytes = nodes_b + tris_b + 8 * (size_t)std::max(n, 1);
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.parent), 4 * (size_t)std::max(n, 1)));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.leaf_parent), 4 * (size_t)std::max(n, 1)));
+  b.bytes = nodes_b + tris_b + 16 * (size_t)std::max(n, 1);
   HIP_TRY(ctx, hipMemsetAsync(b.nodes, 0, nodes_b, st));
   if (n == 0) return LISA_OK;
   Scratch S;
   float *d_part = nullptr, *d_bounds = nullptr;
-  int32_t *d_parent = nullptr, *d_leafp = nullptr;
+  int32_t *d_parent = b.parent, *d_leafp = b.leaf_parent;
   uint32_t *d_k2 = nullptr, *d_v2 = nullptr, *d_hist = nullptr, *d_tot = nullptr, *d_flags = nullptr;
   const int32_t n_tiles = (n + kSortTile - 1) / kSortTile;
   const int32_t n_part = std::min<int32_t>(1024, (n + kBuildBlock - 1) / kBuildBlock);
@@ -1087,8 +1117,6 @@ lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   HIP_TRY(ctx, S.alloc(&d_v2, (size_t)n));
   HIP_TRY(ctx, S.alloc(&d_hist, 256 * (size_t)n_tiles));
   HIP_TRY(ctx, S.alloc(&d_tot, 256));
-  HIP_TRY(ctx, S.alloc(&d_parent, (size_t)n));
-  HIP_TRY(ctx, S.alloc(&d_leafp, (size_t)n));
   HIP_TRY(ctx, S.alloc(&d_flags, (size_t)n));
   k_centroid_partial<<<n_part, kBuildBlock, 0, st>>>(d_v, d_gid, n, d_part);
   k_centroid_final<<<1, kBuildBlock, 0, st>>>(d_part, n_part, d_bounds);
@@ -1122,9 +1150,58 @@ lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   return LISA_OK;
 }
 
+// Rewrites b's triangle records from the soup d_v (Morton order kept) and
+// refits every box bottom-up with the stored parent links.
+lisa_status refit_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d_n, float4* nrm) {
+  if (b.T == 0) return LISA_OK;
+  hipStream_t st = ctx->stream;
+  k_regather<<<grid_for(b.T, kBuildBlock), kBuildBlock, 0, st>>>(d_v, d_n, b.prim, b.T, b.tris, nrm);
+  if (b.T == 1) {
+    k_single_root<<<1, 1, 0, st>>>(b.tris, b.nodes);
+  } else {
+    Scratch S;
+    uint32_t* d_flags = nullptr;
+    HIP_TRY(ctx, S.alloc(&d_flags, (size_t)b.T));
+    HIP_TRY(ctx, hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * (size_t)b.T, st));
+    k_refit<<<grid_for(b.T, kBuildBlock), kBuildBlock, 0, st>>>(b.tris, b.T, b.nodes, b.parent,
+                                                               b.leaf_parent, d_flags);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  return LISA_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float* normals) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!ctx->has_scene) return fail(ctx, LISA_ERR_NO_SCENE, "lisa_set_scene has not been called");
+  const int32_t T = ctx->T;
+  if (T == 0) return LISA_OK;
+  if (!verts) return fail(ctx, LISA_ERR_INVALID_ARG, "verts is NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  Scratch S;
+  float *d_v = nullptr, *d_n = nullptr;
+  HIP_TRY(ctx, S.alloc(&d_v, 9 * (size_t)T));
+  HIP_TRY(ctx, hipMemcpy(d_v, verts, 36 * (size_t)T, hipMemcpyHostToDevice));
+  if (normals) {
+    HIP_TRY(ctx, S.alloc(&d_n, 9 * (size_t)T));
+    HIP_TRY(ctx, hipMemcpy(d_n, normals, 36 * (size_t)T, hipMemcpyHostToDevice));
+  }
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  lisa_status rc = refit_bvh(ctx, ctx->all, d_v, d_n, ctx->d_nrm);
+  if (rc == LISA_OK) rc = refit_bvh(ctx, ctx->emit, d_v, nullptr, nullptr);
+  if (rc != LISA_OK) return rc;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
+  return LISA_OK;
+}
 
 lisa_status lisa_create(lisa_ctx** out, int32_t device) {
   if (!out) return fail(nullptr, LISA_ERR_INVALID_ARG, "out is NULL");

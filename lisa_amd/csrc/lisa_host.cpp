@@ -1,7 +1,12 @@
 // liblisa.so host half: the pieces of LiSA's host code that sit on the
 // drop-in boundary -- camera frame, sRGB quantisation, PPM writer and the
 // .rto / .obj scene reader.  Plain C++17, no device code.
+#include <sched.h>
+
+#include <algorithm>
+#include <atomic>
 #include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,6 +17,7 @@
 #include <regex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lisa.h"
@@ -202,52 +208,216 @@ std::string search_param(const std::string& file, const std::string& param, bool
 
 // parse_obj (parse_obj.cc:24-69): `v` / `vn` lines, faces `f a/b/c ...` with the
 // normal index required, first three face vertices, 1-based indices, tokens
-// separated by single spaces; the soup is de-indexed per face.
+// separated by single spaces; the soup is de-indexed per face.  Large files
+// are parsed in parallel: chunks split at line ends parse their v / vn / f
+// lines independently, then faces are resolved against the concatenated
+// vertex lists.  A face may only use vertices defined before it in the file
+// (the reference reads vertices_tmp as it grows); an index outside that
+// range, a short line or a token that is not a number is an error.
+struct ObjChunk {
+  const char* begin = nullptr;
+  const char* end = nullptr;
+  std::vector<float> v, vn;
+  std::vector<long> face;       // 6 per face: (vertex, normal) x 3, zero-based
+  std::vector<long> v_before;   // v lines of this chunk before each face
+  std::vector<long> vn_before;  // vn lines of this chunk before each face
+  std::string error;
+  long error_line = -1;         // line number inside the chunk
+};
+
+// One space-separated token [b, e) as stof / stoi see it.
+bool token_float(const char* b, const char* e, float& out) {
+  char buf[64];
+  std::string big;
+  const size_t n = (size_t)(e - b);
+  const char* z;
+  if (n < sizeof(buf)) {
+    std::memcpy(buf, b, n);
+    buf[n] = 0;
+    z = buf;
+  } else {
+    big.assign(b, e);
+    z = big.c_str();
+  }
+  char* stop = nullptr;
+  errno = 0;
+  out = std::strtof(z, &stop);
+  return stop != z && errno != ERANGE;
+}
+bool token_long(const char* b, const char* e, long& out) {
+  char buf[32];
+  const size_t n = (size_t)(e - b);
+  if (n >= sizeof(buf)) return false;
+  std::memcpy(buf, b, n);
+  buf[n] = 0;
+  char* stop = nullptr;
+  errno = 0;
+  out = std::strtol(buf, &stop, 10);
+  return stop != buf && errno != ERANGE && out >= INT32_MIN && out <= INT32_MAX;
+}
+
+void parse_chunk(ObjChunk& c) {
+  const char* tok_b[8];
+  const char* tok_e[8];
+  long line_no = 0;
+  for (const char* ls = c.begin; ls < c.end; ++line_no) {
+    const char* le = static_cast<const char*>(std::memchr(ls, '\n', (size_t)(c.end - ls)));
+    if (!le) le = c.end;
+    // split on single spaces, keep the first 5 tokens (a trailing `\r` stays
+    // in the last token, as in the reference's split)
+    int nt = 0;
+    const char* t = ls;
+    for (;;) {
+      const char* sp = static_cast<const char*>(std::memchr(t, ' ', (size_t)(le - t)));
+      const char* te = sp ? sp : le;
+      if (nt < 8) {
+        tok_b[nt] = t;
+        tok_e[nt] = te;
+      }
+      ++nt;
+      if (!sp) break;
+      t = sp + 1;
+    }
+    const size_t l0 = (size_t)(tok_e[0] - tok_b[0]);
+    auto fail_line = [&](const char* what) {
+      if (c.error.empty()) {
+        c.error = what;
+        c.error_line = line_no;
+      }
+    };
+    if ((l0 == 1 && tok_b[0][0] == 'v') || (l0 == 2 && tok_b[0][0] == 'v' && tok_b[0][1] == 'n')) {
+      std::vector<float>& dst = l0 == 1 ? c.v : c.vn;
+      if (nt < 4) {
+        fail_line("short vertex line");
+      } else {
+        float x[3];
+        bool ok = true;
+        for (int i = 0; i < 3; ++i) ok = ok && token_float(tok_b[1 + i], tok_e[1 + i], x[i]);
+        if (!ok) fail_line("invalid number");
+        dst.insert(dst.end(), x, x + 3);
+      }
+    } else if (l0 == 1 && tok_b[0][0] == 'f') {
+      if (nt < 4) {
+        fail_line("face with fewer than 3 vertices");
+      } else {
+        long idx[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < 3; ++i) {
+          // split the token on '/': index 0 = vertex, index 2 = normal
+          const char* b = tok_b[1 + i];
+          const char* e = tok_e[1 + i];
+          const char* s1 = static_cast<const char*>(std::memchr(b, '/', (size_t)(e - b)));
+          const char* s2 = s1 ? static_cast<const char*>(std::memchr(s1 + 1, '/', (size_t)(e - s1 - 1))) : nullptr;
+          if (!s2) {
+            fail_line("face vertex without normal index");
+            break;
+          }
+          const char* s3 = static_cast<const char*>(std::memchr(s2 + 1, '/', (size_t)(e - s2 - 1)));
+          long vi, ni;
+          if (!token_long(b, s1, vi) || !token_long(s2 + 1, s3 ? s3 : e, ni)) {
+            fail_line("invalid face index");
+            break;
+          }
+          idx[2 * i] = vi - 1;
+          idx[2 * i + 1] = ni - 1;
+        }
+        c.face.insert(c.face.end(), idx, idx + 6);
+        c.v_before.push_back((long)c.v.size() / 3);
+        c.vn_before.push_back((long)c.vn.size() / 3);
+      }
+    }
+    ls = le + 1;
+  }
+}
+
+int worker_count() {
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) return std::max(1, std::min(CPU_COUNT(&set), 32));
+  return 1;
+}
+
 void parse_obj(const std::string& path, lisa_scene_file& f, int32_t mat_idx) {
   std::printf("Importing %s...\n", path.c_str());
   std::ifstream in(path, std::ios::binary);
   if (!in.is_open()) throw ParseError{LISA_ERR_IO, path + " not found."};
-  std::string data((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-  std::vector<float> vt, nt;
-  std::vector<std::string> tok, idx;
-  auto split = [](const std::string& txt, char ch, std::vector<std::string>& out) {
-    out.clear();
-    size_t start = 0, pos;
-    while ((pos = txt.find(ch, start)) != std::string::npos) {
-      out.push_back(txt.substr(start, pos - start));
-      start = pos + 1;
+  in.seekg(0, std::ios::end);
+  const std::streamoff size = in.tellg();
+  in.seekg(0, std::ios::beg);
+  std::string data((size_t)std::max<std::streamoff>(size, 0), '\0');
+  if (size > 0) in.read(&data[0], size);
+  // chunks of >= 4 MiB split at line ends
+  const int workers = worker_count();
+  const size_t want = std::max<size_t>(1, std::min<size_t>((size_t)workers * 4, data.size() >> 22));
+  std::vector<ObjChunk> chunks(want);
+  size_t pos = 0;
+  for (size_t i = 0; i < want; ++i) {
+    size_t stop = i + 1 == want ? data.size() : std::max(pos, data.size() * (i + 1) / want);
+    if (stop < data.size()) {
+      const size_t nl = data.find('\n', stop);
+      stop = nl == std::string::npos ? data.size() : nl + 1;
     }
-    out.push_back(txt.substr(start));
-  };
-  int nb = 0;
-  size_t ls = 0;
-  std::string line;
-  while (ls < data.size()) {
-    size_t le = data.find('\n', ls);
-    if (le == std::string::npos) le = data.size();
-    line.assign(data, ls, le - ls);
-    ls = le + 1;
-    split(line, ' ', tok);
-    if (tok[0] == "v" || tok[0] == "vn") {
-      if (tok.size() < 4) throw ParseError{LISA_ERR_PARSE, "obj: short vertex line in " + path};
-      std::vector<float>& dst = tok[0] == "v" ? vt : nt;
-      for (int c = 1; c < 4; ++c) dst.push_back(to_float(tok[c]));
-    } else if (tok[0] == "f") {
-      if (tok.size() < 4) throw ParseError{LISA_ERR_PARSE, "obj: face with fewer than 3 vertices in " + path};
-      for (int i = 1; i < 4; ++i) {
-        split(tok[i], '/', idx);
-        if (idx.size() < 3) throw ParseError{LISA_ERR_PARSE, "obj: face vertex without normal index in " + path};
-        const long vi = (long)to_int(idx[0]) - 1, ni = (long)to_int(idx[2]) - 1;
-        if (vi < 0 || 3 * (size_t)vi + 2 >= vt.size() || ni < 0 || 3 * (size_t)ni + 2 >= nt.size())
-          throw ParseError{LISA_ERR_PARSE, "obj: face index out of range in " + path};
-        for (int c = 0; c < 3; ++c) f.vertices.push_back(vt[3 * vi + c]);
-        for (int c = 0; c < 3; ++c) f.normals.push_back(nt[3 * ni + c]);
-      }
-      f.mat_indices.push_back(mat_idx);
-      ++nb;
-    }
+    chunks[i].begin = data.data() + pos;
+    chunks[i].end = data.data() + stop;
+    pos = stop;
   }
-  std::printf("Done. Imported %d triangles.\n", nb);
+  {
+    std::vector<std::thread> th;
+    std::atomic<size_t> next{0};
+    const int nthreads = std::min<int>(workers, (int)chunks.size());
+    for (int w = 0; w < nthreads; ++w)
+      th.emplace_back([&]() {
+        for (size_t i; (i = next.fetch_add(1)) < chunks.size();) parse_chunk(chunks[i]);
+      });
+    for (auto& t : th) t.join();
+  }
+  // concatenate vertex lists; offsets per chunk
+  std::vector<long> v_off(chunks.size() + 1, 0), vn_off(chunks.size() + 1, 0), f_off(chunks.size() + 1, 0);
+  for (size_t i = 0; i < chunks.size(); ++i) {
+    if (!chunks[i].error.empty())
+      throw ParseError{LISA_ERR_PARSE, "obj: " + chunks[i].error + " in " + path};
+    v_off[i + 1] = v_off[i] + (long)chunks[i].v.size() / 3;
+    vn_off[i + 1] = vn_off[i] + (long)chunks[i].vn.size() / 3;
+    f_off[i + 1] = f_off[i] + (long)chunks[i].face.size() / 6;
+  }
+  std::vector<float> vt(3 * (size_t)v_off.back()), nt(3 * (size_t)vn_off.back());
+  for (size_t i = 0; i < chunks.size(); ++i) {
+    std::copy(chunks[i].v.begin(), chunks[i].v.end(), vt.begin() + 3 * v_off[i]);
+    std::copy(chunks[i].vn.begin(), chunks[i].vn.end(), nt.begin() + 3 * vn_off[i]);
+  }
+  const size_t tri0 = f.mat_indices.size();
+  const long nb = f_off.back();
+  f.vertices.resize(9 * (tri0 + (size_t)nb));
+  f.normals.resize(9 * (tri0 + (size_t)nb));
+  f.mat_indices.resize(tri0 + (size_t)nb, mat_idx);
+  std::vector<char> bad(chunks.size(), 0);
+  {
+    std::vector<std::thread> th;
+    std::atomic<size_t> next{0};
+    const int nthreads = std::min<int>(workers, (int)chunks.size());
+    for (int w = 0; w < nthreads; ++w)
+      th.emplace_back([&]() {
+        for (size_t i; (i = next.fetch_add(1)) < chunks.size();) {
+          const ObjChunk& c = chunks[i];
+          const size_t nf = c.face.size() / 6;
+          for (size_t q = 0; q < nf && !bad[i]; ++q) {
+            const long vlim = v_off[i] + c.v_before[q], nlim = vn_off[i] + c.vn_before[q];
+            const size_t out = tri0 + (size_t)f_off[i] + q;
+            for (int j = 0; j < 3; ++j) {
+              const long vi = c.face[6 * q + 2 * j], ni = c.face[6 * q + 2 * j + 1];
+              if (vi < 0 || vi >= vlim || ni < 0 || ni >= nlim) {
+                bad[i] = 1;
+                break;
+              }
+              std::memcpy(&f.vertices[9 * out + 3 * j], &vt[3 * (size_t)vi], 12);
+              std::memcpy(&f.normals[9 * out + 3 * j], &nt[3 * (size_t)ni], 12);
+            }
+          }
+        }
+      });
+    for (auto& t : th) t.join();
+  }
+  for (char b : bad)
+    if (b) throw ParseError{LISA_ERR_PARSE, "obj: face index out of range in " + path};
+  std::printf("Done. Imported %ld triangles.\n", nb);
 }
 
 void parse_rto(const char* path, lisa_scene_file& f) {

@@ -186,6 +186,67 @@ struct Scene {
 
 inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
 
+// Leaf boxes are the triangle bounds widened outward by one ulp: a ray with
+// a zero direction component whose origin lies exactly on a face plane of
+// the tight box would evaluate 0 * inf = NaN in the slab test; with the
+// widened box such an origin lies strictly inside, and an origin on the
+// widened plane is outside all of the box's geometry.
+void leaf_box(const Bvh& s, int64_t k, float* b) {
+  const float* t = &s.tv[9 * k];
+  for (int a = 0; a < 3; ++a) {
+    b[a] = std::nextafter(std::fmin(std::fmin(t[a], t[3 + a]), t[6 + a]), -INFINITY);
+    b[3 + a] = std::nextafter(std::fmax(std::fmax(t[a], t[3 + a]), t[6 + a]), INFINITY);
+  }
+}
+
+// Refit: every child box from the current triangle copies (tv), post-order,
+// child boxes stored in the parent.  Topology (refs) is unchanged.
+void refit(Bvh& s) {
+  if (s.T == 0) return;
+  if (s.T == 1) {
+    leaf_box(s, 0, &s.boxes[0]);
+    return;
+  }
+  struct Frame {
+    uint32_t node;
+    int stage;
+  };
+  std::vector<Frame> st;
+  st.push_back({0, 0});
+  while (!st.empty()) {
+    Frame& f = st.back();
+    const uint32_t nd = f.node;
+    if (f.stage < 2) {
+      const uint32_t r = s.refs[2 * nd + f.stage];
+      f.stage++;
+      if (!(r & kLeaf)) st.push_back({r, 0});
+      continue;
+    }
+    for (int ch = 0; ch < 2; ++ch) {
+      const uint32_t r = s.refs[2 * nd + ch];
+      float* b = &s.boxes[12 * nd + 6 * ch];
+      if (r & kLeaf) {
+        leaf_box(s, r & ~kLeaf, b);
+      } else {
+        const float* cb = &s.boxes[12 * r];
+        for (int a = 0; a < 3; ++a) {
+          b[a] = std::fmin(cb[a], cb[6 + a]);
+          b[3 + a] = std::fmax(cb[3 + a], cb[9 + a]);
+        }
+      }
+    }
+    st.pop_back();
+  }
+}
+
+// New vertex positions for the same triangles: refresh the triangle copies
+// in Morton order and refit; Morton codes and topology stay those of the build.
+void update_bvh(Bvh& s, const std::vector<float>& v) {
+  for (int64_t k = 0; k < s.T; ++k)
+    std::memcpy(&s.tv[9 * k], &v[9 * (size_t)s.prim[k]], 9 * sizeof(float));
+  refit(s);
+}
+
 // Builds `s` over the triangles gid[0..n) of the soup `v` (9 floats each).
 void build_bvh(Bvh& s, const std::vector<float>& v, const std::vector<uint32_t>& gid) {
   const int64_t n = (int64_t)gid.size();
@@ -228,21 +289,9 @@ void build_bvh(Bvh& s, const std::vector<float>& v, const std::vector<uint32_t>&
   const int64_t nI = std::max<int64_t>(n - 1, 1);
   s.refs.assign(2 * nI, kEmpty);
   s.boxes.assign(12 * nI, 0.0f);
-  // Leaf boxes are the triangle bounds widened outward by one ulp: a ray
-  // with a zero direction component whose origin lies exactly on a face
-  // plane of the tight box would evaluate 0 * inf = NaN in the slab test;
-  // with the widened box such an origin lies strictly inside, and an origin
-  // on the widened plane is outside all of the box's geometry.
-  auto leaf_box = [&](int64_t k, float* b) {
-    const float* t = &s.tv[9 * k];
-    for (int a = 0; a < 3; ++a) {
-      b[a] = std::nextafter(std::fmin(std::fmin(t[a], t[3 + a]), t[6 + a]), -INFINITY);
-      b[3 + a] = std::nextafter(std::fmax(std::fmax(t[a], t[3 + a]), t[6 + a]), INFINITY);
-    }
-  };
   if (n == 1) {
     s.refs[0] = kLeaf | 0u;
-    leaf_box(0, &s.boxes[0]);
+    refit(s);
     return;
   }
   // delta(i, j): common prefix length of index-augmented keys, -1 outside.
@@ -272,37 +321,7 @@ void build_bvh(Bvh& s, const std::vector<float>& v, const std::vector<uint32_t>&
     s.refs[2 * i + 0] = (std::min(i, j) == g) ? (kLeaf | (uint32_t)g) : (uint32_t)g;
     s.refs[2 * i + 1] = (std::max(i, j) == g + 1) ? (kLeaf | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
   }
-  // Refit: post-order, child boxes stored in the parent.
-  struct Frame {
-    uint32_t node;
-    int stage;
-  };
-  std::vector<Frame> st;
-  st.push_back({0, 0});
-  while (!st.empty()) {
-    Frame& f = st.back();
-    const uint32_t nd = f.node;
-    if (f.stage < 2) {
-      const uint32_t r = s.refs[2 * nd + f.stage];
-      f.stage++;
-      if (!(r & kLeaf)) st.push_back({r, 0});
-      continue;
-    }
-    for (int ch = 0; ch < 2; ++ch) {
-      const uint32_t r = s.refs[2 * nd + ch];
-      float* b = &s.boxes[12 * nd + 6 * ch];
-      if (r & kLeaf) {
-        leaf_box(r & ~kLeaf, b);
-      } else {
-        const float* cb = &s.boxes[12 * r];
-        for (int a = 0; a < 3; ++a) {
-          b[a] = std::fmin(cb[a], cb[6 + a]);
-          b[3 + a] = std::fmax(cb[3 + a], cb[9 + a]);
-        }
-      }
-    }
-    st.pop_back();
-  }
+  refit(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -826,6 +845,14 @@ orc_scene* orc_scene_create(const float* verts, const float* normals, const int3
   return o;
 }
 void orc_scene_destroy(orc_scene* s) { delete s; }
+
+void orc_scene_update_vertices(orc_scene* o, const float* verts, const float* normals) {
+  Scene& s = o->s;
+  s.v.assign(verts, verts + 9 * (size_t)s.T);
+  if (normals) s.n.assign(normals, normals + 9 * (size_t)s.T);
+  update_bvh(s.all, s.v);
+  update_bvh(s.emit, s.v);
+}
 int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
 
 int32_t orc_num_emitters(const orc_scene* s) { return s->s.emit.T; }

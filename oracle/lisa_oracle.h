@@ -88,6 +88,9 @@ orc_scene* orc_scene_create(const float* verts /*9T*/, const float* normals /*9T
                             const int32_t* mat_idx /*T*/, int32_t T,
                             const orc_material* mats, int32_t M);
 void orc_scene_destroy(orc_scene* s);
+/* New positions (and optionally normals) for the same T triangles: the LBVH
+ * keeps its Morton order and topology and is refit (topology reuse). */
+void orc_scene_update_vertices(orc_scene* s, const float* verts, const float* normals);
 int32_t orc_num_tris(const orc_scene* s);
 int32_t orc_num_emitters(const orc_scene* s);
 /* which = 0: the BVH over all triangles, 1: over the emitting triangles (N of them).

@@ -76,6 +76,7 @@ def lib():
         L.orc_scene_create.restype = C.c_void_p
         L.orc_scene_create.argtypes = [f32p, f32p, i32p, C.c_int32, C.POINTER(Material), C.c_int32]
         L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_scene_update_vertices.argtypes = [C.c_void_p, f32p, f32p]
         L.orc_num_tris.restype = C.c_int32
         L.orc_num_tris.argtypes = [C.c_void_p]
         L.orc_get_bvh.restype = C.c_int
@@ -240,6 +241,14 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+    def update_vertices(self, verts, normals=None):
+        self.verts = f32(verts).reshape(-1, 9)
+        assert self.verts.shape[0] == self.T
+        n = None if normals is None else f32(normals).reshape(-1, 9)
+        if n is not None:
+            self.normals = n
+        lib().orc_scene_update_vertices(self.h, _p(self.verts, C.c_float), _p(n, C.c_float))
 
     def bvh(self, which=0):
         T = self.T if which == 0 else lib().orc_num_emitters(self.h)

@@ -221,3 +221,26 @@ def test_cli_renders_rto(capi, oracle_mod, tmp_path):
     ref = tmp_path / "ref.ppm"
     oracle_mod.save_ppm(str(ref), oracle_mod.srgb_quantize(r["accum"]), 48, 32)
     assert out.read_bytes() == ref.read_bytes()
+
+
+def test_refit_update_bit_exact(capi, oracle_mod, ctx):
+    """lisa_update_vertices (refit with the build's topology) matches the
+    oracle's refit bit for bit, and the path loop stays in parity after it."""
+    sc = cornell(64, 32, 64, 64, 4, 7)
+    o = _pair(capi, oracle_mod, ctx, sc)
+    rs = np.random.default_rng(11)
+    v2 = (sc["verts"] + rs.normal(scale=0.01, size=sc["verts"].shape)).astype(np.float32)
+    n2 = sc["normals"][::-1].copy()
+    ctx.update_vertices(v2, n2)
+    o.update_vertices(v2, n2)
+    for which in (0, 1):
+        g, r = ctx.bvh(which), o.bvh(which)
+        for key in ("morton", "prim", "refs"):
+            assert np.array_equal(g[key], r[key]), (which, key)
+        assert np.array_equal(g["boxes"].view(np.uint32), r["boxes"].view(np.uint32)), which
+    U, V, W = camera(sc, capi)
+    px = scenes.seeded_pixels(64, 64, 1024, seed=3)
+    p = capi.render_params(64, 64, 4, 7, sc["camera"]["eye"], U, V, W)
+    g = ctx.trace_subset(p, px)
+    r = o.render(64, 64, 4, 7, sc["camera"]["eye"], U, V, W, pixels=px, record=True, threads=8)
+    _subset_compare(g, r)

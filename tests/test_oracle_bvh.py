@@ -206,3 +206,25 @@ def test_occlusion_query_equals_closest_hit(oracle_mod, seed):
         assert (fast >= 0).sum() > 1000  # lights are found
         if hb:
             assert (fast == -4).sum() > 1000 and (fast == -1).sum() > 100
+
+
+def test_refit_update_keeps_topology(oracle_mod):
+    """Refit-only update: new positions, same Morton order and hierarchy;
+    boxes are the unions of the new triangle boxes and traversal still
+    equals the brute-force loop."""
+    sc = cornell(32, 16, 8, 8, 1, 1)
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    before = o.bvh()
+    rs = np.random.default_rng(5)
+    v2 = (sc["verts"] + rs.normal(scale=0.02, size=sc["verts"].shape)).astype(np.float32)
+    o.update_vertices(v2)
+    after = o.bvh()
+    assert np.array_equal(before["morton"], after["morton"])
+    assert np.array_equal(before["prim"], after["prim"])
+    assert np.array_equal(before["refs"], after["refs"])
+    assert not np.array_equal(before["boxes"], after["boxes"])
+    _check_tree({"verts": v2}, after)
+    O, D = random_rays(3000, seed=9)
+    p1, t1, _ = o.closest_hit(O, D)
+    p2, t2, _ = o.closest_hit(O, D, brute=True)
+    assert np.array_equal(p1, p2) and np.array_equal(t1.view(np.uint32), t2.view(np.uint32))

@@ -141,3 +141,43 @@ def test_generated_scene_round_trip(capi, tmp_path):
             assert got["alpha"] == want["alpha"] and got["diffuse"] == pytest.approx(want["diffuse"])
             key = "n" if want["alpha"] < 1 else "roughness"
             assert got[key] == pytest.approx(want[key])
+
+
+def test_large_obj_parallel_round_trip(capi, tmp_path):
+    """The 65k-triangle scene (a 13 MB sphere OBJ, parsed in several chunks)
+    reads back bit for bit."""
+    import time
+    sc = scenes.cornell_65k()
+    rto = scenes.write_rto(sc, str(tmp_path / "c2"), output_image="c2.ppm")
+    t = time.time()
+    f = capi.SceneFile(rto)
+    dt = time.time() - t
+    assert f.num_tris == sc["verts"].shape[0]
+    assert np.array_equal(f.verts, sc["verts"]) and np.array_equal(f.normals, sc["normals"])
+    assert np.array_equal(f.mat_idx, sc["mat_idx"])
+    assert dt < 30
+
+
+@pytest.mark.parametrize("obj,ok", [
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//1\n", True),
+    ("v 0 0 0\nv 1 0 0\nvn 0 0 1\nf 1//1 2//1 3//1\nv 0 1 0\n", False),   # vertex defined later
+    ("v 0 0 0\nv  1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//1\n", False),  # double space
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1 2 3\n", False),             # no normal index
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1\n", False),         # two vertices
+    ("v 0 0 0\r\nv 1 0 0\r\nv 0 1 0\r\nvn 0 0 1\r\nf 1//1 2//1 3//1\r\n", True),  # CRLF
+    ("# c\nvt 0 0\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1/1/1 2/1/1 3/1/1 1/1/1\n", True),
+])
+def test_obj_acceptance(capi, tmp_path, obj, ok):
+    body = BASE.replace("""mesh {
+  obj_file = OBJ
+  material = glass
+}
+""", "")
+    if ok:
+        f = capi.SceneFile(_write(tmp_path, body, obj=obj))
+        assert f.num_tris == 1
+        assert np.array_equal(f.verts[0], np.float32([0, 0, 0, 1, 0, 0, 0, 1, 0]))
+    else:
+        with pytest.raises(capi.LisaError) as e:
+            capi.SceneFile(_write(tmp_path, body, obj=obj))
+        assert e.value.status == capi.ERR_PARSE
