@@ -38,6 +38,7 @@ S_NODE, S_TRI, S_SHADE, S_PIXEL = 64, 48, 96, 16
 WORKLOADS = {
     "cornell65k": ("cornell_65k", 256, 65536),
     "icosphere": ("displaced_icosphere", 64, 2048),
+    "batch64": ("batch_scene", 32, 0),
 }
 
 
@@ -79,6 +80,8 @@ def main():
         sys.exit(2)
     from lisa_amd import capi, scenes
 
+    if a.workload == "batch64":
+        return bench_batch(a)
     factory, def_spp, def_cpu_px = WORKLOADS[a.workload]
     sc = getattr(scenes, factory)()
     W, H, B = sc["width"], sc["height"], sc["bounces"]
@@ -152,6 +155,63 @@ def main():
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_batch(a):
+    """64 seeded scenes (10k-2M triangles each, 256^2-1024^2, 32 spp, 8
+    bounces) in one persistent graph-captured launch after autotuning,
+    beside the same scenes rendered one lisa_render launch after another."""
+    from lisa_amd import capi, scenes
+
+    n = 64
+    ctxs, params, rays, tris = [], [], 0, 0
+    for i in range(n):
+        sc = scenes.batch_scene(i)
+        c = capi.Context(0)
+        c.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        cam = sc["camera"]
+        U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"],
+                                   sc["width"] / sc["height"])
+        p = capi.render_params(sc["width"], sc["height"], a.spp or sc["spp"], sc["bounces"],
+                               cam["eye"], U, V, Wv, flags=capi.FLAG_COUNT)
+        c.render(p, want_accum=False)
+        cnt = c.counters()
+        rays += int(cnt[0] + cnt[1])
+        tris += c.T
+        p.flags = 0
+        ctxs.append(c)
+        params.append(p)
+    print(f"[bench] batch: {n} scenes, {tris} triangles, {rays} rays per step",
+          file=sys.stderr, flush=True)
+    # one launch per scene, back to back
+    for c, p in zip(ctxs, params):
+        c.render(p, want_accum=False)
+    t = time.perf_counter()
+    for c, p in zip(ctxs, params):
+        c.render(p, want_accum=False)
+    seq_s = time.perf_counter() - t
+    batch = capi.Batch(ctxs, params)
+    cfg = batch.autotune(tune_spp=2)
+    print(f"[bench] autotuned grid/chunk: {cfg}", file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        batch.render()
+    t0 = time.perf_counter()
+    ms = [batch.render() for _ in range(a.steps)]
+    wall = time.perf_counter() - t0
+    out = {"metric": METRIC, "value": round(rays * a.steps / wall / 1e6, 2), "unit": "Mrays/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(wall * 1e3 / a.steps, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "batch64 wavefront (persistent queue, hipGraph)", "scenes": n,
+                      "triangles": tris, "spp": params[0].spp, "bounces": params[0].bounces,
+                      "grid_blocks": cfg[0], "chunk": cfg[1], "parallelism": "single GPU"},
+           "rays_per_step": rays, "graph_replay_ms": [round(x, 2) for x in ms],
+           "sequential_plain_ms": round(seq_s * 1e3, 2),
+           "speedup_vs_sequential_plain": round(seq_s / (wall / a.steps), 3)}
+    print(json.dumps(out), flush=True)
+    batch.close()
+    for c in ctxs:
+        c.close()
 
 
 if __name__ == "__main__":

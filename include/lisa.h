@@ -124,6 +124,96 @@ lisa_status lisa_get_counters(const lisa_ctx* ctx, uint64_t* counters /* LISA_NC
 /* The hipStream_t the context launches on (for external event timing). */
 void* lisa_get_stream(const lisa_ctx* ctx);
 
+/* ---- batched wavefront form (no counterpart in the reference) ----
+ * n scenes, each already set on its own context (all on one device), are
+ * rendered by ONE persistent launch: the pixels of every scene, in 8x8-tile
+ * order, form a device work queue; each lane takes `chunk` pixels at a time,
+ * renders them completely and takes more until the queue is empty.  Every
+ * pixel is computed exactly as lisa_render computes it (bit-identical
+ * images); the work of short and long paths and of small and large scenes
+ * is balanced over the whole GPU.  The queue reset and the launch are
+ * captured once in a hipGraph and replayed by every lisa_batch_render.
+ * Results land in each context's accum buffer (lisa_copy_accum).  The batch
+ * must be re-created when a context's scene or image size changes. */
+typedef struct lisa_batch lisa_batch;
+typedef struct lisa_batch_config {
+  int32_t grid_blocks; /* persistent workgroups (64 work-items each) */
+  int32_t chunk;       /* pixels taken from the queue per atomic */
+} lisa_batch_config;
+
+lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,
+                              const lisa_render_params* params, int32_t n);
+void lisa_batch_destroy(lisa_batch* b);
+lisa_status lisa_batch_set_config(lisa_batch* b, const lisa_batch_config* cfg);
+lisa_status lisa_batch_get_config(const lisa_batch* b, lisa_batch_config* cfg);
+/* Replays the graph; elapsed_ms (optional) = HIP-event time of the replay. */
+lisa_status lisa_batch_render(lisa_batch* b, float* elapsed_ms);
+/* Autotuner: times every (grid, chunk) candidate on a copy of the batch
+ * with spp reduced to tune_spp and keeps the fastest configuration. */
+lisa_status lisa_batch_autotune(lisa_batch* b, int32_t tune_spp, lisa_batch_config* best_out);
+
+/* ---- learned radiance cache (no counterpart in the reference) ----
+ * A small MLP maps 12 inputs in [0, 1] (position 3, incoming direction 2,
+ * normal 2 as (theta/pi, phi/2pi), albedo 3, roughness 1, dielectric flag 1),
+ * each encoded as (sin, cos)(pi 2^b x) for b = 0..5 (144 features), through
+ * four ReLU layers of width 64 to 3 linear outputs (RGB radiance), no biases.
+ * BF16 weights and activations, FP32 accumulation (MFMA); FP32 master weights
+ * trained with Adam on the relative L2 loss (y - t)^2 / (y^2 + 0.01).
+ * Weights are LISA_CACHE_PARAMS floats: per layer l, W_l^T[out_pad][in_pad]
+ * row-major at offsets 0, 10240, 14336, 18432, 22528 (in_pad = 160, 64, 64,
+ * 64, 64; out_pad = 64, 64, 64, 64, 16); padding entries are zero. */
+typedef struct lisa_cache lisa_cache;
+#define LISA_CACHE_INPUTS 12
+#define LISA_CACHE_PARAMS 23552
+
+lisa_status lisa_cache_create(lisa_cache** out, int32_t device, uint64_t seed, float lr);
+void lisa_cache_destroy(lisa_cache* c);
+const char* lisa_cache_last_error(const lisa_cache* c);
+lisa_status lisa_cache_set_weights(lisa_cache* c, const float* w);
+lisa_status lisa_cache_get_weights(lisa_cache* c, float* w);
+/* out[n][3] = network(feat[n][12]) */
+lisa_status lisa_cache_infer(lisa_cache* c, const float* feat, int64_t n, float* out);
+/* Gradient of the mean relative-L2 loss over n samples (no update). */
+lisa_status lisa_cache_gradients(lisa_cache* c, const float* feat, const float* target, int64_t n,
+                                 float* grad /* LISA_CACHE_PARAMS, optional */,
+                                 float* loss /* optional */);
+/* One Adam step on n samples; loss (optional) is the loss before the step. */
+lisa_status lisa_cache_train(lisa_cache* c, const float* feat, const float* target, int64_t n,
+                             float* loss);
+/* HIP-event time of the last infer / gradients / train call. */
+float lisa_cache_last_ms(const lisa_cache* c);
+
+/* One frame of the cached renderer on ctx's scene (all on one device):
+ *  1. online training: train_paths full-length paths (train_bounces, the
+ *     plain path loop, pixels drawn with `seed`) record every surface hit;
+ *     each hit's suffix radiance (final - gathered so far) / throughput is a
+ *     training row (at most train_max rows); train_steps Adam steps on them;
+ *  2. the frame: p->spp samples per pixel whose radiance ray number
+ *     prefix_bounces + 1 ends at a surface hit by adding
+ *     throughput * max(cache(features), 0) (misses and lights stay exact) --
+ *     or, when fallback_loss > 0 and this frame's training loss exceeds it,
+ *     the plain tracer with p->bounces;
+ *  3. the frame is blended into the context's accum buffer as a running
+ *     mean with weight 1 / (p->subframe + 1) (frame p->subframe). */
+typedef struct lisa_cache_frame_params {
+  uint32_t prefix_bounces; /* radiance rays traced before the cache answers (>= 1) */
+  uint32_t train_paths;    /* training paths per frame (0: no training) */
+  uint32_t train_bounces;  /* bounces of a training path */
+  uint32_t train_max;      /* at most this many training rows (suffixes) per frame; 0: no cap */
+  uint32_t train_steps;    /* Adam steps per frame over the frame's rows */
+  float fallback_loss;     /* > 0: plain frame when the last training loss exceeds it */
+  uint64_t seed;           /* draws the training pixels */
+} lisa_cache_frame_params;
+
+typedef struct lisa_cache_frame_stats {
+  float trace_train_ms, train_ms, render_ms, infer_ms, total_ms;
+  float loss;
+  int32_t queries, train_samples, fell_back;
+} lisa_cache_frame_stats;
+
+lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_render_params* p,
+                               const lisa_cache_frame_params* cp, lisa_cache_frame_stats* stats);
+
 /* ---- debug / parity hooks (no counterpart in the reference: OptiX hides these) ---- */
 
 /* An LBVH as built (which = 0: all T triangles; 1: the N emitting ones):
@@ -144,7 +234,9 @@ lisa_status lisa_trace_rays(lisa_ctx* ctx, const float* O, const float* D, int32
  * (3 floats), radiance rays traced (bounce count), occlusion rays traced,
  * an FNV-1a hash of every ray's hit id, and the hit id of the first
  * record_bounces radiance rays (-1 miss, -2 zero/non-finite direction,
- * -3 not traced).  Any record pointer may be NULL. */
+ * -3 not traced).  Any record pointer may be NULL.  accum_rgba_out gets each
+ * pixel's mean over these spp samples; p->subframe only seeds the RNG
+ * (shader.cu:141), there is no running mean. */
 lisa_status lisa_trace_subset(lisa_ctx* ctx, const lisa_render_params* p,
                               const int32_t* pixels, int32_t n_pixels, float* accum_rgba_out,
                               float* sample_rgb, int32_t* sample_bounces,

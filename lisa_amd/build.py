@@ -40,15 +40,18 @@ def build(verbose_resources=False, defines=(), out_dir=OUT, cli=True):
     -D tuning macros (LISA_PATH_BLOCK, LISA_STACK_LDS, LISA_PATH_MIN_BLOCKS)."""
     os.makedirs(out_dir, exist_ok=True)
     dev_o = os.path.join(out_dir, "lisa_device.o")
+    cache_o = os.path.join(out_dir, "lisa_cache.o")
     host_o = os.path.join(out_dir, "lisa_host.o")
     lib = os.path.join(out_dir, "liblisa.so")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
     extra += [f"-D{d}" for d in defines]
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, *FP_FLAGS, *HIP_FLAGS, *extra, "-c",
           os.path.join(SRC, "lisa_device.hip"), "-o", dev_o])
+    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, *FP_FLAGS, *HIP_FLAGS, *extra, "-c",
+          os.path.join(SRC, "lisa_cache.hip"), "-o", cache_o])
     _run(["g++", *COMMON, *FP_FLAGS, "-c", os.path.join(SRC, "lisa_host.cpp"), "-o", host_o])
     tmp = lib + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", dev_o, host_o, "-o", tmp,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", dev_o, cache_o, host_o, "-o", tmp,
           "-lpthread"])
     os.replace(tmp, lib)
     if cli:

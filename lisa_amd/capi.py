@@ -38,6 +38,11 @@ EXPORTS = (
     "lisa_render",
     "lisa_copy_accum", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
     "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset", "lisa_copy_probe",
+    "lisa_batch_create", "lisa_batch_destroy", "lisa_batch_set_config", "lisa_batch_get_config",
+    "lisa_batch_render", "lisa_batch_autotune",
+    "lisa_cache_create", "lisa_cache_destroy", "lisa_cache_last_error", "lisa_cache_set_weights",
+    "lisa_cache_get_weights", "lisa_cache_infer", "lisa_cache_gradients", "lisa_cache_train",
+    "lisa_cache_last_ms", "lisa_render_cached",
     "lisa_camera_uvw", "lisa_srgb_quantize", "lisa_save_ppm", "lisa_scene_file_load",
     "lisa_scene_file_free", "lisa_scene_file_info", "lisa_scene_file_vertices",
     "lisa_scene_file_normals", "lisa_scene_file_mat_indices", "lisa_scene_file_materials",
@@ -67,6 +72,10 @@ class Stats(C.Structure):
     _fields_ = [("render_ms", C.c_float), ("bvh_build_ms", C.c_float), ("num_tris", C.c_int32),
                 ("num_nodes", C.c_int32), ("num_emitters", C.c_int32),
                 ("device_bytes", C.c_uint64)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("grid_blocks", C.c_int32), ("chunk", C.c_int32)]
 
 
 class SceneInfo(C.Structure):
@@ -107,6 +116,24 @@ def lib():
         "lisa_trace_subset": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, f32p, f32p,
                                    i32p, i32p, u32p, i32p, C.c_int32]),
         "lisa_copy_probe": (st, [vp, C.c_int64, C.c_int32, f32p]),
+        "lisa_batch_create": (st, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.POINTER(RenderParams), C.c_int32]),
+        "lisa_batch_destroy": (None, [vp]),
+        "lisa_batch_set_config": (st, [vp, C.POINTER(BatchConfig)]),
+        "lisa_batch_get_config": (st, [vp, C.POINTER(BatchConfig)]),
+        "lisa_batch_render": (st, [vp, f32p]),
+        "lisa_batch_autotune": (st, [vp, C.c_int32, C.POINTER(BatchConfig)]),
+        "lisa_cache_create": (st, [C.POINTER(C.c_void_p), C.c_int32, C.c_uint64, C.c_float]),
+        "lisa_cache_destroy": (None, [vp]),
+        "lisa_cache_last_error": (C.c_char_p, [vp]),
+        "lisa_cache_set_weights": (st, [vp, f32p]),
+        "lisa_cache_get_weights": (st, [vp, f32p]),
+        "lisa_cache_infer": (st, [vp, f32p, C.c_int64, f32p]),
+        "lisa_cache_gradients": (st, [vp, f32p, f32p, C.c_int64, f32p, f32p]),
+        "lisa_cache_train": (st, [vp, f32p, f32p, C.c_int64, f32p]),
+        "lisa_cache_last_ms": (C.c_float, [vp]),
+        "lisa_render_cached": (st, [vp, vp, C.POINTER(RenderParams), C.POINTER(CacheFrameParams),
+                                    C.POINTER(CacheFrameStats)]),
         "lisa_camera_uvw": (st, [f32p, f32p, C.c_float, C.c_float, f32p, f32p, f32p]),
         "lisa_srgb_quantize": (st, [f32p, C.c_int64, u8p]),
         "lisa_save_ppm": (st, [C.c_char_p, u8p, C.c_int32, C.c_int32]),
@@ -319,15 +346,18 @@ class Context:
                                        _p(prim, C.c_int32), _p(t, C.c_float)))
         return prim, t
 
-    def trace_subset(self, params, pixels, record_bounces=None):
+    def trace_subset(self, params, pixels, record_bounces=None, records=True):
+        """The plain path loop on a pixel list; `records=False` returns the means only."""
         px = np.ascontiguousarray(pixels, dtype=np.int32)
         n, spp = px.shape[0], params.spp
         rb = params.bounces if record_bounces is None else record_bounces
+        if not records:
+            rb = 0
         acc = np.zeros((n, 4), np.float32)
-        srgb = np.zeros((n, spp, 3), np.float32)
-        sb = np.zeros((n, spp), np.int32)
-        so = np.zeros((n, spp), np.int32)
-        sh = np.zeros((n, spp), np.uint32)
+        srgb = np.zeros((n, spp, 3), np.float32) if records else None
+        sb = np.zeros((n, spp), np.int32) if records else None
+        so = np.zeros((n, spp), np.int32) if records else None
+        sh = np.zeros((n, spp), np.uint32) if records else None
         rh = np.zeros((n, spp, max(rb, 1)), np.int32) if rb > 0 else None
         self._ck(lib().lisa_trace_subset(self.h, C.byref(params), _p(px, C.c_int32), n,
                                          _p(acc, C.c_float), _p(srgb, C.c_float),
@@ -335,3 +365,145 @@ class Context:
                                          _p(sh, C.c_uint32), _p(rh, C.c_int32), rb))
         return {"accum": acc, "sample_rgb": srgb, "sample_bounces": sb, "sample_occl": so,
                 "sample_hash": sh, "rad_hits": rh, "counters": self.counters()}
+
+
+class Batch:
+    """The batched wavefront form (lisa_batch_*): several contexts' scenes in
+    one persistent, graph-captured launch."""
+
+    def __init__(self, ctxs, params):
+        self.ctxs = list(ctxs)
+        arr = (C.c_void_p * len(ctxs))(*[c.h.value if isinstance(c.h, C.c_void_p) else c.h
+                                          for c in ctxs])
+        parr = (RenderParams * len(params))(*params)
+        self.params = list(params)
+        h = C.c_void_p()
+        _check(lib().lisa_batch_create(C.byref(h), arr, parr, len(ctxs)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lisa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self):
+        c = BatchConfig()
+        _check(lib().lisa_batch_get_config(self.h, C.byref(c)))
+        return c.grid_blocks, c.chunk
+
+    def set_config(self, grid_blocks, chunk):
+        c = BatchConfig(grid_blocks, chunk)
+        _check(lib().lisa_batch_set_config(self.h, C.byref(c)))
+
+    def render(self):
+        ms = C.c_float()
+        _check(lib().lisa_batch_render(self.h, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, tune_spp=2):
+        c = BatchConfig()
+        _check(lib().lisa_batch_autotune(self.h, tune_spp, C.byref(c)))
+        return c.grid_blocks, c.chunk
+
+    def accum(self, i):
+        p = self.params[i]
+        return self.ctxs[i].copy_accum(p.width * p.height)
+
+
+CACHE_PARAMS = 23552
+CACHE_INPUTS = 12
+
+
+class CacheFrameParams(C.Structure):
+    _fields_ = [("prefix_bounces", C.c_uint32), ("train_paths", C.c_uint32),
+                ("train_bounces", C.c_uint32), ("train_max", C.c_uint32),
+                ("train_steps", C.c_uint32), ("fallback_loss", C.c_float), ("seed", C.c_uint64)]
+
+
+class CacheFrameStats(C.Structure):
+    _fields_ = [("trace_train_ms", C.c_float), ("train_ms", C.c_float), ("render_ms", C.c_float),
+                ("infer_ms", C.c_float), ("total_ms", C.c_float), ("loss", C.c_float),
+                ("queries", C.c_int32), ("train_samples", C.c_int32), ("fell_back", C.c_int32)]
+
+
+def cache_frame_params(prefix_bounces=2, train_paths=8192, train_bounces=12, train_max=1 << 16,
+                       train_steps=1, fallback_loss=0.0, seed=1):
+    return CacheFrameParams(prefix_bounces, train_paths, train_bounces, train_max, train_steps,
+                            fallback_loss, seed)
+
+
+def render_cached(ctx, cache, params, cparams):
+    """One frame of the cached renderer (lisa_render_cached); returns its stats as a dict."""
+    s = CacheFrameStats()
+    st = lib().lisa_render_cached(ctx.h, cache.h, C.byref(params), C.byref(cparams), C.byref(s))
+    if st != OK:
+        msg = lib().lisa_last_error(ctx.h)
+        raise LisaError(st, msg.decode() if msg else "")
+    return {k: getattr(s, k) for k, _ in CacheFrameStats._fields_}
+
+
+class Cache:
+    """The learned radiance cache (lisa_cache_*): MLP weights + Adam state on a device."""
+
+    def __init__(self, device=0, seed=0, lr=1e-3):
+        h = C.c_void_p()
+        _check(lib().lisa_cache_create(C.byref(h), device, seed, lr))
+        self.h = h
+
+    def _ck(self, status):
+        if status != OK:
+            msg = lib().lisa_cache_last_error(self.h)
+            raise LisaError(status, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lisa_cache_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def weights(self):
+        w = np.zeros(CACHE_PARAMS, np.float32)
+        self._ck(lib().lisa_cache_get_weights(self.h, _p(w, C.c_float)))
+        return w
+
+    def set_weights(self, w):
+        w = f32(w)
+        assert w.size == CACHE_PARAMS
+        self._ck(lib().lisa_cache_set_weights(self.h, _p(w, C.c_float)))
+
+    def infer(self, feat):
+        x = f32(feat).reshape(-1, CACHE_INPUTS)
+        out = np.zeros((x.shape[0], 3), np.float32)
+        self._ck(lib().lisa_cache_infer(self.h, _p(x, C.c_float), x.shape[0], _p(out, C.c_float)))
+        return out
+
+    def gradients(self, feat, target):
+        x = f32(feat).reshape(-1, CACHE_INPUTS)
+        t = f32(target).reshape(-1, 3)
+        g = np.zeros(CACHE_PARAMS, np.float32)
+        loss = C.c_float()
+        self._ck(lib().lisa_cache_gradients(self.h, _p(x, C.c_float), _p(t, C.c_float),
+                                            x.shape[0], _p(g, C.c_float), C.byref(loss)))
+        return loss.value, g
+
+    def train(self, feat, target):
+        x = f32(feat).reshape(-1, CACHE_INPUTS)
+        t = f32(target).reshape(-1, 3)
+        loss = C.c_float()
+        self._ck(lib().lisa_cache_train(self.h, _p(x, C.c_float), _p(t, C.c_float), x.shape[0],
+                                        C.byref(loss)))
+        return loss.value
+
+    def last_ms(self):
+        return lib().lisa_cache_last_ms(self.h)

@@ -1,0 +1,686 @@
+// Learned radiance cache: the MLP of lisa_mlp.h on BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) for gfx950.
+//
+//   k_mlp_infer      frequency encoding + 5-layer forward, 16 queries per
+//                    wave-tile, weights resident in LDS
+//   k_mlp_train      the same forward with every activation kept, the
+//                    relative-L2 loss gradient and the backward data pass
+//                    (dZ_l = (dZ_{l+1} W_{l+1}^T) * [A_{l+1} > 0]); writes the
+//                    layer inputs and dZ feature-major for the weight GEMMs
+//   k_mlp_wgrad      dW_l^T = dZ_l * in_l^T over the batch (split-K, atomics)
+//   k_mlp_adam       Adam on the FP32 master weights, BF16 copies for the
+//                    forward (W^T) and backward (W) layouts
+//
+// MFMA 16x16x32 lane layout (64 lanes, q = lane / 16, r = lane % 16):
+//   A (16 x 32): lane holds A[r][8q .. 8q+7]
+//   B (32 x 16): lane holds B[8q .. 8q+7][r]
+//   D (16 x 16): lane holds D[4q .. 4q+3][r]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lisa.h"
+#include "lisa_internal.h"
+#include "lisa_mlp.h"
+
+using namespace lisa::mlp;
+
+namespace {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet), and back.
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+__device__ inline bf16x8 ld8(const uint16_t* p) {  // 16-byte aligned
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
+}
+__device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS row strides (elements): padded by 8 so the 16 rows a wave reads
+// together fall on different banks; 16-byte alignment is kept.
+constexpr int kXStride = kEncPad + 8;  // 168
+constexpr int kHStride = kHidden + 8;  // 72
+constexpr int kGStride = kHidden + 8;  // dZ tiles
+__host__ __device__ constexpr int wt_stride(int l) { return kInDim[l] + 8; }
+__host__ __device__ constexpr int wt_lds_offset(int l) {
+  int o = 0;
+  for (int k = 0; k < l; ++k) o += kOutDim[k] * wt_stride(k);
+  return o;
+}
+constexpr int kWtLds = wt_lds_offset(kLayers);  // 25728 elements
+// Backward layout W_l[in][out_pad2] for layers 1..4 (out padded to a multiple of 32).
+__host__ __device__ constexpr int wb_out(int l) { return l == kLayers - 1 ? 32 : kHidden; }
+__host__ __device__ constexpr int wb_global_offset(int l) {
+  int o = 0;
+  for (int k = 1; k < l; ++k) o += kHidden * wb_out(k);
+  return o;
+}
+constexpr int kWbGlobal = wb_global_offset(kLayers);  // 14336
+__host__ __device__ constexpr int wb_stride(int l) { return wb_out(l) + 8; }
+__host__ __device__ constexpr int wb_lds_offset(int l) {
+  int o = 0;
+  for (int k = 1; k < l; ++k) o += kHidden * wb_stride(k);
+  return o;
+}
+constexpr int kWbLds = wb_lds_offset(kLayers);  // 16384 elements
+
+// Frequency encoding of one input: (sin, cos)(pi 2^b p) for b = 0..5, written
+// at features 12 i + 2 b + {0, 1}.
+__device__ inline void encode_one(float p, uint16_t* row, int i) {
+#pragma unroll
+  for (int b = 0; b < kBands; ++b) {
+    float s, c;
+    sincospif(p * (float)(1 << b), &s, &c);
+    row[kBands * 2 * i + 2 * b] = f2bf(s);
+    row[kBands * 2 * i + 2 * b + 1] = f2bf(c);
+  }
+}
+
+// Copies the forward weights W^T (global, row stride in_pad) into LDS
+// (row stride in_pad + 8), and optionally the backward copies.
+__device__ void load_weights(const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
+                             uint16_t* s_wt, uint16_t* s_wb) {
+  for (int l = 0; l < kLayers; ++l) {
+    const int rows = kOutDim[l], cols = kInDim[l];
+    for (int e = threadIdx.x; e < rows * cols / 8; e += blockDim.x) {
+      const int r = e / (cols / 8), c = (e % (cols / 8)) * 8;
+      *reinterpret_cast<u32x4*>(s_wt + wt_lds_offset(l) + r * wt_stride(l) + c) =
+          *reinterpret_cast<const u32x4*>(wt + kOffset[l] + r * cols + c);
+    }
+  }
+  if (s_wb) {
+    for (int l = 1; l < kLayers; ++l) {
+      const int rows = kHidden, cols = wb_out(l);
+      for (int e = threadIdx.x; e < rows * cols / 8; e += blockDim.x) {
+        const int r = e / (cols / 8), c = (e % (cols / 8)) * 8;
+        *reinterpret_cast<u32x4*>(s_wb + wb_lds_offset(l) + r * wb_stride(l) + c) =
+            *reinterpret_cast<const u32x4*>(wb + wb_global_offset(l) + r * cols + c);
+      }
+    }
+  }
+}
+
+// Encodes 16 samples (rows of `feat`, 12 floats each; rows >= n are zero)
+// into the wave's X tile.
+__device__ inline void encode_tile(const float* __restrict__ feat, int stride, int64_t base,
+                                   int64_t n, uint16_t* X, int lane) {
+  for (int e = lane; e < 16 * kInputs; e += 64) {
+    const int r = e / kInputs, i = e % kInputs;
+    const int64_t m = base + r;
+    const float p = m < n ? feat[m * stride + i] : 0.0f;
+    encode_one(p, X + r * kXStride, i);
+  }
+  for (int e = lane; e < 16 * (kEncPad - kEnc); e += 64) {
+    const int r = e / (kEncPad - kEnc), c = kEnc + e % (kEncPad - kEnc);
+    X[r * kXStride + c] = 0;
+  }
+}
+
+// One layer on a 16-row tile: out = in (16 x K) * W_l (K x out_pad).
+// Hidden layers apply ReLU and store BF16 rows into `out_tile`; the last
+// layer returns the raw FP32 accumulator of the single 16-column tile.
+__device__ inline f32x4 layer_fwd(int l, const uint16_t* in, int in_stride, const uint16_t* s_wt,
+                                  uint16_t* out_tile, int lane) {
+  const int q = lane >> 4, r = lane & 15;
+  const uint16_t* w = s_wt + wt_lds_offset(l);
+  const int K = kInDim[l], ntiles = kOutDim[l] / 16;
+  f32x4 acc_last = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int nt = 0; nt < ntiles; ++nt) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < K / 32; ++ks) {
+      const bf16x8 a = ld8(in + r * in_stride + ks * 32 + 8 * q);
+      const bf16x8 b = ld8(w + (nt * 16 + r) * wt_stride(l) + ks * 32 + 8 * q);
+      acc = mfma(a, b, acc);
+    }
+    if (l < kLayers - 1) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        out_tile[(4 * q + i) * kHStride + nt * 16 + r] = f2bf(fmaxf(acc[i], 0.0f));
+    } else {
+      acc_last = acc;
+    }
+  }
+  return acc_last;
+}
+
+constexpr int kInferWaves = 4;
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __restrict__ feat,
+                                                               int64_t n,
+                                                               const uint16_t* __restrict__ wt,
+                                                               float* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtLds];
+  __shared__ __attribute__((aligned(16))) uint16_t s_x[kInferWaves][16 * kXStride];
+  __shared__ __attribute__((aligned(16))) uint16_t s_h[kInferWaves][2][16 * kHStride];
+  load_weights(wt, nullptr, s_wt, nullptr);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = lane >> 4, r = lane & 15;
+  uint16_t* X = s_x[wave];
+  const int64_t tiles = (n + 15) / 16;
+  for (int64_t t = (int64_t)blockIdx.x * kInferWaves + wave; t < tiles;
+       t += (int64_t)gridDim.x * kInferWaves) {
+    const int64_t base = t * 16;
+    encode_tile(feat, kInputs, base, n, X, lane);
+    __builtin_amdgcn_wave_barrier();
+    const uint16_t* in = X;
+    int in_stride = kXStride;
+    for (int l = 0; l < kLayers - 1; ++l) {
+      uint16_t* o = s_h[wave][l & 1];
+      layer_fwd(l, in, in_stride, s_wt, o, lane);
+      __builtin_amdgcn_wave_barrier();
+      in = o;
+      in_stride = kHStride;
+    }
+    const f32x4 y = layer_fwd(kLayers - 1, in, in_stride, s_wt, nullptr, lane);
+    if (r < kOut)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t m = base + 4 * q + i;
+        if (m < n) out[m * kOut + r] = y[i];
+      }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+constexpr int kTrainWaves = 2;
+
+// Forward + relative-L2 loss + backward data pass for 16 samples per wave
+// tile.  Writes feature-major (row = feature, column = sample, row stride npad):
+//   act_l (l = 0..4): the input of layer l (X, A1..A4), BF16
+//   dz_l  (l = 0..4): dLoss/dZ_l, BF16 (dz_4 has 16 rows, 3 used)
+__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
+    const float* __restrict__ feat, int feat_stride, const float* __restrict__ target, int64_t n,
+    int64_t npad, float inv_count, const uint16_t* __restrict__ wt,
+    const uint16_t* __restrict__ wb, uint16_t* __restrict__ act, uint16_t* __restrict__ dz,
+    float* __restrict__ loss) {
+  __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtLds];
+  __shared__ __attribute__((aligned(16))) uint16_t s_wb[kWbLds];
+  __shared__ __attribute__((aligned(16))) uint16_t s_x[kTrainWaves][16 * kXStride];
+  __shared__ __attribute__((aligned(16))) uint16_t s_a[kTrainWaves][4][16 * kHStride];
+  __shared__ __attribute__((aligned(16))) uint16_t s_g[kTrainWaves][16 * kGStride];
+  load_weights(wt, wb, s_wt, s_wb);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = lane >> 4, r = lane & 15;
+  uint16_t* X = s_x[wave];
+  uint16_t* G = s_g[wave];
+  // feature-major global buffers
+  uint16_t* act_l[kLayers];
+  uint16_t* dz_l[kLayers];
+  {
+    int64_t off = 0;
+    for (int l = 0; l < kLayers; ++l) {
+      act_l[l] = act + off;
+      off += (int64_t)kInDim[l] * npad;
+    }
+    off = 0;
+    for (int l = 0; l < kLayers; ++l) {
+      dz_l[l] = dz + off;
+      off += (int64_t)kOutDim[l] * npad;
+    }
+  }
+  float loss_acc = 0.0f;
+  const int64_t tiles = npad / 16;
+  for (int64_t t = (int64_t)blockIdx.x * kTrainWaves + wave; t < tiles;
+       t += (int64_t)gridDim.x * kTrainWaves) {
+    const int64_t base = t * 16;
+    encode_tile(feat, feat_stride, base, n, X, lane);
+    __builtin_amdgcn_wave_barrier();
+    // forward, keeping A1..A4
+    const uint16_t* in = X;
+    int in_stride = kXStride;
+    for (int l = 0; l < kLayers - 1; ++l) {
+      layer_fwd(l, in, in_stride, s_wt, s_a[wave][l], lane);
+      __builtin_amdgcn_wave_barrier();
+      in = s_a[wave][l];
+      in_stride = kHStride;
+    }
+    const f32x4 y = layer_fwd(kLayers - 1, in, in_stride, s_wt, nullptr, lane);
+    // layer inputs, feature-major: X (160 rows), A1..A4 (64 rows)
+    for (int f = lane; f < kEncPad; f += 64)
+      for (int i = 0; i < 16; ++i) act_l[0][(int64_t)f * npad + base + i] = X[i * kXStride + f];
+    for (int l = 1; l < kLayers; ++l)
+      for (int i = 0; i < 16; ++i)
+        act_l[l][(int64_t)lane * npad + base + i] = s_a[wave][l - 1][i * kHStride + lane];
+    // dLoss/dY for L = mean over samples and channels of (y - t)^2 / (y^2 + 0.01),
+    // the denominator held constant (relative L2)
+    for (int i = 0; i < 4; ++i) {
+      const int64_t m = base + 4 * q + i;
+      float g = 0.0f;
+      if (r < kOut && m < n) {
+        const float tv = target[m * kOut + r];
+        if (isfinite(tv)) {  // non-finite targets mark unused samples
+          const float d = y[i] - tv;
+          const float den = y[i] * y[i] + 0.01f;
+          g = 2.0f * d / den * inv_count;
+          loss_acc += d * d / den * inv_count;
+        }
+      }
+      const uint16_t gb = f2bf(g);
+      G[(4 * q + i) * kGStride + r] = gb;
+      dz_l[kLayers - 1][(int64_t)r * npad + m] = gb;
+    }
+    for (int e = lane; e < 16 * 16; e += 64) G[(e / 16) * kGStride + 16 + e % 16] = 0;  // K pad 16..31
+    __builtin_amdgcn_wave_barrier();
+    // backward data pass: dZ_{l-1} = (dZ_l W_l^T) * [A_l > 0], l = 4 .. 1
+    for (int l = kLayers - 1; l >= 1; --l) {
+      const int K = wb_out(l);
+      const uint16_t* wl = s_wb + wb_lds_offset(l);
+      f32x4 acc[4];
+      for (int nt = 0; nt < 4; ++nt) {
+        acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < K / 32; ++ks) {
+          const bf16x8 a = ld8(G + r * kGStride + ks * 32 + 8 * q);
+          const bf16x8 b = ld8(wl + (nt * 16 + r) * wb_stride(l) + ks * 32 + 8 * q);
+          acc[nt] = mfma(a, b, acc[nt]);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      const uint16_t* A = s_a[wave][l - 1];  // input of layer l = output of layer l-1
+      for (int nt = 0; nt < 4; ++nt)
+        for (int i = 0; i < 4; ++i) {
+          const int row = 4 * q + i, col = nt * 16 + r;
+          const float g = bf2f(A[row * kHStride + col]) > 0.0f ? acc[nt][i] : 0.0f;
+          const uint16_t gb = f2bf(g);
+          G[row * kGStride + col] = gb;
+          dz_l[l - 1][(int64_t)col * npad + base + row] = gb;
+        }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  // wave-reduce the loss, one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) loss_acc += __shfl_xor(loss_acc, o, 64);
+  if (lane == 0) atomicAdd(loss, loss_acc);
+}
+
+// dW_l^T (out_pad x in_pad) += dZ_l (out_pad x npad) * in_l^T (npad x in_pad);
+// one wave per (16 x 16 tile, K split).
+__global__ __launch_bounds__(256) void k_mlp_wgrad(const uint16_t* __restrict__ dz,
+                                                   const uint16_t* __restrict__ in, int64_t npad,
+                                                   int out_pad, int in_pad, int ksplit,
+                                                   float* __restrict__ dwt) {
+  const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int q = lane >> 4, r = lane & 15;
+  const int tiles_in = in_pad / 16, tiles = (out_pad / 16) * tiles_in;
+  if (wave >= tiles * ksplit) return;
+  const int tile = wave / ksplit, split = wave % ksplit;
+  const int ot = tile / tiles_in, it = tile % tiles_in;
+  const int64_t steps = npad / 32;
+  const int64_t s0 = steps * split / ksplit, s1 = steps * (split + 1) / ksplit;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  const uint16_t* arow = dz + (int64_t)(ot * 16 + r) * npad + 8 * q;
+  const uint16_t* brow = in + (int64_t)(it * 16 + r) * npad + 8 * q;
+  for (int64_t s = s0; s < s1; ++s) acc = mfma(ld8(arow + 32 * s), ld8(brow + 32 * s), acc);
+  for (int i = 0; i < 4; ++i)
+    atomicAdd(&dwt[(int64_t)(ot * 16 + 4 * q + i) * in_pad + it * 16 + r], acc[i]);
+}
+
+// Adam (Kingma & Ba) on the FP32 master weights (packed W^T layout); padded
+// entries stay zero.  Writes the BF16 forward (W^T) and backward (W) copies.
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ w, float* __restrict__ m,
+                                                  float* __restrict__ v,
+                                                  const float* __restrict__ g, float lr, float b1,
+                                                  float b2, float eps, float c1, float c2,
+                                                  uint16_t* __restrict__ wt,
+                                                  uint16_t* __restrict__ wb) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= kParams) return;
+  int l = 0;
+  while (p >= kOffset[l + 1]) ++l;
+  const int o = (p - kOffset[l]) / kInDim[l], i = (p - kOffset[l]) % kInDim[l];
+  if (o >= real_out(l) || i >= real_in(l)) return;
+  const float gr = g[p];
+  const float mm = b1 * m[p] + (1.0f - b1) * gr;
+  const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
+  m[p] = mm;
+  v[p] = vv;
+  const float ww = w[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+  w[p] = ww;
+  const uint16_t h = f2bf(ww);
+  wt[p] = h;
+  if (l >= 1) wb[wb_global_offset(l) + i * wb_out(l) + o] = h;
+}
+
+__global__ void k_zero_f32(float* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    p[i] = 0.0f;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI: lisa_cache_*
+// ===========================================================================
+struct lisa_cache {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr, *loss = nullptr;
+  uint16_t *wt = nullptr, *wb = nullptr;
+  float lr = 1e-3f, b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int64_t step = 0;
+  // training scratch (grown on demand)
+  int64_t cap = 0;
+  uint16_t *act = nullptr, *dz = nullptr;
+  float *feat = nullptr, *tgt = nullptr, *out = nullptr;
+  int64_t io_cap = 0;
+  float last_ms = 0.0f;
+};
+
+namespace {
+
+lisa_status cfail(lisa_cache* c, lisa_status code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    lisa::set_global_error(msg);
+  return code;
+}
+
+#define CHIP(c, expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return cfail((c), LISA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// He-uniform initial weights (bound sqrt(6 / fan_in)) from a splitmix64 stream.
+void init_weights(std::vector<float>& w, uint64_t seed) {
+  w.assign(kParams, 0.0f);
+  uint64_t s = seed;
+  auto next = [&]() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) / (double)(1ull << 53);
+  };
+  for (int l = 0; l < kLayers; ++l) {
+    const float bound = std::sqrt(6.0f / (float)real_in(l));
+    for (int o = 0; o < real_out(l); ++o)
+      for (int i = 0; i < real_in(l); ++i)
+        w[kOffset[l] + o * kInDim[l] + i] = (float)((2.0 * next() - 1.0) * bound);
+  }
+}
+
+lisa_status upload_weights(lisa_cache* c, const std::vector<float>& w) {
+  std::vector<uint16_t> wt(kParams), wb(kWbGlobal, 0);
+  for (int l = 0; l < kLayers; ++l)
+    for (int o = 0; o < kOutDim[l]; ++o)
+      for (int i = 0; i < kInDim[l]; ++i) {
+        const int p = kOffset[l] + o * kInDim[l] + i;
+        const bool real = o < real_out(l) && i < real_in(l);
+        const float x = real ? w[p] : 0.0f;
+        wt[p] = f2bf(x);
+        if (l >= 1) wb[wb_global_offset(l) + i * wb_out(l) + o] = f2bf(x);
+      }
+  std::vector<float> wm(w);
+  for (int l = 0; l < kLayers; ++l)
+    for (int o = 0; o < kOutDim[l]; ++o)
+      for (int i = 0; i < kInDim[l]; ++i)
+        if (!(o < real_out(l) && i < real_in(l))) wm[kOffset[l] + o * kInDim[l] + i] = 0.0f;
+  CHIP(c, hipMemcpy(c->w, wm.data(), 4 * kParams, hipMemcpyHostToDevice));
+  CHIP(c, hipMemcpy(c->wt, wt.data(), 2 * kParams, hipMemcpyHostToDevice));
+  CHIP(c, hipMemcpy(c->wb, wb.data(), 2 * kWbGlobal, hipMemcpyHostToDevice));
+  return LISA_OK;
+}
+
+lisa_status ensure_io(lisa_cache* c, int64_t n) {
+  if (n <= c->io_cap) return LISA_OK;
+  (void)hipFree(c->feat);
+  (void)hipFree(c->tgt);
+  (void)hipFree(c->out);
+  c->feat = c->tgt = c->out = nullptr;
+  c->io_cap = 0;
+  const int64_t cap = std::max<int64_t>(n, 1 << 16);
+  CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->feat), 4 * kInputs * (size_t)cap));
+  CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->tgt), 4 * kOut * (size_t)cap));
+  CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->out), 4 * kOut * (size_t)cap));
+  c->io_cap = cap;
+  return LISA_OK;
+}
+
+lisa_status ensure_train(lisa_cache* c, int64_t npad) {
+  if (npad <= c->cap) return LISA_OK;
+  (void)hipFree(c->act);
+  (void)hipFree(c->dz);
+  c->act = c->dz = nullptr;
+  c->cap = 0;
+  int64_t rows_act = 0, rows_dz = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    rows_act += kInDim[l];
+    rows_dz += kOutDim[l];
+  }
+  CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->act), 2 * (size_t)rows_act * npad));
+  CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->dz), 2 * (size_t)rows_dz * npad));
+  c->cap = npad;
+  return LISA_OK;
+}
+
+// Gradients of the mean relative-L2 loss over n device-resident samples
+// into c->g (zeroed first); loss into c->loss.
+lisa_status grads_device(lisa_cache* c, const float* d_feat, int feat_stride, const float* d_tgt,
+                         int64_t n) {
+  const int64_t npad = (n + 31) / 32 * 32;
+  lisa_status rc = ensure_train(c, npad);
+  if (rc != LISA_OK) return rc;
+  hipStream_t st = c->stream;
+  k_zero_f32<<<64, 256, 0, st>>>(c->g, kParams);
+  k_zero_f32<<<1, 256, 0, st>>>(c->loss, 1);
+  const int64_t tiles = npad / 16;
+  const int blocks = (int)std::min<int64_t>((tiles + kTrainWaves - 1) / kTrainWaves, 2048);
+  k_mlp_train<<<blocks, 64 * kTrainWaves, 0, st>>>(d_feat, feat_stride, d_tgt, n, npad,
+                                                    1.0f / (3.0f * (float)n), c->wt, c->wb, c->act,
+                                                    c->dz, c->loss);
+  int64_t off_a = 0, off_d = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    const int tiles_l = (kOutDim[l] / 16) * (kInDim[l] / 16);
+    const int ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(npad / 32 / 16, 64));
+    const int waves = tiles_l * ksplit;
+    k_mlp_wgrad<<<(waves + 3) / 4, 256, 0, st>>>(c->dz + off_d, c->act + off_a, npad, kOutDim[l],
+                                                 kInDim[l], ksplit, c->g + kOffset[l]);
+    off_a += (int64_t)kInDim[l] * npad;
+    off_d += (int64_t)kOutDim[l] * npad;
+  }
+  CHIP(c, hipGetLastError());
+  return LISA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lisa_status lisa_cache_create(lisa_cache** out, int32_t device, uint64_t seed, float lr) {
+  if (!out) return cfail(nullptr, LISA_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (!(lr > 0.0f)) return cfail(nullptr, LISA_ERR_INVALID_ARG, "lr must be > 0");
+  lisa_cache* c = new lisa_cache;
+  c->device = device;
+  c->lr = lr;
+  hipError_t e;
+  if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&c->stream)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->w), 4 * kParams)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->m), 4 * kParams)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->v), 4 * kParams)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->g), 4 * kParams)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->loss), 4)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->wt), 2 * kParams)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->wb), 2 * kWbGlobal)) != hipSuccess ||
+      (e = hipMemset(c->m, 0, 4 * kParams)) != hipSuccess ||
+      (e = hipMemset(c->v, 0, 4 * kParams)) != hipSuccess) {
+    const std::string msg = std::string("lisa_cache_create: ") + hipGetErrorString(e);
+    lisa_cache_destroy(c);
+    return cfail(nullptr, LISA_ERR_HIP, msg);
+  }
+  std::vector<float> w;
+  init_weights(w, seed);
+  lisa_status rc = upload_weights(c, w);
+  if (rc != LISA_OK) {
+    const std::string msg = c->err;
+    lisa_cache_destroy(c);
+    return cfail(nullptr, rc, msg);
+  }
+  *out = c;
+  return LISA_OK;
+}
+
+void lisa_cache_destroy(lisa_cache* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)c->w, (void*)c->m, (void*)c->v, (void*)c->g, (void*)c->loss, (void*)c->wt,
+                  (void*)c->wb, (void*)c->act, (void*)c->dz, (void*)c->feat, (void*)c->tgt,
+                  (void*)c->out})
+    (void)hipFree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* lisa_cache_last_error(const lisa_cache* c) { return c ? c->err.c_str() : lisa::global_error(); }
+
+lisa_status lisa_cache_set_weights(lisa_cache* c, const float* w) {
+  if (!c || !w) return cfail(c, LISA_ERR_INVALID_ARG, "NULL argument");
+  CHIP(c, hipSetDevice(c->device));
+  std::vector<float> v(w, w + kParams);
+  lisa_status rc = upload_weights(c, v);
+  if (rc != LISA_OK) return rc;
+  CHIP(c, hipMemset(c->m, 0, 4 * kParams));
+  CHIP(c, hipMemset(c->v, 0, 4 * kParams));
+  c->step = 0;
+  return LISA_OK;
+}
+
+lisa_status lisa_cache_get_weights(lisa_cache* c, float* w) {
+  if (!c || !w) return cfail(c, LISA_ERR_INVALID_ARG, "NULL argument");
+  CHIP(c, hipSetDevice(c->device));
+  CHIP(c, hipStreamSynchronize(c->stream));
+  CHIP(c, hipMemcpy(w, c->w, 4 * kParams, hipMemcpyDeviceToHost));
+  return LISA_OK;
+}
+
+lisa_status lisa_cache_infer(lisa_cache* c, const float* feat, int64_t n, float* out) {
+  if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
+  if (n < 0 || (n > 0 && (!feat || !out))) return cfail(c, LISA_ERR_INVALID_ARG, "bad arrays");
+  if (n == 0) return LISA_OK;
+  CHIP(c, hipSetDevice(c->device));
+  lisa_status rc = ensure_io(c, n);
+  if (rc != LISA_OK) return rc;
+  CHIP(c, hipMemcpy(c->feat, feat, 4 * kInputs * (size_t)n, hipMemcpyHostToDevice));
+  const int64_t tiles = (n + 15) / 16;
+  const int blocks = (int)std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, 4096);
+  CHIP(c, hipEventRecord(c->ev0, c->stream));
+  k_mlp_infer<<<blocks, 64 * kInferWaves, 0, c->stream>>>(c->feat, n, c->wt, c->out);
+  CHIP(c, hipGetLastError());
+  CHIP(c, hipEventRecord(c->ev1, c->stream));
+  CHIP(c, hipStreamSynchronize(c->stream));
+  CHIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  CHIP(c, hipMemcpy(out, c->out, 4 * kOut * (size_t)n, hipMemcpyDeviceToHost));
+  return LISA_OK;
+}
+
+lisa_status lisa_cache_gradients(lisa_cache* c, const float* feat, const float* target, int64_t n,
+                                 float* grad, float* loss) {
+  if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
+  if (n <= 0 || !feat || !target) return cfail(c, LISA_ERR_INVALID_ARG, "bad arrays");
+  CHIP(c, hipSetDevice(c->device));
+  lisa_status rc = ensure_io(c, n);
+  if (rc != LISA_OK) return rc;
+  CHIP(c, hipMemcpy(c->feat, feat, 4 * kInputs * (size_t)n, hipMemcpyHostToDevice));
+  CHIP(c, hipMemcpy(c->tgt, target, 4 * kOut * (size_t)n, hipMemcpyHostToDevice));
+  rc = grads_device(c, c->feat, kInputs, c->tgt, n);
+  if (rc != LISA_OK) return rc;
+  CHIP(c, hipStreamSynchronize(c->stream));
+  if (grad) CHIP(c, hipMemcpy(grad, c->g, 4 * kParams, hipMemcpyDeviceToHost));
+  if (loss) CHIP(c, hipMemcpy(loss, c->loss, 4, hipMemcpyDeviceToHost));
+  return LISA_OK;
+}
+
+}  // extern "C"
+
+// One Adam step on n device-resident samples (features n x 12, targets n x 3).
+lisa_status lisa::cache_train_device(lisa_cache* c, const float* d_feat, int feat_stride,
+                                     const float* d_tgt, int64_t n, float* loss) {
+  if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
+  if (n <= 0) return cfail(c, LISA_ERR_INVALID_ARG, "n must be > 0");
+  CHIP(c, hipSetDevice(c->device));
+  CHIP(c, hipEventRecord(c->ev0, c->stream));
+  lisa_status rc = grads_device(c, d_feat, feat_stride, d_tgt, n);
+  if (rc != LISA_OK) return rc;
+  ++c->step;
+  const float c1 = 1.0f - std::pow(c->b1, (float)c->step), c2 = 1.0f - std::pow(c->b2, (float)c->step);
+  k_mlp_adam<<<(kParams + 255) / 256, 256, 0, c->stream>>>(c->w, c->m, c->v, c->g, c->lr, c->b1,
+                                                          c->b2, c->eps, c1, c2, c->wt, c->wb);
+  CHIP(c, hipGetLastError());
+  CHIP(c, hipEventRecord(c->ev1, c->stream));
+  if (loss) {
+    CHIP(c, hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  CHIP(c, hipStreamSynchronize(c->stream));
+  CHIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  return LISA_OK;
+}
+
+extern "C" {
+
+lisa_status lisa_cache_train(lisa_cache* c, const float* feat, const float* target, int64_t n,
+                             float* loss) {
+  if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
+  if (n <= 0 || !feat || !target) return cfail(c, LISA_ERR_INVALID_ARG, "bad arrays");
+  CHIP(c, hipSetDevice(c->device));
+  lisa_status rc = ensure_io(c, n);
+  if (rc != LISA_OK) return rc;
+  CHIP(c, hipMemcpy(c->feat, feat, 4 * kInputs * (size_t)n, hipMemcpyHostToDevice));
+  CHIP(c, hipMemcpy(c->tgt, target, 4 * kOut * (size_t)n, hipMemcpyHostToDevice));
+  return lisa::cache_train_device(c, c->feat, kInputs, c->tgt, n, loss);
+}
+
+float lisa_cache_last_ms(const lisa_cache* c) { return c ? c->last_ms : 0.0f; }
+
+}  // extern "C"
+
+// Forward pass on n device-resident feature rows (12 floats) into d_out (3 per row).
+lisa_status lisa::cache_infer_device(lisa_cache* c, const float* d_feat, int64_t n, float* d_out) {
+  if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
+  if (n <= 0) return LISA_OK;
+  CHIP(c, hipSetDevice(c->device));
+  const int64_t tiles = (n + 15) / 16;
+  const int blocks = (int)std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, 4096);
+  CHIP(c, hipEventRecord(c->ev0, c->stream));
+  k_mlp_infer<<<blocks, 64 * kInferWaves, 0, c->stream>>>(d_feat, n, c->wt, d_out);
+  CHIP(c, hipGetLastError());
+  CHIP(c, hipEventRecord(c->ev1, c->stream));
+  CHIP(c, hipStreamSynchronize(c->stream));
+  CHIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  return LISA_OK;
+}
+
+extern "C" {
+
+}  // extern "C"

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -310,33 +311,76 @@ struct PathArgs {
   int32_t* rad_hits;
   int32_t rb;
   unsigned long long* counters;
+  // learned radiance cache (path_lane's kMode 1 and 2)
+  uint32_t cache_at;  // kMode 1: this radiance ray's surface hit queries the cache
+  float* q_feat;      // kMode 1: 12 features, 3 throughput, pixel item per query
+  float* q_att;
+  int32_t* q_item;
+  uint32_t* q_count;
+  uint32_t q_cap;
+  float* t_rec;       // kMode 2: 18 floats per vertex slot (12 features, 3 throughput, 3 radiance so far)
+  uint32_t t_slots;   // kMode 2: slots per training path
+  float* t_feat;      // kMode 2: compact training rows, 12 features each
+  float* t_tgt;       // kMode 2: 3 target floats per row
+  uint32_t* t_count;  // kMode 2: rows appended (may pass t_cap; rows past it are dropped)
+  uint32_t t_cap;
+  F3 bmin, binv;      // position normalisation for the features
 };
 
-template <bool kRecord, bool kCount>
-__global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
-  __shared__ uint32_t stk[kStackLds * kPathBlock];
-  uint32_t* my_stk = stk + threadIdx.x;
-  int32_t item;
+// Cache features of a surface hit (lisa.h, lisa_cache_*): position in the
+// scene box, incoming direction and shading normal as (theta/pi, phi/2pi),
+// albedo, roughness, dielectric flag -- each in [0, 1].
+__device__ __forceinline__ void cache_features(const PathArgs& p, F3 P, F3 D, F3 N, const DevMat& m,
+                                               float* f) {
+  f[0] = fminf(fmaxf((P.x - p.bmin.x) * p.binv.x, 0.0f), 1.0f);
+  f[1] = fminf(fmaxf((P.y - p.bmin.y) * p.binv.y, 0.0f), 1.0f);
+  f[2] = fminf(fmaxf((P.z - p.bmin.z) * p.binv.z, 0.0f), 1.0f);
+  const float dl = sqrtf(dot(D, D));
+  const float dy = dl > 0.0f ? D.y / dl : 0.0f;
+  f[3] = acosf(fminf(fmaxf(dy, -1.0f), 1.0f)) * 0.31830988618f;
+  f[4] = (atan2f(D.z, D.x) + 3.14159265359f) * 0.15915494309f;
+  f[5] = acosf(fminf(fmaxf(N.y, -1.0f), 1.0f)) * 0.31830988618f;
+  f[6] = (atan2f(N.z, N.x) + 3.14159265359f) * 0.15915494309f;
+  const bool diel = m.diffuse_alpha.w < 1.0f;
+  f[7] = fminf(fmaxf(m.diffuse_alpha.x, 0.0f), 1.0f);
+  f[8] = fminf(fmaxf(m.diffuse_alpha.y, 0.0f), 1.0f);
+  f[9] = fminf(fmaxf(m.diffuse_alpha.z, 0.0f), 1.0f);
+  f[10] = diel ? 0.0f : fminf(fmaxf(m.rough_n.x, 0.0f), 1.0f);
+  f[11] = diel ? 1.0f : 0.0f;
+}
+
+// A pixel of work for one lane: image coordinates and the index of its
+// result in the launch's accum / record arrays.
+struct Job {
   uint32_t x, y;
-  if (p.pixels) {
-    item = blockIdx.x * kPathBlock + threadIdx.x;
-    if (item >= p.n_items) return;
-    const uint32_t pix = (uint32_t)p.pixels[item];
-    x = pix % p.width;
-    y = pix / p.width;
-  } else {
-    const uint32_t tx = blockIdx.x % p.tiles_x, ty = blockIdx.x / p.tiles_x;
-    x = tx * kTileW + threadIdx.x % kTileW;
-    y = ty * kTileH + threadIdx.x / kTileW;
-    if (x >= p.width || y >= p.height) return;
-    item = (int32_t)(y * p.width + x);
-  }
+  int32_t item;
+};
+
+// The per-lane path loop over the pixels a source hands out.  `src.sc` and
+// `src.p` describe the scene and launch of the current pixel; `src.next(j)`
+// gives the lane its next pixel (false: no work left).
+// kMode 0: the plain path loop.  kMode 1: the radiance ray numbered
+// p.cache_at ends at a surface hit by queuing a cache query (features, the
+// path throughput, the pixel); misses and emitters are handled exactly.
+// kMode 2: every surface hit of the (full-length) path records its features,
+// throughput and the radiance gathered so far; at the end of the sample the
+// suffix radiance (final - so far) / throughput of each vertex becomes a
+// training row.  Without accum_prev the launch writes its own mean (no
+// running mean over subframes; the subframe then only picks the RNG stream).
+template <bool kRecord, bool kCount, class Src, int kMode = 0>
+__device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
+  Job job;
+  if (!src.next(job)) return;
+  const SceneView& sc = src.sc;
+  const PathArgs& p = src.p;
+  uint32_t x = job.x, y = job.y;
+  int32_t item = job.item;
 
   uint32_t seed = tea16(y * p.width + x, p.subframe);  // shader.cu:141
-  const float fx = (float)x, fy = (float)y;
-  const float sx = (float)p.width, sy = (float)p.height;
+  float fx = (float)x, fy = (float)y;
+  float sx = (float)p.width, sy = (float)p.height;
 
-  uint64_t c_rad = 0, c_occ = 0, c_rhit = 0, c_inv = 0;
+  uint64_t c_rad = 0, c_occ = 0, c_rhit = 0, c_inv = 0, c_smp = 0, c_px = 0;
   TravCount tc;
 
   F3 accum = mk(0.0f, 0.0f, 0.0f);
@@ -353,10 +397,27 @@ __global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(Scene
   uint32_t b = 0, s = 0;
   uint32_t hash = 0;
   int32_t occl_cnt = 0;
-  const size_t rec_base = (size_t)item * p.spp;
+  size_t rec_base = (size_t)item * p.spp;
 
+  uint32_t t_used = 0;  // kMode 2: vertex slots recorded in this sample
   // accum_color += trace(...) (shader.cu:154) and the per-sample records.
   auto finish_sample = [&]() {
+    if (kMode == 2 && t_used > 0) {
+      // one reservation per path; suffix radiance per unit throughput
+      const uint32_t row0 = atomicAdd(p.t_count, t_used);
+      const size_t base = (size_t)item * p.t_slots;
+      for (uint32_t v = 0; v < t_used && row0 + v < p.t_cap; ++v) {
+        const float* r = p.t_rec + 18 * (base + v);
+        float* f = p.t_feat + 12 * (size_t)(row0 + v);
+        for (int q = 0; q < 12; ++q) f[q] = r[q];
+        float* tg = p.t_tgt + 3 * (size_t)(row0 + v);
+        tg[0] = (color.x - r[15]) / r[12];
+        tg[1] = (color.y - r[16]) / r[13];
+        tg[2] = (color.z - r[17]) / r[14];
+      }
+      t_used = 0;
+    }
+    if (kCount) c_smp++;
     accum = accum + color;
     if (p.fp64) {
       ad0 += (double)color.x;
@@ -425,8 +486,44 @@ __global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(Scene
     return true;
   };
 
+  // shader.cu:156-166: the pixel's mean (and running mean over subframes).
+  auto write_pixel = [&]() {
+    F3 out;
+    if (p.fp64) {
+      const double inv = 1.0 / (double)p.spp;
+      out = mk((float)(ad0 * inv), (float)(ad1 * inv), (float)(ad2 * inv));
+    } else {
+      const float inv = 1.0f / (float)p.spp;
+      out = accum * inv;
+    }
+    if (kMode == 0 && p.subframe > 0 && p.accum_prev) {
+      const float a = 1.0f / (float)(p.subframe + 1);
+      out = lerp(xyz(p.accum_prev[item]), out, a);
+    }
+    p.accum[item] = make_float4(out.x, out.y, out.z, 1.0f);
+    if (kCount) c_px++;
+  };
+
   bool alive = begin_sample();
-  while (alive) {
+  for (;;) {
+    if (!alive) {
+      write_pixel();
+      if (!src.next(job)) break;
+      x = job.x;
+      y = job.y;
+      item = job.item;
+      seed = tea16(y * p.width + x, p.subframe);
+      fx = (float)x;
+      fy = (float)y;
+      sx = (float)p.width;
+      sy = (float)p.height;
+      accum = mk(0.0f, 0.0f, 0.0f);
+      ad0 = ad1 = ad2 = 0.0;
+      s = 0;
+      rec_base = (size_t)item * p.spp;
+      alive = begin_sample();
+      continue;
+    }
     // -- NEE occlusion rays, first query: the closest emitter (emitter BVH).
     // A miss and a non-emitting closest hit both leave a false hit flag
     // false (shader.cu:172-184), so with the flag false a ray that hits no
@@ -465,7 +562,7 @@ __global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(Scene
       }
       if (!alive) break;
     }
-    if (!alive) break;
+    if (!alive) continue;  // the pixel is written at the top of the loop
 
     // -- one full traversal per iteration: the closest hit of a radiance ray
     // (qstage 0) or the any-hit query of an occlusion ray (qstage 2).
@@ -516,7 +613,32 @@ __global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(Scene
             const float4* nq = sc.nrm + 3 * (size_t)k;
             N = bary_normal(P, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
           }
-          if (m.diffuse_alpha.w < 1.0f) {
+          bool cached = false;
+          if (kMode == 1 && b == p.cache_at) {
+            const uint32_t slot = atomicAdd(p.q_count, 1u);
+            if (slot < p.q_cap) {
+              cache_features(p, P, Din, N, m, p.q_feat + 12 * (size_t)slot);
+              p.q_att[3 * (size_t)slot] = att.x;
+              p.q_att[3 * (size_t)slot + 1] = att.y;
+              p.q_att[3 * (size_t)slot + 2] = att.z;
+              p.q_item[slot] = item;
+            }
+            cached = true;
+            finish = true;
+          }
+          if (kMode == 2 && t_used < p.t_slots) {
+            float* r = p.t_rec + 18 * ((size_t)item * p.t_slots + t_used);
+            cache_features(p, P, Din, N, m, r);
+            r[12] = att.x;
+            r[13] = att.y;
+            r[14] = att.z;
+            r[15] = color.x;
+            r[16] = color.y;
+            r[17] = color.z;
+            ++t_used;
+          }
+          if (cached) {
+          } else if (m.diffuse_alpha.w < 1.0f) {
             D = Dielectric::scatter(D, N, m.rough_n.y, xyz(m.diffuse_alpha), att, seed);
             O = P;
             if (b == p.bounces) finish = true;
@@ -556,32 +678,162 @@ __global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(Scene
     }
   }
 
-  // shader.cu:156-166
-  F3 out;
-  if (p.fp64) {
-    const double inv = 1.0 / (double)p.spp;
-    out = mk((float)(ad0 * inv), (float)(ad1 * inv), (float)(ad2 * inv));
-  } else {
-    const float inv = 1.0f / (float)p.spp;
-    out = accum * inv;
-  }
-  if (p.subframe > 0) {
-    const float a = 1.0f / (float)(p.subframe + 1);
-    out = lerp(xyz(p.accum_prev[item]), out, a);
-  }
-  p.accum[item] = make_float4(out.x, out.y, out.z, 1.0f);
-
   if (kCount) {
     atomicAdd(&p.counters[0], (unsigned long long)c_rad);
     atomicAdd(&p.counters[1], (unsigned long long)c_occ);
     atomicAdd(&p.counters[2], (unsigned long long)tc.visits);
     atomicAdd(&p.counters[3], (unsigned long long)tc.tests);
     atomicAdd(&p.counters[4], (unsigned long long)c_rhit);
-    atomicAdd(&p.counters[5], (unsigned long long)p.spp);
+    atomicAdd(&p.counters[5], (unsigned long long)c_smp);
     atomicAdd(&p.counters[6], (unsigned long long)c_inv);
-    atomicAdd(&p.counters[7], 1ull);
+    atomicAdd(&p.counters[7], (unsigned long long)c_px);
   }
 }
+
+// One pixel per work-item: the listed pixels, or the full image in
+// kTileW x kTileH tiles.
+struct OnePixel {
+  SceneView sc;
+  PathArgs p;
+  bool given;
+  __device__ bool next(Job& j) {
+    if (given) return false;
+    given = true;
+    if (p.pixels) {
+      j.item = blockIdx.x * kPathBlock + threadIdx.x;
+      if (j.item >= p.n_items) return false;
+      const uint32_t pix = (uint32_t)p.pixels[j.item];
+      j.x = pix % p.width;
+      j.y = pix / p.width;
+      return true;
+    }
+    const uint32_t tx = blockIdx.x % p.tiles_x, ty = blockIdx.x / p.tiles_x;
+    j.x = tx * kTileW + threadIdx.x % kTileW;
+    j.y = ty * kTileH + threadIdx.x / kTileW;
+    if (j.x >= p.width || j.y >= p.height) return false;
+    j.item = (int32_t)(j.y * p.width + j.x);
+    return true;
+  }
+};
+
+template <bool kRecord, bool kCount>
+__global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
+  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  OnePixel src{sc, p, false};
+  path_lane<kRecord, kCount>(src, stk + threadIdx.x);
+}
+
+// Path loop in a cache mode (1: cached render of the full image in tiles,
+// 2: training paths on a pixel list).
+template <int kMode>
+__global__ __launch_bounds__(kPathBlock) void k_path_mode(SceneView sc, PathArgs p) {
+  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  OnePixel src{sc, p, false};
+  path_lane<false, false, OnePixel, kMode>(src, stk + threadIdx.x);
+}
+
+// frame[item] += throughput * max(network output, 0) / spp for every query.
+__global__ __launch_bounds__(256) void k_cache_add(const float* __restrict__ pred,
+                                                   const float* __restrict__ att,
+                                                   const int32_t* __restrict__ item, uint32_t n,
+                                                   float inv_spp, float4* frame) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float* f = reinterpret_cast<float*>(frame + item[i]);
+  for (int c = 0; c < 3; ++c) atomicAdd(f + c, att[3 * i + c] * fmaxf(pred[3 * i + c], 0.0f) * inv_spp);
+}
+
+// Running mean over frames: accum = frame (first) or lerp(accum, frame, a).
+__global__ __launch_bounds__(256) void k_blend(float4* accum, const float4* __restrict__ frame,
+                                               int64_t n, float a, int first) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 f = frame[i];
+  if (first) {
+    accum[i] = make_float4(f.x, f.y, f.z, 1.0f);
+  } else {
+    const float4 o = accum[i];
+    const F3 r = lerp(mk(o.x, o.y, o.z), mk(f.x, f.y, f.z), a);
+    accum[i] = make_float4(r.x, r.y, r.z, 1.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched wavefront form: one persistent launch renders many scenes.  Work is
+// the concatenation of every scene's pixels in 8x8-tile order; each lane
+// takes `chunk` work indices at a time from a device counter, renders those
+// pixels completely (the same per-pixel computation as k_path, so the images
+// are bit-identical) and takes more until the queue is empty.  Lanes whose
+// paths end early keep pulling pixels instead of idling behind the wave's
+// longest path, and small and large scenes share one launch.
+// ---------------------------------------------------------------------------
+struct BatchScene {
+  SceneView sc;
+  PathArgs p;      // the scene's launch (tiles_x counts 8x8 tiles)
+  uint32_t first;  // first work index of the scene
+};
+
+struct QueueSrc {
+  const BatchScene* __restrict__ scenes;
+  int32_t n_scenes;
+  uint32_t* counter;
+  uint32_t total;
+  uint32_t chunk;
+  uint32_t cur, end;
+  int32_t scene;
+  SceneView sc;
+  PathArgs p;
+  __device__ bool next(Job& j) {
+    for (;;) {
+      if (cur >= end) {
+        const uint32_t base = atomicAdd(counter, chunk);
+        if (base >= total) return false;
+        cur = base;
+        end = min(base + chunk, total);
+      }
+      const uint32_t idx = cur++;
+      int32_t lo = 0, hi = n_scenes - 1;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (scenes[mid].first <= idx)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      if (lo != scene) {
+        sc = scenes[lo].sc;
+        p = scenes[lo].p;
+        scene = lo;
+      }
+      const uint32_t local = idx - scenes[lo].first;
+      const uint32_t tile = local >> 6, w = local & 63u;
+      const uint32_t x = (tile % p.tiles_x) * 8 + (w & 7u), y = (tile / p.tiles_x) * 8 + (w >> 3);
+      if (x >= p.width || y >= p.height) continue;  // outside a partial edge tile
+      j.x = x;
+      j.y = y;
+      j.item = (int32_t)(y * p.width + x);
+      return true;
+    }
+  }
+};
+
+__global__ __launch_bounds__(kPathBlock) void k_path_queue(const BatchScene* scenes, int32_t n,
+                                                           uint32_t* counter, uint32_t total,
+                                                           uint32_t chunk) {
+  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  QueueSrc src;
+  src.scenes = scenes;
+  src.n_scenes = n;
+  src.counter = counter;
+  src.total = total;
+  src.chunk = chunk;
+  src.cur = src.end = 0;
+  src.scene = -1;
+  path_lane<false, false>(src, stk + threadIdx.x);
+}
+
+// Queue reset inside the captured graph (a kernel, not a memset node).
+__global__ void k_queue_reset(uint32_t* counter) { *counter = 0u; }
 
 // ---------------------------------------------------------------------------
 // LBVH build (Karras, HPG 2012).
@@ -981,6 +1233,27 @@ struct DevBvh {
   uint64_t bytes = 0;
 };
 
+// Device buffer that only grows: reallocated when a call needs more.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t want) {
+    if (p && want <= cap) return hipSuccess;
+    (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(want, 1) * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 struct lisa_ctx {
   int32_t device = 0;
   hipStream_t stream = nullptr;
@@ -997,6 +1270,28 @@ struct lisa_ctx {
   uint64_t counters[LISA_NCOUNTERS] = {0, 0, 0, 0, 0, 0, 0, 0};
   float render_ms = 0.0f, bvh_ms = 0.0f;
   uint64_t scene_bytes = 0, accum_bytes = 0;
+  float bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};  // scene box (cache features)
+  // radiance-cache frame scratch (lisa_render_cached), grown on demand
+  DevBuf<float4> c_frame, c_tacc;
+  DevBuf<float> c_qfeat, c_qatt, c_qout, c_trec, c_tfeat, c_ttgt;
+  DevBuf<int32_t> c_qitem, c_tpix;
+  DevBuf<uint32_t> c_qcount;
+};
+
+struct lisa_batch {
+  int32_t device = 0, n = 0, cus = 256;
+  std::vector<lisa_ctx*> ctxs;
+  std::vector<lisa_render_params> params;
+  std::vector<size_t> accum_px;
+  uint32_t total = 0;
+  hipStream_t stream = nullptr;
+  BatchScene* d_scenes = nullptr;
+  uint32_t* d_counter = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  lisa_batch_config cfg{0, 1};
+  float last_ms = 0.0f;
 };
 
 namespace {
@@ -1229,6 +1524,17 @@ void lisa_destroy(lisa_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->c_frame.release();
+  c->c_tacc.release();
+  c->c_qfeat.release();
+  c->c_qatt.release();
+  c->c_qout.release();
+  c->c_trec.release();
+  c->c_tfeat.release();
+  c->c_ttgt.release();
+  c->c_qitem.release();
+  c->c_tpix.release();
+  c->c_qcount.release();
   free_scene(c);
   (void)hipFree(c->d_accum);
   (void)hipFree(c->d_counters);
@@ -1298,6 +1604,15 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
   HIP_TRY(ctx, hipStreamSynchronize(st));
   HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
   ctx->scene_bytes += ctx->all.bytes + ctx->emit.bytes;
+  if (T > 0) {  // scene box = union of the root's child boxes
+    float rb[16];
+    HIP_TRY(ctx, hipMemcpy(rb, ctx->all.nodes, 64, hipMemcpyDeviceToHost));
+    const bool two = T > 1;
+    for (int a = 0; a < 3; ++a) {
+      ctx->bmin[a] = two ? std::min(rb[a], rb[6 + a]) : rb[a];
+      ctx->bmax[a] = two ? std::max(rb[3 + a], rb[9 + a]) : rb[3 + a];
+    }
+  }
   ctx->has_scene = true;
   return LISA_OK;
 }
@@ -1369,7 +1684,6 @@ lisa_status lisa_trace_subset(lisa_ctx* ctx, const lisa_render_params* p, const 
   if (rc != LISA_OK) return rc;
   if (n_pixels <= 0 || !pixels || !accum_rgba_out)
     return fail(ctx, LISA_ERR_INVALID_ARG, "pixels / accum_rgba_out required, n_pixels > 0");
-  if (p->subframe > 0) return fail(ctx, LISA_ERR_UNSUPPORTED, "lisa_trace_subset runs subframe 0 only");
   if (record_bounces < 0) return fail(ctx, LISA_ERR_INVALID_ARG, "record_bounces < 0");
   const int64_t npx = (int64_t)p->width * p->height;
   for (int32_t i = 0; i < n_pixels; ++i)
@@ -1512,6 +1826,328 @@ lisa_status lisa_copy_probe(lisa_ctx* ctx, int64_t bytes, int32_t iters, float* 
   float ms = 0.0f;
   HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   *avg_ms = ms / (float)iters;
+  return LISA_OK;
+}
+
+lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,
+                              const lisa_render_params* params, int32_t n) {
+  if (!out) return fail(nullptr, LISA_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (n <= 0 || !ctxs || !params) return fail(nullptr, LISA_ERR_INVALID_ARG, "empty batch");
+  const int32_t dev = ctxs[0] ? ctxs[0]->device : -1;
+  uint64_t total = 0;
+  std::vector<BatchScene> hs(n);
+  for (int32_t i = 0; i < n; ++i) {
+    lisa_ctx* c = ctxs[i];
+    if (!c) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL context in batch");
+    if (c->device != dev) return fail(nullptr, LISA_ERR_INVALID_ARG, "batch contexts on different devices");
+    lisa_status rc = check_params(c, &params[i]);
+    if (rc != LISA_OK) {
+      lisa::set_global_error("batch scene " + std::to_string(i) + ": " + c->err);
+      return rc;
+    }
+    if (params[i].subframe > 0) return fail(nullptr, LISA_ERR_UNSUPPORTED, "batched renders use subframe 0");
+    if (params[i].flags & LISA_FLAG_COUNT) return fail(nullptr, LISA_ERR_UNSUPPORTED, "no counters in batches");
+    for (int32_t k = 0; k < i; ++k)
+      if (ctxs[k] == c) return fail(nullptr, LISA_ERR_INVALID_ARG, "a context appears twice in the batch");
+    if (hipSetDevice(dev) != hipSuccess) return fail(nullptr, LISA_ERR_HIP, "hipSetDevice");
+    const size_t npx = (size_t)params[i].width * params[i].height;
+    if (c->accum_px != npx) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(c->d_accum);
+      c->d_accum = nullptr;
+      c->accum_px = 0;
+      if (hipMalloc(reinterpret_cast<void**>(&c->d_accum), npx * sizeof(float4)) != hipSuccess)
+        return fail(nullptr, LISA_ERR_ALLOC, "accum buffer allocation failed");
+      c->accum_px = npx;
+      c->accum_bytes = npx * sizeof(float4);
+    }
+    PathArgs a = make_args(&params[i]);
+    a.tiles_x = (params[i].width + 7) / 8;
+    const uint32_t tiles_y = (params[i].height + 7) / 8;
+    a.n_items = (int32_t)npx;
+    a.accum = c->d_accum;
+    a.accum_prev = c->d_accum;
+    hs[i].sc = view(c);
+    hs[i].p = a;
+    hs[i].first = (uint32_t)total;
+    total += (uint64_t)a.tiles_x * tiles_y * 64;
+  }
+  if (total >= (1ull << 32) - (1ull << 20)) return fail(nullptr, LISA_ERR_INVALID_ARG, "batch too large");
+  lisa_batch* b = new lisa_batch;
+  b->ctxs.assign(ctxs, ctxs + n);
+  b->params.assign(params, params + n);
+  b->accum_px.resize(n);
+  for (int32_t i = 0; i < n; ++i) b->accum_px[i] = ctxs[i]->accum_px;
+  b->device = dev;
+  b->total = (uint32_t)total;
+  b->stream = ctxs[0]->stream;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) b->cus = prop.multiProcessorCount;
+  b->cfg.grid_blocks = b->cus * 16;
+  b->cfg.chunk = 1;
+  hipError_t e;
+  if ((e = hipMalloc(reinterpret_cast<void**>(&b->d_scenes), sizeof(BatchScene) * n)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&b->d_counter), 256)) != hipSuccess ||
+      (e = hipMemcpy(b->d_scenes, hs.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
+    lisa_batch_destroy(b);
+    return fail(nullptr, LISA_ERR_HIP, std::string("lisa_batch_create: ") + hipGetErrorString(e));
+  }
+  b->n = n;
+  *out = b;
+  return LISA_OK;
+}
+
+void lisa_batch_destroy(lisa_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipFree(b->d_scenes);
+  (void)hipFree(b->d_counter);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  delete b;
+}
+
+lisa_status lisa_batch_set_config(lisa_batch* b, const lisa_batch_config* cfg) {
+  if (!b || !cfg) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL argument");
+  if (cfg->grid_blocks <= 0 || cfg->chunk <= 0 || cfg->chunk > 4096)
+    return fail(nullptr, LISA_ERR_INVALID_ARG, "grid_blocks and chunk must be positive (chunk <= 4096)");
+  if (cfg->grid_blocks != b->cfg.grid_blocks || cfg->chunk != b->cfg.chunk) {
+    b->cfg = *cfg;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  return LISA_OK;
+}
+
+lisa_status lisa_batch_get_config(const lisa_batch* b, lisa_batch_config* cfg) {
+  if (!b || !cfg) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL argument");
+  *cfg = b->cfg;
+  return LISA_OK;
+}
+
+lisa_status lisa_batch_render(lisa_batch* b, float* elapsed_ms) {
+  if (!b) return fail(nullptr, LISA_ERR_INVALID_ARG, "batch is NULL");
+  for (int32_t i = 0; i < b->n; ++i)
+    if (b->ctxs[i]->accum_px != b->accum_px[i] || !b->ctxs[i]->has_scene)
+      return fail(nullptr, LISA_ERR_INVALID_ARG, "a context of the batch changed: re-create the batch");
+  if (hipSetDevice(b->device) != hipSuccess) return fail(nullptr, LISA_ERR_HIP, "hipSetDevice");
+  hipError_t e;
+  if (!b->exec) {  // capture reset + persistent launch once, replay afterwards
+    if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return fail(nullptr, LISA_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    k_queue_reset<<<1, 1, 0, b->stream>>>(b->d_counter);
+    k_path_queue<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter,
+                                                                   b->total, (uint32_t)b->cfg.chunk);
+    e = hipStreamEndCapture(b->stream, &b->graph);
+    if (e != hipSuccess) return fail(nullptr, LISA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+      return fail(nullptr, LISA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  if ((e = hipEventRecord(b->ev0, b->stream)) != hipSuccess ||
+      (e = hipGraphLaunch(b->exec, b->stream)) != hipSuccess ||
+      (e = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(b->stream)) != hipSuccess)
+    return fail(nullptr, LISA_ERR_HIP, std::string("lisa_batch_render: ") + hipGetErrorString(e));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+  b->last_ms = ms;
+  if (elapsed_ms) *elapsed_ms = ms;
+  return LISA_OK;
+}
+
+lisa_status lisa_batch_autotune(lisa_batch* b, int32_t tune_spp, lisa_batch_config* best_out) {
+  if (!b) return fail(nullptr, LISA_ERR_INVALID_ARG, "batch is NULL");
+  if (tune_spp <= 0) tune_spp = 2;
+  // A reduced-spp copy of the batch stands in for the real work.
+  std::vector<lisa_render_params> tp(b->params);
+  for (auto& q : tp) q.spp = std::min<uint32_t>(q.spp, (uint32_t)tune_spp);
+  lisa_batch* t = nullptr;
+  lisa_status rc = lisa_batch_create(&t, b->ctxs.data(), tp.data(), b->n);
+  if (rc != LISA_OK) return rc;
+  const int32_t per_cu[] = {4, 8, 12, 16, 20, 24, 32};
+  const int32_t chunks[] = {1, 4, 16};
+  float best = 1e30f;
+  lisa_batch_config best_cfg = b->cfg;
+  for (int32_t pc : per_cu)
+    for (int32_t ch : chunks) {
+      lisa_batch_config c{b->cus * pc, ch};
+      float ms = 0.0f, ms2 = 0.0f;
+      if ((rc = lisa_batch_set_config(t, &c)) != LISA_OK || (rc = lisa_batch_render(t, &ms)) != LISA_OK ||
+          (rc = lisa_batch_render(t, &ms2)) != LISA_OK) {
+        lisa_batch_destroy(t);
+        return rc;
+      }
+      ms = std::min(ms, ms2);
+      if (ms < best) {
+        best = ms;
+        best_cfg = c;
+      }
+    }
+  lisa_batch_destroy(t);
+  rc = lisa_batch_set_config(b, &best_cfg);
+  if (rc != LISA_OK) return rc;
+  if (best_out) *best_out = best_cfg;
+  return LISA_OK;
+}
+
+lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_render_params* p,
+                               const lisa_cache_frame_params* cp, lisa_cache_frame_stats* stats) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!cache || !cp) return fail(ctx, LISA_ERR_INVALID_ARG, "cache / cache params NULL");
+  lisa_status rc = check_params(ctx, p);
+  if (rc != LISA_OK) return rc;
+  if (p->flags & LISA_FLAG_COUNT) return fail(ctx, LISA_ERR_UNSUPPORTED, "no counters in cached renders");
+  if (cp->prefix_bounces == 0 || cp->prefix_bounces > 64)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 1..64");
+  if (cp->train_paths > 0 && (cp->train_bounces == 0 || cp->train_bounces > 64))
+    return fail(ctx, LISA_ERR_INVALID_ARG, "train_bounces must be in 1..64");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const auto t_start = std::chrono::steady_clock::now();
+  lisa_cache_frame_stats out_stats;
+  std::memset(&out_stats, 0, sizeof(out_stats));
+  const size_t npx = (size_t)p->width * p->height;
+  if (ctx->accum_px != npx) {
+    if (p->subframe > 0) return fail(ctx, LISA_ERR_INVALID_ARG, "frame > 0 but the image size changed");
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    (void)hipFree(ctx->d_accum);
+    ctx->d_accum = nullptr;
+    ctx->accum_px = 0;
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_accum), npx * sizeof(float4)));
+    ctx->accum_px = npx;
+    ctx->accum_bytes = npx * sizeof(float4);
+  }
+  PathArgs base = make_args(p);
+  base.bmin = mk(ctx->bmin[0], ctx->bmin[1], ctx->bmin[2]);
+  base.binv = mk(1.0f / std::max(ctx->bmax[0] - ctx->bmin[0], 1e-20f),
+                 1.0f / std::max(ctx->bmax[1] - ctx->bmin[1], 1e-20f),
+                 1.0f / std::max(ctx->bmax[2] - ctx->bmin[2], 1e-20f));
+  base.fp64 = 0;
+  float loss = 0.0f;
+  // 1. online training on suffixes of full-length paths traced by the plain loop
+  if (cp->train_paths > 0 && cp->train_steps > 0) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t nt = cp->train_paths, slots = cp->train_bounces;
+    const size_t cap = cp->train_max > 0 ? cp->train_max : nt * slots;
+    HIP_TRY(ctx, ctx->c_trec.ensure(nt * slots * 18));
+    HIP_TRY(ctx, ctx->c_tfeat.ensure(cap * 12));
+    HIP_TRY(ctx, ctx->c_ttgt.ensure(cap * 3));
+    HIP_TRY(ctx, ctx->c_tacc.ensure(nt));
+    HIP_TRY(ctx, ctx->c_tpix.ensure(nt));
+    HIP_TRY(ctx, ctx->c_qcount.ensure(64));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p + 1, 0, 4, st));
+    std::vector<int32_t> px(nt);
+    uint64_t sm = cp->seed * 0x9E3779B97F4A7C15ull + p->subframe + 1;
+    for (size_t i = 0; i < nt; ++i) {
+      sm += 0x9E3779B97F4A7C15ull;
+      uint64_t z = sm;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      px[i] = (int32_t)(z % npx);
+    }
+    HIP_TRY(ctx, hipMemcpy(ctx->c_tpix.p, px.data(), 4 * nt, hipMemcpyHostToDevice));
+    PathArgs a = base;
+    a.spp = 1;
+    a.bounces = cp->train_bounces;
+    a.subframe = 0x40000000u ^ p->subframe;  // an RNG stream apart from the rendered samples
+    a.pixels = ctx->c_tpix.p;
+    a.n_items = (int32_t)nt;
+    a.accum = ctx->c_tacc.p;
+    a.accum_prev = nullptr;
+    a.t_rec = ctx->c_trec.p;
+    a.t_slots = (uint32_t)slots;
+    a.t_feat = ctx->c_tfeat.p;
+    a.t_tgt = ctx->c_ttgt.p;
+    a.t_count = ctx->c_qcount.p + 1;
+    a.t_cap = (uint32_t)cap;
+    k_path_mode<2><<<grid_for((int64_t)nt, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
+    HIP_TRY(ctx, hipGetLastError());
+    uint32_t rows = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&rows, ctx->c_qcount.p + 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    rows = std::min<uint32_t>(rows, (uint32_t)cap);
+    const auto t1 = std::chrono::steady_clock::now();
+    for (uint32_t k = 0; k < cp->train_steps && rows > 0; ++k) {
+      float l = 0.0f;
+      lisa_status r2 = lisa::cache_train_device(cache, ctx->c_tfeat.p, 12, ctx->c_ttgt.p, (int64_t)rows, &l);
+      if (r2 != LISA_OK) return fail(ctx, r2, std::string("cache training: ") + lisa_cache_last_error(cache));
+      loss = l;
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    out_stats.trace_train_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    out_stats.train_ms = std::chrono::duration<float, std::milli>(t2 - t1).count();
+    out_stats.train_samples = (int32_t)rows;
+  }
+  out_stats.loss = loss;
+  // 2. this frame: cached paths, or the plain tracer as the fallback
+  HIP_TRY(ctx, ctx->c_frame.ensure(npx));
+  const bool fallback = cp->fallback_loss > 0.0f && loss > cp->fallback_loss;
+  const unsigned tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
+  const auto t3 = std::chrono::steady_clock::now();
+  if (fallback) {
+    PathArgs a = base;
+    a.tiles_x = (p->width + kTileW - 1) / kTileW;
+    a.n_items = (int32_t)npx;
+    a.accum = ctx->c_frame.p;
+    a.accum_prev = nullptr;
+    k_path<false, false><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    out_stats.fell_back = 1;
+  } else {
+    const size_t qcap = npx * p->spp;
+    HIP_TRY(ctx, ctx->c_qfeat.ensure(qcap * 12));
+    HIP_TRY(ctx, ctx->c_qatt.ensure(qcap * 3));
+    HIP_TRY(ctx, ctx->c_qout.ensure(qcap * 3));
+    HIP_TRY(ctx, ctx->c_qitem.ensure(qcap));
+    HIP_TRY(ctx, ctx->c_qcount.ensure(64));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p, 0, 4, st));
+    PathArgs a = base;
+    a.tiles_x = (p->width + kTileW - 1) / kTileW;
+    a.n_items = (int32_t)npx;
+    a.accum = ctx->c_frame.p;
+    a.accum_prev = nullptr;
+    a.bounces = cp->prefix_bounces + 1;
+    a.cache_at = cp->prefix_bounces + 1;
+    a.q_feat = ctx->c_qfeat.p;
+    a.q_att = ctx->c_qatt.p;
+    a.q_item = ctx->c_qitem.p;
+    a.q_count = ctx->c_qcount.p;
+    a.q_cap = (uint32_t)qcap;
+    k_path_mode<1><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
+    HIP_TRY(ctx, hipGetLastError());
+    uint32_t nq = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    nq = std::min<uint32_t>(nq, (uint32_t)qcap);
+    const auto t4 = std::chrono::steady_clock::now();
+    if (nq > 0) {
+      lisa_status r2 = lisa::cache_infer_device(cache, ctx->c_qfeat.p, nq, ctx->c_qout.p);
+      if (r2 != LISA_OK) return fail(ctx, r2, std::string("cache inference: ") + lisa_cache_last_error(cache));
+      k_cache_add<<<(nq + 255) / 256, 256, 0, st>>>(ctx->c_qout.p, ctx->c_qatt.p, ctx->c_qitem.p, nq,
+                                                    1.0f / (float)p->spp, ctx->c_frame.p);
+      HIP_TRY(ctx, hipGetLastError());
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    out_stats.infer_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t4).count();
+    out_stats.queries = (int32_t)nq;
+  }
+  out_stats.render_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t3).count();
+  // 3. running mean over frames
+  k_blend<<<grid_for((int64_t)npx, 256), 256, 0, st>>>(ctx->d_accum, ctx->c_frame.p, (int64_t)npx,
+                                                      1.0f / (float)(p->subframe + 1), p->subframe == 0);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  out_stats.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (stats) *stats = out_stats;
   return LISA_OK;
 }
 

@@ -62,25 +62,26 @@ def _box(lo, hi):
 
 
 def _uv_sphere(center, radius, seg, rings):
-    """UV sphere: `seg` segments around, `rings` rings pole to pole, smooth normals."""
+    """UV sphere: `seg` segments around, `rings` rings pole to pole, smooth normals.
+
+    Per ring i and segment j, triangles (a, c, b) unless i is the top ring and
+    (a, d, c) unless i is the bottom ring, in that order."""
     th = np.linspace(0.0, np.pi, rings + 1)
     ph = np.linspace(0.0, 2.0 * np.pi, seg + 1)
     T, P = np.meshgrid(th, ph, indexing="ij")
     nrm = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1)
     pos = np.asarray(center, np.float64) + radius * nrm
-    tris_v, tris_n = [], []
-    for i in range(rings):
-        for j in range(seg):
-            a, b = (i, j), (i, j + 1)
-            c, d = (i + 1, j + 1), (i + 1, j)
-            if i != 0:
-                tris_v.append([pos[a], pos[c], pos[b]])
-                tris_n.append([nrm[a], nrm[c], nrm[b]])
-            if i != rings - 1:
-                tris_v.append([pos[a], pos[d], pos[c]])
-                tris_n.append([nrm[a], nrm[d], nrm[c]])
-    v = np.asarray(tris_v, np.float32).reshape(-1, 9)
-    n = np.asarray(tris_n, np.float32).reshape(-1, 9)
+    i, j = np.meshgrid(np.arange(rings), np.arange(seg), indexing="ij")
+    corner = {"a": (i, j), "b": (i, j + 1), "c": (i + 1, j + 1), "d": (i + 1, j)}
+    def tri(arr, k1, k2, k3):
+        return np.stack([arr[corner[k][0], corner[k][1]] for k in (k1, k2, k3)], 2)
+    tv = np.stack([tri(pos, "a", "c", "b"), tri(pos, "a", "d", "c")], 2)  # [rings, seg, 2, 3, 3]
+    tn = np.stack([tri(nrm, "a", "c", "b"), tri(nrm, "a", "d", "c")], 2)
+    keep = np.ones((rings, seg, 2), bool)
+    keep[0, :, 0] = False
+    keep[rings - 1, :, 1] = False
+    v = tv[keep].astype(np.float32).reshape(-1, 9)
+    n = tn[keep].astype(np.float32).reshape(-1, 9)
     return v, n
 
 
@@ -320,3 +321,148 @@ def seeded_pixels(width, height, n, seed=0):
     rs = np.random.default_rng(seed)
     n = min(n, width * height)
     return np.sort(rs.choice(width * height, size=n, replace=False)).astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Batched multi-scene workload: 64 seeded scenes with log-uniform triangle
+# counts in [10k, 2M] built from a sine-noise heightfield, boxes and UV
+# spheres; materials 70% Lambertian / 20% dielectric (n in [1.3, 1.7]) /
+# 10% emissive, plus an emissive quad; square images of 256..1024 pixels.
+# ---------------------------------------------------------------------------
+def _heightfield(n, size, amp, rs):
+    """n x n vertex grid over [-size, size]^2 with a sine-noise height field."""
+    xs = np.linspace(-size, size, n)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    Y = np.zeros_like(X)
+    for _ in range(4):
+        fx, fz = rs.uniform(0.5, 6.0, 2) * np.pi / size
+        ph = rs.uniform(0, 2 * np.pi, 2)
+        Y += rs.uniform(0.2, 1.0) * np.sin(fx * X + ph[0]) * np.sin(fz * Z + ph[1])
+    Y *= amp / 4.0
+    P = np.stack([X, Y, Z], -1)
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a, b, c, d = P[i, j], P[i + 1, j], P[i + 1, j + 1], P[i, j + 1]
+    tri = np.concatenate([np.stack([a, c, b], 2), np.stack([a, d, c], 2)]).reshape(-1, 3, 3)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    nrm[nrm[:, 1] < 0] *= -1.0
+    return (tri.astype(np.float32).reshape(-1, 9),
+            np.repeat(nrm[:, None], 3, 1).astype(np.float32).reshape(-1, 9))
+
+
+def _random_material(rs, name):
+    u = rs.uniform()
+    if u < 0.7:
+        return {"name": name, "diffuse": tuple(rs.uniform(0.2, 0.9, 3)), "alpha": 1.0,
+                "roughness": float(rs.uniform(0.0, 1.0)), "n": 1.0}
+    if u < 0.9:
+        return {"name": name, "diffuse": (1.0, 1.0, 1.0), "alpha": 0.0, "roughness": 0.0,
+                "n": float(rs.uniform(1.3, 1.7))}
+    return {"name": name, "diffuse": (0.0, 0.0, 0.0), "alpha": 1.0, "emit": True,
+            "emission": tuple(rs.uniform(1.0, 5.0, 3))}
+
+
+def batch_scene(index, seed=0, spp=32, bounces=8):
+    """Scene `index` (0..63) of the seeded multi-scene batch."""
+    rs = np.random.default_rng([seed, index])
+    target = int(np.exp(rs.uniform(np.log(10_000), np.log(2_000_000))))
+    mats = [_random_material(rs, f"m{k}") for k in range(8)]
+    mats.append({"name": "sun", "diffuse": (0, 0, 0), "alpha": 1.0, "emit": True,
+                 "emission": (4.0, 4.0, 4.0)})
+    groups = []
+    size = 1.0
+    n = int(np.clip(np.sqrt(0.6 * target / 2.0) + 1, 8, 2048))
+    groups.append((_heightfield(n, size, 0.15, rs), f"m{rs.integers(0, 8)}"))
+    budget = target - 2 * (n - 1) ** 2
+    while budget > 12:
+        if rs.uniform() < 0.5 or budget < 200:
+            c = rs.uniform(-0.8, 0.8, 3) * np.array([1, 0, 1]) + np.array([0, 0.2, 0])
+            h = rs.uniform(0.03, 0.2, 3)
+            groups.append((_box(tuple(c - h), tuple(c + h)), f"m{rs.integers(0, 8)}"))
+            budget -= 12
+        else:
+            seg = int(np.clip(np.sqrt(budget / 4.0), 8, 512))
+            rings = max(seg // 2, 3)
+            c = (rs.uniform(-0.7, 0.7), rs.uniform(0.1, 0.5), rs.uniform(-0.7, 0.7))
+            groups.append((_uv_sphere(c, rs.uniform(0.05, 0.3), seg, rings), f"m{rs.integers(0, 8)}"))
+            budget -= 2 * seg * (rings - 1)
+    s = 0.4
+    groups.append((_quad((-s, 1.5, -s), (-s, 1.5, s), (s, 1.5, s), (s, 1.5, -s), (0, -1, 0)), "sun"))
+    res = int(rs.choice([256, 384, 512, 768, 1024]))
+    sc = _scene(groups, mats, res, res, spp, bounces, f"batch{index:02d}")
+    sc["camera"] = {"eye": (0.0, 1.2, 2.2), "look_at": (0.0, 0.0, 0.0), "fov": 45.0}
+    return sc
+
+
+# ---------------------------------------------------------------------------
+# Learned-radiance-cache workload: a seeded fBm heightfield island on a
+# 2048^2 vertex grid (2 * 2047^2 = 8,380,418 triangles, smooth normals), its
+# patches Lambertian with albedo U[0.2, 0.8], a dielectric water plane at
+# n = 1.33 and a quad sun light; 1920x1080.
+# ---------------------------------------------------------------------------
+def _fbm_grid(n, octaves, base, seed):
+    """fBm of smoothstep-interpolated value noise on an n x n grid over [0,1]^2."""
+    rs = np.random.default_rng(seed)
+    u = np.linspace(0.0, 1.0, n)
+    out = np.zeros((n, n))
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        f = base * (2 ** o)
+        lat = rs.uniform(-1.0, 1.0, (f + 2, f + 2))
+        q = u * f
+        i0 = np.minimum(np.floor(q).astype(np.int64), f)
+        t = q - i0
+        w = t * t * (3.0 - 2.0 * t)
+        a = lat[i0[:, None], i0[None, :]]
+        b = lat[i0[:, None] + 1, i0[None, :]]
+        c = lat[i0[:, None], i0[None, :] + 1]
+        d = lat[i0[:, None] + 1, i0[None, :] + 1]
+        wx, wz = w[:, None], w[None, :]
+        out += amp * ((a * (1 - wx) + b * wx) * (1 - wz) + (c * (1 - wx) + d * wx) * wz)
+        norm += amp
+        amp *= 0.5
+    return out / norm
+
+
+def island(n=2048, width=1920, height=1080, spp=64, bounces=12, seed=0, patches=16):
+    rs = np.random.default_rng(seed)
+    size = 1.0
+    h = _fbm_grid(n, 7, 4, seed)
+    xs = np.linspace(-size, size, n)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    r2 = (X * X + Z * Z) / (size * size)
+    Y = 0.35 * (h + 0.55) * np.clip(1.0 - r2, 0.0, 1.0) - 0.08
+    # smooth vertex normals from central differences
+    dx = 2 * size / (n - 1)
+    gx = np.gradient(Y, dx, axis=0)
+    gz = np.gradient(Y, dx, axis=1)
+    N = np.stack([-gx, np.ones_like(Y), -gz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    P = np.stack([X, Y, Z], -1)
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    corners = ((i, j), (i + 1, j + 1), (i + 1, j)), ((i, j), (i, j + 1), (i + 1, j + 1))
+    tv = np.concatenate([np.stack([P[a] for a in c], 2) for c in corners]).reshape(-1, 9)
+    tn = np.concatenate([np.stack([N[a] for a in c], 2) for c in corners]).reshape(-1, 9)
+    # albedo patches: 32x32-quad blocks pick one of `patches` materials
+    blk = (i // 32) * ((n - 2) // 32 + 1) + (j // 32)
+    pick = rs.integers(0, patches, blk.max() + 1)[blk]
+    mat = np.concatenate([pick, pick]).reshape(-1).astype(np.int32)
+    materials = [{"name": f"ground{k}", "diffuse": tuple(rs.uniform(0.2, 0.8, 3)), "alpha": 1.0,
+                  "roughness": 1.0, "n": 1.0} for k in range(patches)]
+    materials.append({"name": "water", "diffuse": (1.0, 1.0, 1.0), "alpha": 0.0,
+                      "roughness": 0.0, "n": 1.33})
+    materials.append({"name": "sun", "diffuse": (0.0, 0.0, 0.0), "alpha": 1.0, "emit": True,
+                      "emission": (12.0, 11.0, 9.0)})
+    w = 1.6
+    wv, wn = _quad((-w, 0.0, -w), (-w, 0.0, w), (w, 0.0, w), (w, 0.0, -w), (0, 1, 0))
+    s, sy = 0.5, 2.5
+    lv, ln = _quad((-s + 0.6, sy, -s), (-s + 0.6, sy, s), (s + 0.6, sy, s), (s + 0.6, sy, -s),
+                   (0, -1, 0))
+    verts = np.ascontiguousarray(np.concatenate([tv.astype(np.float32), wv, lv]))
+    norms = np.ascontiguousarray(np.concatenate([tn.astype(np.float32), wn, ln]))
+    mat_idx = np.concatenate([mat, np.full(2, patches, np.int32), np.full(2, patches + 1, np.int32)])
+    return {"name": f"island_{n}", "verts": verts, "normals": norms,
+            "mat_idx": np.ascontiguousarray(mat_idx, dtype=np.int32),
+            "materials": materials, "meshes": [],
+            "camera": {"eye": (0.0, 1.1, 2.3), "look_at": (0.0, 0.0, 0.0), "fov": 40.0},
+            "width": width, "height": height, "spp": spp, "bounces": bounces}

@@ -733,7 +733,7 @@ void render_pixel(const Scene& s, const orc_render_params& p, uint32_t pix, floa
   }
   const float inv = 1.0f / (float)p.spp;
   accum = accum * inv;
-  if (p.subframe > 0) {
+  if (p.subframe > 0 && prev4) {  // shader.cu:160-164; without a previous buffer the subframe only seeds
     const float a = 1.0f / (float)(p.subframe + 1);
     accum = lerp(ld3(prev4), accum, a);
   }
@@ -924,7 +924,6 @@ int orc_render(const orc_scene* o, const orc_render_params* p, const int32_t* pi
                int32_t* rad_hits, uint64_t* counters, int threads) {
   const Scene& s = o->s;
   if (p->width == 0 || p->height == 0 || p->spp == 0) return -1;
-  if (p->subframe > 0 && !accum_prev) return -3;
   const int64_t npx = (int64_t)p->width * p->height;
   const int64_t n = pixel_ids ? n_pixels : npx;
   for (int64_t i = 0; i < n; ++i)

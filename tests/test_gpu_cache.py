@@ -1,0 +1,86 @@
+"""GPU: the radiance-cache MLP kernels (BF16 MFMA, FP32 accumulation)
+against the CPU oracle oracle/mlp.py run with the same BF16 rounding points.
+
+Tolerances: outputs and weight gradients are compared with a relative error
+bound of 2e-2 (per layer, Frobenius); BF16 activations rounded after a
+differently ordered FP32 sum can land one BF16 ulp (2^-8) apart.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cache(capi):
+    c = capi.Cache(0, seed=7, lr=1e-3)
+    yield c
+    c.close()
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_matches_oracle(cache):
+    rs = np.random.default_rng(0)
+    feat = rs.uniform(0, 1, (1000, 12)).astype(np.float32)
+    w = cache.weights()
+    got = cache.infer(feat)
+    want, _ = mlp.Net(w, "bf16").forward(feat)
+    print("forward rel err", _rel(got, want), flush=True)
+    assert _rel(got, want) < 2e-2
+    assert np.abs(got).max() > 1e-3
+
+
+def test_gradients_match_oracle(cache):
+    rs = np.random.default_rng(1)
+    feat = rs.uniform(0, 1, (4096, 12)).astype(np.float32)
+    tgt = rs.uniform(0, 1, (4096, 3)).astype(np.float32)
+    w = cache.weights()
+    loss, g = cache.gradients(feat, tgt)
+    rloss, rg = mlp.Net(w, "bf16").loss_and_grads(feat, tgt)
+    assert loss == pytest.approx(rloss, rel=2e-2)
+    for l in range(mlp.LAYERS):
+        a = g[mlp.OFFSET[l]:mlp.OFFSET[l + 1]]
+        b = rg[mlp.OFFSET[l]:mlp.OFFSET[l + 1]]
+        print("layer", l, "grad rel err", _rel(a, b), flush=True)
+        assert _rel(a, b) < 2e-2, l
+    assert np.all(mlp.mask_padding(g) == g)  # padding has no gradient
+
+
+def test_adam_step_matches_oracle(cache):
+    rs = np.random.default_rng(2)
+    feat = rs.uniform(0, 1, (2048, 12)).astype(np.float32)
+    tgt = rs.uniform(0, 1, (2048, 3)).astype(np.float32)
+    w0 = cache.weights()
+    cache.set_weights(w0)  # resets Adam state
+    _, g = cache.gradients(feat, tgt)
+    cache.train(feat, tgt)
+    w1 = cache.weights()
+    want, _, _ = mlp.adam(w0, np.zeros_like(w0), np.zeros_like(w0), g, 1, 1e-3)
+    assert np.allclose(w1, want, rtol=1e-5, atol=1e-7)
+    cache.set_weights(w0)
+
+
+def test_training_reduces_loss(capi):
+    c = capi.Cache(0, seed=3, lr=3e-3)
+    rs = np.random.default_rng(3)
+
+    def target(f):
+        return np.stack([np.sin(2 * np.pi * f[:, 0]) * 0.5 + 0.6, f[:, 1] * f[:, 2] + 0.1,
+                         0.3 + 0.2 * f[:, 3]], 1).astype(np.float32)
+
+    losses = []
+    for step in range(300):
+        f = rs.uniform(0, 1, (4096, 12)).astype(np.float32)
+        losses.append(c.train(f, target(f)))
+    print("loss first/last", losses[0], np.mean(losses[-20:]), flush=True)
+    assert np.mean(losses[-20:]) < 0.1 * losses[0]
+    f = rs.uniform(0, 1, (2000, 12)).astype(np.float32)
+    err = np.abs(c.infer(f) - target(f)).mean()
+    assert err < 0.05, err
+    c.close()

@@ -244,3 +244,37 @@ def test_refit_update_bit_exact(capi, oracle_mod, ctx):
     g = ctx.trace_subset(p, px)
     r = o.render(64, 64, 4, 7, sc["camera"]["eye"], U, V, W, pixels=px, record=True, threads=8)
     _subset_compare(g, r)
+
+
+def test_batched_wavefront_bit_exact(capi, oracle_mod):
+    """Three scenes of different sizes, spp and bounces in one persistent,
+    graph-captured launch: every image equals lisa_render's bit for bit,
+    for several queue configurations and after autotuning."""
+    specs = [(32, 16, 64, 48, 4, 7), (64, 32, 40, 72, 3, 5), (16, 8, 96, 40, 2, 9)]
+    ctxs, params, refs = [], [], []
+    for seg, rings, W, H, spp, B in specs:
+        sc = cornell(seg, rings, W, H, spp, B)
+        c = capi.Context(0)
+        c.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        U, V, Wv = camera(sc, capi)
+        p = capi.render_params(W, H, spp, B, sc["camera"]["eye"], U, V, Wv)
+        refs.append(c.render(p)[0])
+        ctxs.append(c)
+        params.append(p)
+    batch = capi.Batch(ctxs, params)
+    for cfg in [None, (64, 1), (1024, 4), (300, 16)]:
+        if cfg:
+            batch.set_config(*cfg)
+        for rep in range(2):  # graph replays
+            batch.render()
+            for i in range(3):
+                got = batch.accum(i)
+                assert np.array_equal(got.view(np.uint32), refs[i].view(np.uint32)), (cfg, rep, i)
+    best = batch.autotune(tune_spp=1)
+    assert best[0] > 0 and best[1] > 0
+    batch.render()
+    for i in range(3):
+        assert np.array_equal(batch.accum(i).view(np.uint32), refs[i].view(np.uint32))
+    batch.close()
+    for c in ctxs:
+        c.close()
