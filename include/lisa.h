@@ -196,7 +196,7 @@ float lisa_cache_last_ms(const lisa_cache* c);
  *  3. the frame is blended into the context's accum buffer as a running
  *     mean with weight 1 / (p->subframe + 1) (frame p->subframe). */
 typedef struct lisa_cache_frame_params {
-  uint32_t prefix_bounces; /* radiance rays traced before the cache answers (>= 1) */
+  uint32_t prefix_bounces; /* radiance rays traced before the cache answers (0: at the camera ray's hit) */
   uint32_t train_paths;    /* training paths per frame (0: no training) */
   uint32_t train_bounces;  /* bounces of a training path */
   uint32_t train_max;      /* at most this many training rows (suffixes) per frame; 0: no cap */
@@ -213,6 +213,30 @@ typedef struct lisa_cache_frame_stats {
 
 lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_render_params* p,
                                const lisa_cache_frame_params* cp, lisa_cache_frame_stats* stats);
+
+/* The two path forms of lisa_render_cached on a pixel list, for checking
+ * against a CPU model.  Cache features of a surface hit (12, each in [0, 1]):
+ * position in the scene box (the union of the BVH root's child boxes), the
+ * incoming ray direction and the shading normal as (acos(y)/pi,
+ * (atan2(z, x) + pi)/2pi), albedo rgb, roughness (0 for a dielectric),
+ * dielectric flag.
+ *
+ * lisa_cache_trace_queries: p->spp samples per listed pixel (seeded by
+ * p->subframe), at most prefix_bounces + 1 radiance rays; the surface hit of
+ * radiance ray prefix_bounces + 1 ends the path in a query (features,
+ * throughput, index into `pixels`).  accum_out (4n) = per-pixel mean of the
+ * radiance gathered before the queries.  Queries come in no fixed order;
+ * *n_queries may exceed q_cap (the rest is dropped).
+ *
+ * lisa_cache_trace_rows: plain paths (p->spp, p->bounces) on the listed
+ * pixels; one row per surface hit: features and the target (path radiance -
+ * radiance gathered before the hit) / throughput on arrival.  Rows of one
+ * path are contiguous, paths in no fixed order; *n_rows may exceed cap. */
+lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p, uint32_t prefix_bounces,
+                                     const int32_t* pixels, int32_t n, float* accum_out, float* q_feat,
+                                     float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries);
+lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, const int32_t* pixels,
+                                  int32_t n, float* feat, float* tgt, int32_t cap, int32_t* n_rows);
 
 /* ---- debug / parity hooks (no counterpart in the reference: OptiX hides these) ---- */
 

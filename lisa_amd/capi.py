@@ -42,7 +42,7 @@ EXPORTS = (
     "lisa_batch_render", "lisa_batch_autotune",
     "lisa_cache_create", "lisa_cache_destroy", "lisa_cache_last_error", "lisa_cache_set_weights",
     "lisa_cache_get_weights", "lisa_cache_infer", "lisa_cache_gradients", "lisa_cache_train",
-    "lisa_cache_last_ms", "lisa_render_cached",
+    "lisa_cache_last_ms", "lisa_render_cached", "lisa_cache_trace_queries", "lisa_cache_trace_rows",
     "lisa_camera_uvw", "lisa_srgb_quantize", "lisa_save_ppm", "lisa_scene_file_load",
     "lisa_scene_file_free", "lisa_scene_file_info", "lisa_scene_file_vertices",
     "lisa_scene_file_normals", "lisa_scene_file_mat_indices", "lisa_scene_file_materials",
@@ -134,6 +134,10 @@ def lib():
         "lisa_cache_last_ms": (C.c_float, [vp]),
         "lisa_render_cached": (st, [vp, vp, C.POINTER(RenderParams), C.POINTER(CacheFrameParams),
                                     C.POINTER(CacheFrameStats)]),
+        "lisa_cache_trace_queries": (st, [vp, C.POINTER(RenderParams), C.c_uint32, i32p, C.c_int32,
+                                          f32p, f32p, f32p, i32p, C.c_int32, i32p]),
+        "lisa_cache_trace_rows": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, f32p, f32p,
+                                       C.c_int32, i32p]),
         "lisa_camera_uvw": (st, [f32p, f32p, C.c_float, C.c_float, f32p, f32p, f32p]),
         "lisa_srgb_quantize": (st, [f32p, C.c_int64, u8p]),
         "lisa_save_ppm": (st, [C.c_char_p, u8p, C.c_int32, C.c_int32]),
@@ -446,6 +450,37 @@ def render_cached(ctx, cache, params, cparams):
         msg = lib().lisa_last_error(ctx.h)
         raise LisaError(st, msg.decode() if msg else "")
     return {k: getattr(s, k) for k, _ in CacheFrameStats._fields_}
+
+
+def cache_trace_queries(ctx, params, prefix_bounces, pixels):
+    """Prefix paths on a pixel list (lisa_cache_trace_queries): accum, features, throughput, item."""
+    px = np.ascontiguousarray(pixels, dtype=np.int32)
+    n = px.shape[0]
+    cap = n * params.spp
+    acc = np.zeros((n, 4), np.float32)
+    f = np.zeros((cap, CACHE_INPUTS), np.float32)
+    at = np.zeros((cap, 3), np.float32)
+    it = np.zeros(cap, np.int32)
+    nq = C.c_int32()
+    ctx._ck(lib().lisa_cache_trace_queries(ctx.h, C.byref(params), prefix_bounces, _p(px, C.c_int32), n,
+                                           _p(acc, C.c_float), _p(f, C.c_float), _p(at, C.c_float),
+                                           _p(it, C.c_int32), cap, C.byref(nq)))
+    k = min(nq.value, cap)
+    return {"accum": acc, "feat": f[:k], "att": at[:k], "item": it[:k], "n": nq.value}
+
+
+def cache_trace_rows(ctx, params, pixels, cap=None):
+    """Training rows of plain paths on a pixel list (lisa_cache_trace_rows): features, targets."""
+    px = np.ascontiguousarray(pixels, dtype=np.int32)
+    n = px.shape[0]
+    cap = n * params.spp * params.bounces if cap is None else cap
+    f = np.zeros((cap, CACHE_INPUTS), np.float32)
+    t = np.zeros((cap, 3), np.float32)
+    nr = C.c_int32()
+    ctx._ck(lib().lisa_cache_trace_rows(ctx.h, C.byref(params), _p(px, C.c_int32), n, _p(f, C.c_float),
+                                        _p(t, C.c_float), cap, C.byref(nr)))
+    k = min(nr.value, cap)
+    return {"feat": f[:k], "tgt": t[:k], "n": nr.value}
 
 
 class Cache:

@@ -1469,6 +1469,100 @@ lisa_status refit_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
 
 }  // namespace
 
+// Scene box of the cache features: union of the root's two child boxes.
+lisa_status read_scene_box(lisa_ctx* ctx) {
+  const int64_t T = ctx->all.T;
+  if (T <= 0) return LISA_OK;
+  float rb[16];
+  HIP_TRY(ctx, hipMemcpy(rb, ctx->all.nodes, 64, hipMemcpyDeviceToHost));
+  const bool two = T > 1;
+  for (int a = 0; a < 3; ++a) {
+    ctx->bmin[a] = two ? std::min(rb[a], rb[6 + a]) : rb[a];
+    ctx->bmax[a] = two ? std::max(rb[3 + a], rb[9 + a]) : rb[3 + a];
+  }
+  return LISA_OK;
+}
+
+// Launch arguments of a cache-mode path loop: the plain ones plus the
+// position normalisation of the scene box (union of the root's child boxes).
+PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
+  PathArgs a = make_args(p);
+  a.bmin = mk(ctx->bmin[0], ctx->bmin[1], ctx->bmin[2]);
+  a.binv = mk(1.0f / std::max(ctx->bmax[0] - ctx->bmin[0], 1e-20f),
+              1.0f / std::max(ctx->bmax[1] - ctx->bmin[1], 1e-20f),
+              1.0f / std::max(ctx->bmax[2] - ctx->bmin[2], 1e-20f));
+  a.fp64 = 0;
+  return a;
+}
+
+// Training paths (path loop kMode 2) on n listed device pixels with a's spp
+// and bounces: rows (features, suffix targets) into ctx->c_tfeat / c_ttgt, at
+// most `cap`; *rows = rows produced (may exceed cap).
+lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, size_t cap,
+                             uint32_t* rows) {
+  hipStream_t st = ctx->stream;
+  const size_t slots = a.bounces;
+  HIP_TRY(ctx, ctx->c_trec.ensure(n * slots * 18));
+  HIP_TRY(ctx, ctx->c_tfeat.ensure(cap * 12));
+  HIP_TRY(ctx, ctx->c_ttgt.ensure(cap * 3));
+  HIP_TRY(ctx, ctx->c_tacc.ensure(n));
+  HIP_TRY(ctx, ctx->c_qcount.ensure(64));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p + 1, 0, 4, st));
+  a.pixels = d_pix;
+  a.n_items = (int32_t)n;
+  a.accum = ctx->c_tacc.p;
+  a.accum_prev = nullptr;
+  a.t_rec = ctx->c_trec.p;
+  a.t_slots = (uint32_t)slots;
+  a.t_feat = ctx->c_tfeat.p;
+  a.t_tgt = ctx->c_ttgt.p;
+  a.t_count = ctx->c_qcount.p + 1;
+  a.t_cap = (uint32_t)cap;
+  k_path_mode<2><<<grid_for((int64_t)n, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(rows, ctx->c_qcount.p + 1, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return LISA_OK;
+}
+
+// Prefix paths (path loop kMode 1): the full image in tiles (d_pix null) or
+// n listed device pixels; radiance gathered before the cache into `frame`
+// (per pixel mean), queries into ctx->c_q* (capacity n * spp); *nq = queries.
+lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, uint32_t prefix,
+                          float4* frame, uint32_t* nq) {
+  hipStream_t st = ctx->stream;
+  const size_t qcap = n * a.spp;
+  HIP_TRY(ctx, ctx->c_qfeat.ensure(qcap * 12));
+  HIP_TRY(ctx, ctx->c_qatt.ensure(qcap * 3));
+  HIP_TRY(ctx, ctx->c_qout.ensure(qcap * 3));
+  HIP_TRY(ctx, ctx->c_qitem.ensure(qcap));
+  HIP_TRY(ctx, ctx->c_qcount.ensure(64));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p, 0, 4, st));
+  a.pixels = d_pix;
+  a.n_items = (int32_t)n;
+  a.accum = frame;
+  a.accum_prev = nullptr;
+  a.bounces = prefix + 1;
+  a.cache_at = prefix + 1;
+  a.q_feat = ctx->c_qfeat.p;
+  a.q_att = ctx->c_qatt.p;
+  a.q_item = ctx->c_qitem.p;
+  a.q_count = ctx->c_qcount.p;
+  a.q_cap = (uint32_t)qcap;
+  unsigned blocks;
+  if (d_pix) {
+    blocks = grid_for((int64_t)n, kPathBlock);
+  } else {
+    a.tiles_x = (a.width + kTileW - 1) / kTileW;
+    blocks = a.tiles_x * ((a.height + kTileH - 1) / kTileH);
+  }
+  k_path_mode<1><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return LISA_OK;
+}
+
 extern "C" {
 
 lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float* normals) {
@@ -1495,7 +1589,7 @@ lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float*
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
-  return LISA_OK;
+  return read_scene_box(ctx);
 }
 
 lisa_status lisa_create(lisa_ctx** out, int32_t device) {
@@ -1604,15 +1698,8 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
   HIP_TRY(ctx, hipStreamSynchronize(st));
   HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
   ctx->scene_bytes += ctx->all.bytes + ctx->emit.bytes;
-  if (T > 0) {  // scene box = union of the root's child boxes
-    float rb[16];
-    HIP_TRY(ctx, hipMemcpy(rb, ctx->all.nodes, 64, hipMemcpyDeviceToHost));
-    const bool two = T > 1;
-    for (int a = 0; a < 3; ++a) {
-      ctx->bmin[a] = two ? std::min(rb[a], rb[6 + a]) : rb[a];
-      ctx->bmax[a] = two ? std::max(rb[3 + a], rb[9 + a]) : rb[3 + a];
-    }
-  }
+  rc = read_scene_box(ctx);
+  if (rc != LISA_OK) return rc;
   ctx->has_scene = true;
   return LISA_OK;
 }
@@ -2004,16 +2091,16 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
   if (p->flags & LISA_FLAG_COUNT) return fail(ctx, LISA_ERR_UNSUPPORTED, "no counters in cached renders");
-  if (cp->prefix_bounces == 0 || cp->prefix_bounces > 64)
-    return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 1..64");
+  if (cp->prefix_bounces > 63) return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 0..63");
   if (cp->train_paths > 0 && (cp->train_bounces == 0 || cp->train_bounces > 64))
     return fail(ctx, LISA_ERR_INVALID_ARG, "train_bounces must be in 1..64");
+  const size_t npx = (size_t)p->width * p->height;
+  if (npx * p->spp >= (1ull << 31)) return fail(ctx, LISA_ERR_INVALID_ARG, "width * height * spp >= 2^31");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const auto t_start = std::chrono::steady_clock::now();
   lisa_cache_frame_stats out_stats;
   std::memset(&out_stats, 0, sizeof(out_stats));
-  const size_t npx = (size_t)p->width * p->height;
   if (ctx->accum_px != npx) {
     if (p->subframe > 0) return fail(ctx, LISA_ERR_INVALID_ARG, "frame > 0 but the image size changed");
     HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -2024,28 +2111,17 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     ctx->accum_px = npx;
     ctx->accum_bytes = npx * sizeof(float4);
   }
-  PathArgs base = make_args(p);
-  base.bmin = mk(ctx->bmin[0], ctx->bmin[1], ctx->bmin[2]);
-  base.binv = mk(1.0f / std::max(ctx->bmax[0] - ctx->bmin[0], 1e-20f),
-                 1.0f / std::max(ctx->bmax[1] - ctx->bmin[1], 1e-20f),
-                 1.0f / std::max(ctx->bmax[2] - ctx->bmin[2], 1e-20f));
-  base.fp64 = 0;
+  PathArgs base = cache_args(ctx, p);
   float loss = 0.0f;
   // 1. online training on suffixes of full-length paths traced by the plain loop
   if (cp->train_paths > 0 && cp->train_steps > 0) {
     const auto t0 = std::chrono::steady_clock::now();
-    const size_t nt = cp->train_paths, slots = cp->train_bounces;
-    const size_t cap = cp->train_max > 0 ? cp->train_max : nt * slots;
-    HIP_TRY(ctx, ctx->c_trec.ensure(nt * slots * 18));
-    HIP_TRY(ctx, ctx->c_tfeat.ensure(cap * 12));
-    HIP_TRY(ctx, ctx->c_ttgt.ensure(cap * 3));
-    HIP_TRY(ctx, ctx->c_tacc.ensure(nt));
+    const size_t nt = cp->train_paths;
+    const size_t cap = cp->train_max > 0 ? cp->train_max : nt * cp->train_bounces;
     HIP_TRY(ctx, ctx->c_tpix.ensure(nt));
-    HIP_TRY(ctx, ctx->c_qcount.ensure(64));
-    HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p + 1, 0, 4, st));
     std::vector<int32_t> px(nt);
     uint64_t sm = cp->seed * 0x9E3779B97F4A7C15ull + p->subframe + 1;
-    for (size_t i = 0; i < nt; ++i) {
+    for (size_t i = 0; i < nt; ++i) {  // splitmix64
       sm += 0x9E3779B97F4A7C15ull;
       uint64_t z = sm;
       z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -2053,26 +2129,14 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
       z ^= z >> 31;
       px[i] = (int32_t)(z % npx);
     }
-    HIP_TRY(ctx, hipMemcpy(ctx->c_tpix.p, px.data(), 4 * nt, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->c_tpix.p, px.data(), 4 * nt, hipMemcpyHostToDevice, st));
     PathArgs a = base;
     a.spp = 1;
     a.bounces = cp->train_bounces;
     a.subframe = 0x40000000u ^ p->subframe;  // an RNG stream apart from the rendered samples
-    a.pixels = ctx->c_tpix.p;
-    a.n_items = (int32_t)nt;
-    a.accum = ctx->c_tacc.p;
-    a.accum_prev = nullptr;
-    a.t_rec = ctx->c_trec.p;
-    a.t_slots = (uint32_t)slots;
-    a.t_feat = ctx->c_tfeat.p;
-    a.t_tgt = ctx->c_ttgt.p;
-    a.t_count = ctx->c_qcount.p + 1;
-    a.t_cap = (uint32_t)cap;
-    k_path_mode<2><<<grid_for((int64_t)nt, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
-    HIP_TRY(ctx, hipGetLastError());
     uint32_t rows = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(&rows, ctx->c_qcount.p + 1, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipStreamSynchronize(st));
+    rc = trace_train_rows(ctx, a, ctx->c_tpix.p, nt, cap, &rows);
+    if (rc != LISA_OK) return rc;
     rows = std::min<uint32_t>(rows, (uint32_t)cap);
     const auto t1 = std::chrono::steady_clock::now();
     for (uint32_t k = 0; k < cp->train_steps && rows > 0; ++k) {
@@ -2090,7 +2154,6 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   // 2. this frame: cached paths, or the plain tracer as the fallback
   HIP_TRY(ctx, ctx->c_frame.ensure(npx));
   const bool fallback = cp->fallback_loss > 0.0f && loss > cp->fallback_loss;
-  const unsigned tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
   const auto t3 = std::chrono::steady_clock::now();
   if (fallback) {
     PathArgs a = base;
@@ -2098,36 +2161,16 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     a.n_items = (int32_t)npx;
     a.accum = ctx->c_frame.p;
     a.accum_prev = nullptr;
+    const unsigned tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
     k_path<false, false><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipStreamSynchronize(st));
     out_stats.fell_back = 1;
   } else {
-    const size_t qcap = npx * p->spp;
-    HIP_TRY(ctx, ctx->c_qfeat.ensure(qcap * 12));
-    HIP_TRY(ctx, ctx->c_qatt.ensure(qcap * 3));
-    HIP_TRY(ctx, ctx->c_qout.ensure(qcap * 3));
-    HIP_TRY(ctx, ctx->c_qitem.ensure(qcap));
-    HIP_TRY(ctx, ctx->c_qcount.ensure(64));
-    HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p, 0, 4, st));
-    PathArgs a = base;
-    a.tiles_x = (p->width + kTileW - 1) / kTileW;
-    a.n_items = (int32_t)npx;
-    a.accum = ctx->c_frame.p;
-    a.accum_prev = nullptr;
-    a.bounces = cp->prefix_bounces + 1;
-    a.cache_at = cp->prefix_bounces + 1;
-    a.q_feat = ctx->c_qfeat.p;
-    a.q_att = ctx->c_qatt.p;
-    a.q_item = ctx->c_qitem.p;
-    a.q_count = ctx->c_qcount.p;
-    a.q_cap = (uint32_t)qcap;
-    k_path_mode<1><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
-    HIP_TRY(ctx, hipGetLastError());
     uint32_t nq = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(&nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipStreamSynchronize(st));
-    nq = std::min<uint32_t>(nq, (uint32_t)qcap);
+    rc = trace_queries(ctx, base, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, &nq);
+    if (rc != LISA_OK) return rc;
+    nq = std::min<uint32_t>(nq, (uint32_t)(npx * p->spp));
     const auto t4 = std::chrono::steady_clock::now();
     if (nq > 0) {
       lisa_status r2 = lisa::cache_infer_device(cache, ctx->c_qfeat.p, nq, ctx->c_qout.p);
@@ -2148,6 +2191,71 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   HIP_TRY(ctx, hipStreamSynchronize(st));
   out_stats.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (stats) *stats = out_stats;
+  return LISA_OK;
+}
+
+namespace {
+// Host copy of a pixel list, checked against the image size.
+lisa_status upload_pixels(lisa_ctx* ctx, const lisa_render_params* p, const int32_t* pixels, int32_t n) {
+  if (n <= 0 || !pixels) return fail(ctx, LISA_ERR_INVALID_ARG, "pixels required, n > 0");
+  const int64_t npx = (int64_t)p->width * p->height;
+  for (int32_t i = 0; i < n; ++i)
+    if (pixels[i] < 0 || pixels[i] >= npx) return fail(ctx, LISA_ERR_INVALID_ARG, "pixel id out of range");
+  HIP_TRY(ctx, ctx->c_tpix.ensure((size_t)n));
+  HIP_TRY(ctx, hipMemcpy(ctx->c_tpix.p, pixels, 4 * (size_t)n, hipMemcpyHostToDevice));
+  return LISA_OK;
+}
+}  // namespace
+
+lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p, uint32_t prefix_bounces,
+                                     const int32_t* pixels, int32_t n, float* accum_out, float* q_feat,
+                                     float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  lisa_status rc = check_params(ctx, p);
+  if (rc != LISA_OK) return rc;
+  if (prefix_bounces > 63) return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 0..63");
+  if (!accum_out || !n_queries || q_cap < 0 || (q_cap > 0 && (!q_feat || !q_att || !q_item)))
+    return fail(ctx, LISA_ERR_INVALID_ARG, "output arrays");
+  if ((size_t)n * p->spp >= (1ull << 31)) return fail(ctx, LISA_ERR_INVALID_ARG, "n * spp >= 2^31");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  rc = upload_pixels(ctx, p, pixels, n);
+  if (rc != LISA_OK) return rc;
+  HIP_TRY(ctx, ctx->c_frame.ensure((size_t)n));
+  PathArgs a = cache_args(ctx, p);
+  uint32_t nq = 0;
+  rc = trace_queries(ctx, a, ctx->c_tpix.p, (size_t)n, prefix_bounces, ctx->c_frame.p, &nq);
+  if (rc != LISA_OK) return rc;
+  HIP_TRY(ctx, hipMemcpy(accum_out, ctx->c_frame.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  const size_t k = std::min<size_t>(nq, (size_t)q_cap);
+  if (k > 0) {
+    HIP_TRY(ctx, hipMemcpy(q_feat, ctx->c_qfeat.p, 48 * k, hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(q_att, ctx->c_qatt.p, 12 * k, hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(q_item, ctx->c_qitem.p, 4 * k, hipMemcpyDeviceToHost));
+  }
+  *n_queries = (int32_t)nq;
+  return LISA_OK;
+}
+
+lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, const int32_t* pixels,
+                                  int32_t n, float* feat, float* tgt, int32_t cap, int32_t* n_rows) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  lisa_status rc = check_params(ctx, p);
+  if (rc != LISA_OK) return rc;
+  if (p->bounces == 0 || p->bounces > 64) return fail(ctx, LISA_ERR_INVALID_ARG, "bounces must be in 1..64");
+  if (!n_rows || cap <= 0 || !feat || !tgt) return fail(ctx, LISA_ERR_INVALID_ARG, "output arrays");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  rc = upload_pixels(ctx, p, pixels, n);
+  if (rc != LISA_OK) return rc;
+  PathArgs a = cache_args(ctx, p);
+  uint32_t rows = 0;
+  rc = trace_train_rows(ctx, a, ctx->c_tpix.p, (size_t)n, (size_t)cap, &rows);
+  if (rc != LISA_OK) return rc;
+  const size_t k = std::min<size_t>(rows, (size_t)cap);
+  if (k > 0) {
+    HIP_TRY(ctx, hipMemcpy(feat, ctx->c_tfeat.p, 48 * k, hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(tgt, ctx->c_ttgt.p, 12 * k, hipMemcpyDeviceToHost));
+  }
+  *n_rows = (int32_t)rows;
   return LISA_OK;
 }
 

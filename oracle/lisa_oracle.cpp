@@ -596,11 +596,72 @@ struct SampleRec {
   }
 };
 
+// Position normalisation of the radiance-cache features: the scene box is
+// the union of the root's two child boxes (lisa_render_cached's set-up).
+struct CacheBox {
+  V3 bmin{0, 0, 0}, binv{1, 1, 1};
+};
+
+CacheBox cache_box(const Scene& s) {
+  CacheBox c;
+  if (s.T == 0) return c;
+  const float* b = s.all.boxes.data();
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = s.T > 1 ? std::min(b[a], b[6 + a]) : b[a];
+    hi[a] = s.T > 1 ? std::max(b[3 + a], b[9 + a]) : b[3 + a];
+  }
+  c.bmin = mk(lo[0], lo[1], lo[2]);
+  c.binv = mk(1.0f / std::max(hi[0] - lo[0], 1e-20f), 1.0f / std::max(hi[1] - lo[1], 1e-20f),
+              1.0f / std::max(hi[2] - lo[2], 1e-20f));
+  return c;
+}
+
+inline float clamp01(float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); }
+
+// The 12 cache inputs of a surface hit, each in [0, 1]: position in the
+// scene box (3), incoming direction and shading normal as (theta/pi, phi/2pi)
+// with theta measured from +y (2 + 2), albedo (3), roughness (0 for a
+// dielectric), dielectric flag.
+void cache_features(const CacheBox& c, V3 P, V3 D, V3 N, const orc_material& m, float* f) {
+  f[0] = clamp01((P.x - c.bmin.x) * c.binv.x);
+  f[1] = clamp01((P.y - c.bmin.y) * c.binv.y);
+  f[2] = clamp01((P.z - c.bmin.z) * c.binv.z);
+  const float dl = std::sqrt(dot(D, D));
+  const float dy = dl > 0.0f ? D.y / dl : 0.0f;
+  f[3] = std::acos(std::fmin(std::fmax(dy, -1.0f), 1.0f)) * 0.31830988618f;
+  f[4] = (std::atan2(D.z, D.x) + 3.14159265359f) * 0.15915494309f;
+  f[5] = std::acos(std::fmin(std::fmax(N.y, -1.0f), 1.0f)) * 0.31830988618f;
+  f[6] = (std::atan2(N.z, N.x) + 3.14159265359f) * 0.15915494309f;
+  const bool diel = m.alpha < 1.0f;
+  f[7] = clamp01(m.diffuse[0]);
+  f[8] = clamp01(m.diffuse[1]);
+  f[9] = clamp01(m.diffuse[2]);
+  f[10] = diel ? 0.0f : clamp01(m.roughness);
+  f[11] = diel ? 1.0f : 0.0f;
+}
+
+// A surface hit of a training path: features, throughput on arrival and the
+// radiance gathered before it.
+struct CacheVertex {
+  float f[12];
+  V3 att, color;
+};
+
 struct Tracer {
   const Scene& s;
   Counters& cnt;
   uint32_t& seed;
   SampleRec& rec;
+  // radiance-cache modes: cache_at > 0 ends the path at the surface hit of
+  // radiance ray number cache_at with a query (features + throughput);
+  // verts != nullptr records every surface hit (training paths).
+  int32_t cache_at = 0;
+  const CacheBox* box = nullptr;
+  std::vector<CacheVertex>* verts = nullptr;
+  bool queried = false;
+  float qf[12] = {};
+  V3 qatt{0, 0, 0};
 
   const orc_material& mat_of(int32_t prim) const { return s.mats[s.mat[prim]]; }
 
@@ -658,6 +719,20 @@ struct Tracer {
       const float* v = &s.v[9 * (size_t)p];
       const float* n = &s.n[9 * (size_t)p];
       st.normal = bary_normal(st.origin, ld3(n), ld3(n + 3), ld3(n + 6), ld3(v), ld3(v + 3), ld3(v + 6));
+    }
+    if (cache_at > 0 && rec.bounces == cache_at) {
+      cache_features(*box, st.origin, D, st.normal, m, qf);
+      qatt = st.att;
+      queried = true;
+      st.done = true;
+      return;
+    }
+    if (verts) {
+      CacheVertex cv;
+      cache_features(*box, st.origin, D, st.normal, m, cv.f);
+      cv.att = st.att;
+      cv.color = st.color;
+      verts->push_back(cv);
     }
     if (m.alpha < 1.0f) {
       float cosI = dot(D, st.normal);
@@ -916,6 +991,123 @@ void orc_closest_hit_brute(const orc_scene* o, const float* O, const float* D, i
     prim_out[i] = p;
     t_out[i] = p >= 0 ? t : 0.0f;
   }
+}
+
+void orc_cache_bounds(const orc_scene* o, float* bmin, float* binv) {
+  const CacheBox c = cache_box(o->s);
+  st3(bmin, c.bmin);
+  st3(binv, c.binv);
+}
+
+int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t prefix_bounces,
+                      const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
+                      float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads) {
+  const Scene& s = o->s;
+  if (p->width == 0 || p->height == 0 || p->spp == 0 || n <= 0) return -1;
+  const int64_t npx = (int64_t)p->width * p->height;
+  for (int32_t i = 0; i < n; ++i)
+    if (pixel_ids[i] < 0 || pixel_ids[i] >= npx) return -2;
+  const CacheBox box = cache_box(s);
+  const size_t spp = p->spp;
+  std::vector<uint8_t> has(spp * n, 0);
+  std::vector<float> f(12 * spp * n), at(3 * spp * n);
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+  {
+    Counters cnt;
+#pragma omp for schedule(dynamic, 4)
+    for (int32_t i = 0; i < n; ++i) {
+      const uint32_t pix = (uint32_t)pixel_ids[i];
+      const uint32_t x = pix % p->width, y = pix / p->width;
+      uint32_t seed = tea16(y * p->width + x, p->subframe);
+      const V3 eye = ld3(p->eye), U = ld3(p->U), V = ld3(p->V), W = ld3(p->W);
+      V3 accum = mk(0.0f, 0.0f, 0.0f);
+      for (size_t k = 0; k < spp; ++k) {
+        const float jx = rng(seed);
+        const float jy = rng(seed);
+        const float dx = (2.0f * (float)x + jx) / (float)p->width - 1.0f;
+        const float dy = (2.0f * (float)y + jy) / (float)p->height - 1.0f;
+        const V3 D = normalize(dx * U + dy * V + W);
+        SampleRec rec;
+        Tracer tr{s, cnt, seed, rec};
+        tr.cache_at = (int32_t)prefix_bounces + 1;
+        tr.box = &box;
+        const V3 c = tr.trace(eye, D, (int)prefix_bounces + 1);
+        accum = accum + c;
+        if (tr.queried) {
+          const size_t q = (size_t)i * spp + k;
+          has[q] = 1;
+          std::copy(tr.qf, tr.qf + 12, &f[12 * q]);
+          st3(&at[3 * q], tr.qatt);
+        }
+      }
+      st3(accum_out + 4 * (size_t)i, accum * (1.0f / (float)p->spp));
+      accum_out[4 * (size_t)i + 3] = 1.0f;
+    }
+  }
+  int32_t nq = 0;
+  for (size_t q = 0; q < has.size(); ++q) {
+    if (!has[q]) continue;
+    if (nq < q_cap) {
+      std::copy(&f[12 * q], &f[12 * q] + 12, q_feat + 12 * (size_t)nq);
+      std::copy(&at[3 * q], &at[3 * q] + 3, q_att + 3 * (size_t)nq);
+      q_item[nq] = (int32_t)(q / spp);
+    }
+    ++nq;
+  }
+  *n_queries = nq;
+  return 0;
+}
+
+int orc_cache_train_rows(const orc_scene* o, const orc_render_params* p, const int32_t* pixel_ids,
+                         int32_t n, float* feat, float* tgt, int32_t* row_path, int32_t cap,
+                         int32_t* n_rows, int threads) {
+  const Scene& s = o->s;
+  if (p->width == 0 || p->height == 0 || p->spp == 0 || p->bounces == 0 || n <= 0) return -1;
+  const int64_t npx = (int64_t)p->width * p->height;
+  for (int32_t i = 0; i < n; ++i)
+    if (pixel_ids[i] < 0 || pixel_ids[i] >= npx) return -2;
+  const CacheBox box = cache_box(s);
+  std::vector<std::vector<float>> rows(n);  // 15 floats per row: features, target
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+  {
+    Counters cnt;
+#pragma omp for schedule(dynamic, 4)
+    for (int32_t i = 0; i < n; ++i) {
+      const uint32_t pix = (uint32_t)pixel_ids[i];
+      const uint32_t x = pix % p->width, y = pix / p->width;
+      uint32_t seed = tea16(y * p->width + x, p->subframe);
+      const V3 eye = ld3(p->eye), U = ld3(p->U), V = ld3(p->V), W = ld3(p->W);
+      for (uint32_t k = 0; k < p->spp; ++k) {
+        const float jx = rng(seed);
+        const float jy = rng(seed);
+        const float dx = (2.0f * (float)x + jx) / (float)p->width - 1.0f;
+        const float dy = (2.0f * (float)y + jy) / (float)p->height - 1.0f;
+        const V3 D = normalize(dx * U + dy * V + W);
+        SampleRec rec;
+        std::vector<CacheVertex> verts;
+        Tracer tr{s, cnt, seed, rec};
+        tr.box = &box;
+        tr.verts = &verts;
+        const V3 c = tr.trace(eye, D, (int)p->bounces);
+        for (const CacheVertex& v : verts) {  // suffix radiance per unit throughput
+          rows[i].insert(rows[i].end(), v.f, v.f + 12);
+          rows[i].push_back((c.x - v.color.x) / v.att.x);
+          rows[i].push_back((c.y - v.color.y) / v.att.y);
+          rows[i].push_back((c.z - v.color.z) / v.att.z);
+        }
+      }
+    }
+  }
+  int32_t nr = 0;
+  for (int32_t i = 0; i < n; ++i)
+    for (size_t r = 0; r < rows[i].size() / 15; ++r, ++nr) {
+      if (nr >= cap) continue;
+      std::copy(&rows[i][15 * r], &rows[i][15 * r] + 12, feat + 12 * (size_t)nr);
+      std::copy(&rows[i][15 * r + 12], &rows[i][15 * r] + 15, tgt + 3 * (size_t)nr);
+      if (row_path) row_path[nr] = i;
+    }
+  *n_rows = nr;
+  return 0;
 }
 
 int orc_render(const orc_scene* o, const orc_render_params* p, const int32_t* pixel_ids,

@@ -112,9 +112,29 @@ void orc_closest_hit(const orc_scene* s, const float* O, const float* D, int32_t
 void orc_closest_hit_brute(const orc_scene* s, const float* O, const float* D, int32_t n,
                            float tmin, float tmax, int32_t* prim_out, float* t_out);
 
+/* --- radiance-cache path modes (lisa_render_cached) ---
+ * Scene box of the position features: min corner and 1 / extent per axis. */
+void orc_cache_bounds(const orc_scene* s, float* bmin, float* binv);
+/* Paths of p->spp samples on the listed pixels, prefix_bounces + 1 radiance
+ * rays at most: the surface hit of radiance ray prefix_bounces + 1 ends the
+ * path in a query (12 features, throughput).  accum_out (4n): per-pixel mean
+ * of the radiance gathered before the queries.  Queries in (pixel, sample)
+ * order; q_item = index into pixel_ids; *n_queries may exceed q_cap. */
+int orc_cache_queries(const orc_scene* s, const orc_render_params* p, uint32_t prefix_bounces,
+                      const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
+                      float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads);
+/* Training rows of plain paths (p->bounces) on the listed pixels: one row per
+ * surface hit, 12 features and the target (final - gathered before the hit)
+ * / throughput on arrival; in (pixel, sample, vertex) order, row_path = index
+ * into pixel_ids (may be NULL); *n_rows may exceed cap. */
+int orc_cache_train_rows(const orc_scene* s, const orc_render_params* p, const int32_t* pixel_ids,
+                         int32_t n, float* feat, float* tgt, int32_t* row_path, int32_t cap,
+                         int32_t* n_rows, int threads);
+
 /* Render pixels (all if pixel_ids == NULL, else n_pixels listed ids).
- * accum_out: float4 per listed pixel (rgb, 1).  accum_prev (same layout) is read
- * when subframe > 0.  Per-sample records are optional (NULL to skip):
+ * accum_out: float4 per listed pixel (rgb, 1).  accum_prev (same layout), when
+ * given and subframe > 0, is blended as a running mean; without it the
+ * subframe only seeds the RNG.  Per-sample records are optional (NULL to skip):
  *   sample_rgb[n*spp*3], sample_bounces[n*spp], sample_occl[n*spp],
  *   sample_hash[n*spp], rad_hits[n*spp*record_bounces]. */
 int orc_render(const orc_scene* s, const orc_render_params* p,

@@ -91,6 +91,13 @@ def lib():
         L.orc_render.restype = C.c_int
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), i32p, C.c_int32, f32p, f32p,
                                  f32p, i32p, i32p, u32p, i32p, u64p, C.c_int]
+        L.orc_cache_bounds.argtypes = [C.c_void_p, f32p, f32p]
+        L.orc_cache_queries.restype = C.c_int
+        L.orc_cache_queries.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_uint32, i32p,
+                                        C.c_int32, f32p, f32p, f32p, i32p, C.c_int32, i32p, C.c_int]
+        L.orc_cache_train_rows.restype = C.c_int
+        L.orc_cache_train_rows.argtypes = [C.c_void_p, C.POINTER(RenderParams), i32p, C.c_int32,
+                                           f32p, f32p, i32p, C.c_int32, i32p, C.c_int]
         _lib = L
     return _lib
 
@@ -287,6 +294,60 @@ class Scene:
                                   _p(prim, C.c_int32), _p(t, C.c_float), _p(cnt, C.c_uint64),
                                   threads)
         return prim, t, cnt
+
+    def cache_bounds(self):
+        """(bmin, 1 / extent) of the cache's position features."""
+        bmin = np.zeros(3, np.float32)
+        binv = np.zeros(3, np.float32)
+        lib().orc_cache_bounds(self.h, _p(bmin, C.c_float), _p(binv, C.c_float))
+        return bmin, binv
+
+    @staticmethod
+    def _params(width, height, spp, bounces, eye, U, V, W, subframe):
+        p = RenderParams()
+        p.width, p.height, p.spp, p.bounces, p.subframe = width, height, spp, bounces, subframe
+        p.eye[:], p.U[:], p.V[:], p.W[:] = list(eye), list(U), list(V), list(W)
+        p.record_bounces = 0
+        return p
+
+    def cache_queries(self, width, height, spp, prefix_bounces, eye, U, V, W, pixels, subframe=0,
+                      threads=1):
+        """Prefix paths ending in cache queries (orc_cache_queries)."""
+        p = self._params(width, height, spp, prefix_bounces + 1, eye, U, V, W, subframe)
+        px = np.ascontiguousarray(pixels, dtype=np.int32)
+        n = px.shape[0]
+        cap = n * spp
+        acc = np.zeros((n, 4), np.float32)
+        f = np.zeros((cap, 12), np.float32)
+        at = np.zeros((cap, 3), np.float32)
+        it = np.zeros(cap, np.int32)
+        nq = C.c_int32()
+        rc = lib().orc_cache_queries(self.h, C.byref(p), prefix_bounces, _p(px, C.c_int32), n,
+                                     _p(acc, C.c_float), _p(f, C.c_float), _p(at, C.c_float),
+                                     _p(it, C.c_int32), cap, C.byref(nq), threads)
+        if rc != 0:
+            raise RuntimeError(f"orc_cache_queries failed: {rc}")
+        k = nq.value
+        return {"accum": acc, "feat": f[:k], "att": at[:k], "item": it[:k], "n": k}
+
+    def cache_train_rows(self, width, height, spp, bounces, eye, U, V, W, pixels, subframe=0,
+                         threads=1):
+        """Training rows of plain paths (orc_cache_train_rows)."""
+        p = self._params(width, height, spp, bounces, eye, U, V, W, subframe)
+        px = np.ascontiguousarray(pixels, dtype=np.int32)
+        n = px.shape[0]
+        cap = n * spp * bounces
+        f = np.zeros((cap, 12), np.float32)
+        t = np.zeros((cap, 3), np.float32)
+        path = np.zeros(cap, np.int32)
+        nr = C.c_int32()
+        rc = lib().orc_cache_train_rows(self.h, C.byref(p), _p(px, C.c_int32), n, _p(f, C.c_float),
+                                        _p(t, C.c_float), _p(path, C.c_int32), cap, C.byref(nr),
+                                        threads)
+        if rc != 0:
+            raise RuntimeError(f"orc_cache_train_rows failed: {rc}")
+        k = nr.value
+        return {"feat": f[:k], "tgt": t[:k], "path": path[:k], "n": k}
 
     def render(self, width, height, spp, bounces, eye, U, V, W, subframe=0, pixels=None,
                accum_prev=None, record=False, record_bounces=None, threads=1):

@@ -1,0 +1,164 @@
+"""GPU: the radiance-cache path forms of lisa_render_cached against the CPU
+oracle (orc_cache_queries, orc_cache_train_rows) and the plain tracer.
+
+Bars: query items, throughputs, position / albedo / roughness / dielectric
+features and training targets are bit-exact (the path loop is, and the
+features are FP32 arithmetic without transcendental functions); the
+direction and normal features go through acosf / atan2f, whose GPU and glibc
+results may differ by a few ulps: |diff| <= 2e-6 (the features lie in
+[0, 1]).  Per-pixel means: 1e-5 relative, as in the plain path parity.
+Queries and rows come out of the GPU in atomic order, so both sides are
+sorted by (item, exact features) before the comparison.
+"""
+import numpy as np
+import pytest
+
+from tests.scene_util import camera, cornell
+
+pytestmark = pytest.mark.gpu
+
+EXACT = [0, 1, 2, 7, 8, 9, 10, 11]
+TRIG = [3, 4, 5, 6]
+
+
+@pytest.fixture(scope="module")
+def ctx(capi):
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def _scene(capi, oracle_mod, ctx, W=64, H=64):
+    sc = cornell(32, 16, W, H, 4, 7)
+    ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    return sc, o, camera(sc, capi)
+
+
+def _order(item, feat):
+    keys = [feat[:, k] for k in (2, 1, 0)] + [item]
+    return np.lexsort(keys)
+
+
+def _close(a, b, rtol=1e-5):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.all(np.abs(a - b) <= rtol * np.abs(b) + 1e-30)
+
+
+@pytest.mark.parametrize("spp,prefix", [(1, 2), (4, 2), (2, 1), (2, 0)])
+def test_queries_match_oracle(capi, oracle_mod, ctx, spp, prefix):
+    W = H = 64
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    px = np.sort(np.random.default_rng(spp).choice(W * H, 1500, replace=False)).astype(np.int32)
+    p = capi.render_params(W, H, spp, prefix + 1, eye, U, V, Wv, subframe=3)
+    g = capi.cache_trace_queries(ctx, p, prefix, px)
+    r = o.cache_queries(W, H, spp, prefix, eye, U, V, Wv, px, subframe=3, threads=8)
+    print("queries", g["n"], r["n"], flush=True)
+    assert g["n"] == r["n"] > 0
+    gi, ri = _order(g["item"], g["feat"]), _order(r["item"], r["feat"])
+    assert np.array_equal(g["item"][gi], r["item"][ri])
+    assert np.array_equal(g["feat"][gi][:, EXACT], r["feat"][ri][:, EXACT])
+    assert np.abs(g["feat"][gi][:, TRIG] - r["feat"][ri][:, TRIG]).max() <= 2e-6
+    assert np.array_equal(g["att"][gi], r["att"][ri])
+    assert _close(g["accum"], r["accum"])
+
+
+def test_training_rows_match_oracle(capi, oracle_mod, ctx):
+    W = H = 64
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    px = np.random.default_rng(9).integers(0, W * H, 1024).astype(np.int32)  # repeats allowed
+    p = capi.render_params(W, H, 2, 7, eye, U, V, Wv, subframe=5)
+    g = capi.cache_trace_rows(ctx, p, px)
+    r = o.cache_train_rows(W, H, 2, 7, eye, U, V, Wv, px, subframe=5, threads=8)
+    print("rows", g["n"], r["n"], flush=True)
+    assert g["n"] == r["n"] > 1024
+    gi = np.lexsort([g["tgt"][:, 0], g["feat"][:, 2], g["feat"][:, 1], g["feat"][:, 0]])
+    ri = np.lexsort([r["tgt"][:, 0], r["feat"][:, 2], r["feat"][:, 1], r["feat"][:, 0]])
+    assert np.array_equal(g["feat"][gi][:, EXACT], r["feat"][ri][:, EXACT])
+    assert np.abs(g["feat"][gi][:, TRIG] - r["feat"][ri][:, TRIG]).max() <= 2e-6
+    # targets bit-exact, including the non-finite ones of zero-throughput channels
+    assert np.array_equal(g["tgt"][gi], r["tgt"][ri], equal_nan=True)
+    # a cap keeps the first rows only and reports the full count
+    g2 = capi.cache_trace_rows(ctx, p, px, cap=100)
+    assert g2["n"] == g["n"] and g2["feat"].shape[0] == 100
+
+
+def test_forced_fallback_is_the_plain_progressive_render(capi, oracle_mod, ctx):
+    """With a fallback threshold below any training loss every frame is the
+    plain tracer, and the running mean over frames is lisa_render's
+    progressive subframe mean, bit for bit."""
+    W = H = 64
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    cache = capi.Cache(0, seed=1, lr=1e-3)
+    plain = capi.Context(0)
+    try:
+        plain.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        cp = capi.cache_frame_params(prefix_bounces=2, train_paths=512, train_bounces=7,
+                                     train_steps=1, fallback_loss=1e-30, seed=2)
+        for f in range(3):
+            p = capi.render_params(W, H, 2, 7, eye, U, V, Wv, subframe=f)
+            st = capi.render_cached(ctx, cache, p, cp)
+            assert st["fell_back"] == 1 and st["train_samples"] > 0
+            want, _ = plain.render(p)
+            got = ctx.copy_accum(W * H)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f
+    finally:
+        plain.close()
+        cache.close()
+
+
+def _cached_rmse(capi, ctx, cache, cp, params_of, frames, ref):
+    for f in range(frames):
+        st = capi.render_cached(ctx, cache, params_of(f), cp)
+        assert st["fell_back"] == 0
+    img = ctx.copy_accum(ref.shape[0])
+    assert np.all(np.isfinite(img)) and np.all(img[:, :3] >= 0)
+    return float(np.sqrt(np.mean((img[:, :3].astype(np.float64) - ref) ** 2))), st
+
+
+def test_trained_cache_gains_over_truncated_paths(capi, oracle_mod, ctx):
+    """What the trained network gains: the same prefix paths (2 bounces, then
+    the cache) with the network's answer replaced by 0 are plain paths cut
+    after 3 radiance rays, and the Cornell box loses its indirect light.
+    Trained online for 48 frames, the network recovers most of it: the RMSE
+    against an 8192-spp plain render drops well below the cut paths' and the
+    untrained network's."""
+    W = H = 64
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    plain = capi.Context(0)
+    try:
+        plain.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        ref = plain.trace_subset(capi.render_params(W, H, 8192, 7, eye, U, V, Wv, subframe=77),
+                                 np.arange(W * H, dtype=np.int32), records=False)["accum"]
+        ref = ref[:, :3].astype(np.float64)
+    finally:
+        plain.close()
+
+    def params_of(f):
+        return capi.render_params(W, H, 4, 7, eye, U, V, Wv, subframe=f)
+
+    frames = 48
+    trained = capi.Cache(0, seed=1, lr=1e-3)
+    untrained = capi.Cache(0, seed=1, lr=1e-3)
+    zero = capi.Cache(0, seed=1, lr=1e-3)
+    try:
+        zero.set_weights(np.zeros(capi.CACHE_PARAMS, np.float32))
+        cp = capi.cache_frame_params(prefix_bounces=2, train_paths=4096, train_bounces=7,
+                                     train_steps=2, seed=2)
+        off = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_bounces=7,
+                                      train_steps=0, seed=2)
+        e_tr, st = _cached_rmse(capi, ctx, trained, cp, params_of, frames, ref)
+        assert st["queries"] > 0 and st["train_samples"] > 0
+        e_un, _ = _cached_rmse(capi, ctx, untrained, off, params_of, frames, ref)
+        e_0, _ = _cached_rmse(capi, ctx, zero, off, params_of, frames, ref)
+        print(f"RMSE vs 8192-spp plain: trained {e_tr:.5f}, untrained {e_un:.5f}, "
+              f"cut paths (zero cache) {e_0:.5f}", flush=True)
+        assert e_tr < 0.5 * e_0
+        assert e_tr < 0.5 * e_un
+    finally:
+        for c in (trained, untrained, zero):
+            c.close()

@@ -278,3 +278,59 @@ def test_batched_wavefront_bit_exact(capi, oracle_mod):
     batch.close()
     for c in ctxs:
         c.close()
+
+
+def test_converged_within_3_sigma(capi, oracle_mod, ctx):
+    """Independent seeds: the GPU's 256-spp pixel means and the oracle's
+    (another subframe, so another RNG stream) agree within the per-pixel
+    3-sigma Monte-Carlo bound, sigma from each side's own per-sample spread.
+    Pixels with zero spread on both sides must agree exactly."""
+    W = H = 64
+    spp = 256
+    sc = cornell(32, 16, W, H, spp, 7)
+    o = _pair(capi, oracle_mod, ctx, sc)
+    U, V, Wv = camera(sc, capi)
+    eye = sc["camera"]["eye"]
+    px = np.arange(W * H, dtype=np.int32)
+    g = ctx.trace_subset(capi.render_params(W, H, spp, 7, eye, U, V, Wv, subframe=0), px,
+                         record_bounces=0)
+    r = o.render(W, H, spp, 7, eye, U, V, Wv, subframe=1, pixels=px, record=True,
+                 record_bounces=0, threads=8)
+    m1 = g["sample_rgb"].astype(np.float64).mean(1)
+    m2 = r["sample_rgb"].astype(np.float64).mean(1)
+    s1 = g["sample_rgb"].astype(np.float64).std(1, ddof=1) / np.sqrt(spp)
+    s2 = r["sample_rgb"].astype(np.float64).std(1, ddof=1) / np.sqrt(spp)
+    sig = np.sqrt(s1 ** 2 + s2 ** 2)
+    flat = sig == 0
+    assert np.array_equal(m1[flat], m2[flat])
+    z = np.abs(m1 - m2)[~flat] / sig[~flat]
+    frac = float(np.mean(z > 3))
+    print("3-sigma outliers", frac, "of", z.size, flush=True)
+    assert frac < 0.01
+
+
+def test_rmse_against_16k_spp_oracle(capi, oracle_mod, ctx):
+    """RMSE of GPU renders (64 .. 1024 spp) against a 16384-spp oracle render
+    of 128 seeded pixels with an independent seed: it falls as 1/sqrt(spp)
+    within the bound set by the measured per-sample variance."""
+    W = H = 64
+    sc = cornell(32, 16, W, H, 16, 7)
+    o = _pair(capi, oracle_mod, ctx, sc)
+    U, V, Wv = camera(sc, capi)
+    eye = sc["camera"]["eye"]
+    px = np.sort(np.random.default_rng(17).choice(W * H, 128, replace=False)).astype(np.int32)
+    ref = o.render(W, H, 16384, 7, eye, U, V, Wv, subframe=99, pixels=px, threads=8)["accum"]
+    ref = ref[:, :3].astype(np.float64)
+    var = None
+    errs = {}
+    for spp in (64, 256, 1024):
+        g = ctx.trace_subset(capi.render_params(W, H, spp, 7, eye, U, V, Wv, subframe=0), px,
+                             record_bounces=0)
+        if var is None:
+            var = float(np.mean(g["sample_rgb"].astype(np.float64).var(1, ddof=1)))
+        errs[spp] = float(np.sqrt(np.mean((g["accum"][:, :3] - ref) ** 2)))
+    print("RMSE vs 16k-spp oracle:", errs, "per-sample variance", var, flush=True)
+    for spp, e in errs.items():
+        expect = np.sqrt(var / spp + var / 16384)
+        assert 0.5 * expect < e < 1.6 * expect, (spp, e, expect)
+    assert errs[1024] < errs[64] / 2.5
