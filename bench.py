@@ -39,6 +39,8 @@ WORKLOADS = {
     "cornell65k": ("cornell_65k", 256, 65536),
     "icosphere": ("displaced_icosphere", 64, 2048),
     "batch64": ("batch_scene", 32, 0),
+    "island": ("island", 0, 0),
+    "cornell_cache": ("cornell_65k", 0, 0),
 }
 
 
@@ -52,7 +54,15 @@ def parse():
     ap.add_argument("--cpu-pixels", type=int, default=None)
     ap.add_argument("--workload", default="cornell65k", choices=sorted(WORKLOADS),
                     help="cornell65k: the headline configuration; icosphere: the "
-                         "traversal-bound 5.2M-triangle scene")
+                         "traversal-bound 5.2M-triangle scene; batch64: the batched "
+                         "wavefront form; island / cornell_cache: time to RMSE <= 1e-2, "
+                         "learned cache vs plain")
+    ap.add_argument("--cache-spp", type=int, default=4, help="spp per cached frame")
+    ap.add_argument("--prefix", type=int, default=2, help="radiance rays before the cache")
+    ap.add_argument("--train-paths", type=int, default=16384)
+    ap.add_argument("--train-steps", type=int, default=2)
+    ap.add_argument("--rmse-limit-s", type=float, default=30.0,
+                    help="give up a method after this much render time")
     return ap.parse_args()
 
 
@@ -82,6 +92,8 @@ def main():
 
     if a.workload == "batch64":
         return bench_batch(a)
+    if a.workload in ("island", "cornell_cache"):
+        return bench_rmse(a)
     factory, def_spp, def_cpu_px = WORKLOADS[a.workload]
     sc = getattr(scenes, factory)()
     W, H, B = sc["width"], sc["height"], sc["bounces"]
@@ -154,6 +166,148 @@ def main():
            "rays_per_step": rays, "bvh_build_ms": round(bvh_ms, 3),
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+RMSE_TARGET = 1e-2
+REF_PIXELS = 65536
+MLP_WEIGHTS = 21696  # 144*64 + 3*64*64 + 64*3 (no biases)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_rmse(a):
+    """Time to RMSE <= 1e-2 against a 16k-spp plain render: learned cache vs plain.
+
+    Reference: 16 x 1024 spp with FP64 accumulation on 65536 seeded pixels
+    (the plain loop on a pixel list; subframes 1000..1015 seed it apart from
+    every measured render); the RMSE of an image is taken over those pixels
+    (an unbiased estimate of the full-image mean squared error).  Plain: the
+    spp sweep over powers of two, each a single full-frame render; its time
+    is the first render that reaches the target.  Plain progressive: 16-spp
+    subframes, time at the first subframe that reaches it.  Cache: frames of
+    lisa_render_cached (online training + `--cache-spp` samples of paths cut
+    after `--prefix` radiance rays), time summed over frames up to the first
+    frame count 2^k that reaches it.  Times are wall time around the
+    synchronous render calls; reading the image back for the RMSE is not
+    timed."""
+    import numpy as np
+    from lisa_amd import capi, scenes
+
+    factory = WORKLOADS[a.workload][0]
+    sc = getattr(scenes, factory)()
+    W, H = sc["width"], sc["height"]
+    B = 12 if a.workload == "island" else sc["bounces"]
+    cam = sc["camera"]
+    U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+    ctx = capi.Context(0)
+    ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    pix = scenes.seeded_pixels(W, H, REF_PIXELS, seed=7)
+    t = time.perf_counter()
+    ref = np.zeros((REF_PIXELS, 3))
+    for k in range(16):
+        p = capi.render_params(W, H, 1024, B, cam["eye"], U, V, Wv, subframe=1000 + k,
+                               flags=capi.FLAG_FP64_ACCUM)
+        ref += ctx.trace_subset(p, pix, records=False)["accum"][:, :3].astype(np.float64) / 16
+        print(f"[bench] reference chunk {k + 1}/16", file=sys.stderr, flush=True)
+    ref_s = time.perf_counter() - t
+
+    def rmse(img):
+        return float(np.sqrt(np.mean((img[pix, :3].astype(np.float64) - ref) ** 2)))
+
+    # plain: power-of-two spp sweep, one render each
+    sweep = []
+    plain_t = None
+    spp = 1
+    while spp <= 65536:
+        p = capi.render_params(W, H, spp, B, cam["eye"], U, V, Wv)
+        ctx.synchronize()
+        t = time.perf_counter()
+        acc, _ = ctx.render(p)
+        dt = time.perf_counter() - t
+        e = rmse(acc)
+        sweep.append({"spp": spp, "s": round(dt, 4), "rmse": round(e, 6)})
+        print(f"[bench] plain {spp} spp: {dt:.3f} s, rmse {e:.5f}", file=sys.stderr, flush=True)
+        if e <= RMSE_TARGET:
+            plain_t = dt
+            break
+        if dt > a.rmse_limit_s:
+            break
+        spp *= 2
+    # plain progressive, 16 spp per subframe
+    prog_t, tot = None, 0.0
+    for f in range(100000):
+        p = capi.render_params(W, H, 16, B, cam["eye"], U, V, Wv, subframe=f)
+        t = time.perf_counter()
+        ctx.render(p, want_accum=False)
+        ctx.synchronize()
+        tot += time.perf_counter() - t
+        if rmse(ctx.copy_accum(W * H)) <= RMSE_TARGET:
+            prog_t = tot
+            break
+        if tot > a.rmse_limit_s:
+            break
+    print(f"[bench] plain progressive: {prog_t}", file=sys.stderr, flush=True)
+    # cache
+    cache = capi.Cache(0, seed=1, lr=1e-3)
+    cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
+                                 train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
+                                 seed=3)
+    cache_t, tot, frames, last = None, 0.0, 0, None
+    for f in range(1 << 20):
+        p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
+        t = time.perf_counter()
+        last = capi.render_cached(ctx, cache, p, cp)
+        tot += time.perf_counter() - t
+        frames = f + 1
+        if frames & (frames - 1) == 0 or frames % 16 == 0:
+            e = rmse(ctx.copy_accum(W * H))
+            if e <= RMSE_TARGET:
+                cache_t = tot
+                break
+        if tot > a.rmse_limit_s:
+            break
+    e_cache = rmse(ctx.copy_accum(W * H))
+    print(f"[bench] cache: {cache_t} s after {frames} frames, rmse {e_cache:.5f}, last {last}",
+          file=sys.stderr, flush=True)
+    mlp = None
+    if last is not None and last["queries"] > 0:
+        # algorithmic FLOPs: 2 per real weight per query forward, 3x that per
+        # training row (forward + backward data + weight gradients)
+        f_q = 2 * MLP_WEIGHTS
+        inf_tf = last["queries"] * f_q / (last["infer_kernel_ms"] * 1e-3) / 1e12
+        tr_tf = (last["train_samples"] * 3 * f_q * a.train_steps
+                 / max(last["train_kernel_ms"] * 1e-3, 1e-12) / 1e12)
+        mlp = {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "infer_queries": last["queries"], "infer_kernel_ms": round(last["infer_kernel_ms"], 4),
+               "infer_achieved": round(inf_tf, 2),
+               "infer_frac": round(inf_tf / MFMA_BF16_PEAK_TFLOPS, 5),
+               "train_rows": last["train_samples"], "train_steps": a.train_steps,
+               "train_kernel_ms": round(last["train_kernel_ms"], 4),
+               "train_achieved": round(tr_tf, 2),
+               "train_frac": round(tr_tf / MFMA_BF16_PEAK_TFLOPS, 5)}
+    out = {"metric": "time to RMSE<=1e-2 vs 16k-spp plain reference (s)",
+           "value": None if cache_t is None else round(cache_t, 4), "unit": "s", "n_gpus": 1,
+           "steps": frames, "warmup": 0,
+           "ms_per_step": round(tot * 1e3 / max(frames, 1), 3),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": f"{sc['name']} learned cache vs plain", "width": W, "height": H,
+                      "bounces": B, "triangles": ctx.T, "cache_prefix_bounces": a.prefix,
+                      "cache_spp_per_frame": a.cache_spp, "train_paths": a.train_paths,
+                      "train_rows_max": 1 << 16, "train_steps": a.train_steps, "lr": 1e-3,
+                      "parallelism": "single GPU"},
+           "plain_time_s": None if plain_t is None else round(plain_t, 4),
+           "plain_progressive_time_s": None if prog_t is None else round(prog_t, 4),
+           "speedup_vs_plain": (None if (plain_t is None or cache_t is None)
+                                else round(plain_t / cache_t, 3)),
+           "speedup_vs_plain_progressive": (None if (prog_t is None or cache_t is None)
+                                            else round(prog_t / cache_t, 3)),
+           "cache_frames": frames, "cache_rmse": round(e_cache, 6), "plain_sweep": sweep,
+           "reference": {"spp": 16384, "pixels": REF_PIXELS, "fp64": True,
+                         "seconds": round(ref_s, 2)},
+           "last_frame": last, "mlp_roofline": mlp, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+    cache.close()
     ctx.close()
 
 

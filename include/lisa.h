@@ -185,9 +185,11 @@ float lisa_cache_last_ms(const lisa_cache* c);
 
 /* One frame of the cached renderer on ctx's scene (all on one device):
  *  1. online training: train_paths full-length paths (train_bounces, the
- *     plain path loop, pixels drawn with `seed`) record every surface hit;
- *     each hit's suffix radiance (final - gathered so far) / throughput is a
- *     training row (at most train_max rows); train_steps Adam steps on them;
+ *     plain path loop, pixels drawn with `seed`) record the surface hit of
+ *     radiance ray prefix_bounces + 1, where the frame's paths query the
+ *     cache; its suffix radiance (final - gathered before it) / throughput
+ *     is a training row (at most train_max rows), a target with the same
+ *     remaining bounce budget as the queries; train_steps Adam steps on them;
  *  2. the frame: p->spp samples per pixel whose radiance ray number
  *     prefix_bounces + 1 ends at a surface hit by adding
  *     throughput * max(cache(features), 0) (misses and lights stay exact) --
@@ -206,9 +208,11 @@ typedef struct lisa_cache_frame_params {
 } lisa_cache_frame_params;
 
 typedef struct lisa_cache_frame_stats {
-  float trace_train_ms, train_ms, render_ms, infer_ms, total_ms;
+  float trace_train_ms, train_ms, render_ms, infer_ms, total_ms;  /* host wall time */
   float loss;
   int32_t queries, train_samples, fell_back;
+  float train_kernel_ms; /* HIP-event time of the training steps (forward, backward, Adam) */
+  float infer_kernel_ms; /* HIP-event time of the inference launch */
 } lisa_cache_frame_stats;
 
 lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_render_params* p,
@@ -229,14 +233,16 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
  * *n_queries may exceed q_cap (the rest is dropped).
  *
  * lisa_cache_trace_rows: plain paths (p->spp, p->bounces) on the listed
- * pixels; one row per surface hit: features and the target (path radiance -
- * radiance gathered before the hit) / throughput on arrival.  Rows of one
- * path are contiguous, paths in no fixed order; *n_rows may exceed cap. */
+ * pixels; one row per surface hit (depth = 0) or per surface hit of radiance
+ * ray number `depth`: features and the target (path radiance - radiance
+ * gathered before the hit) / throughput on arrival.  Rows of one path are
+ * contiguous, paths in no fixed order; *n_rows may exceed cap. */
 lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p, uint32_t prefix_bounces,
                                      const int32_t* pixels, int32_t n, float* accum_out, float* q_feat,
                                      float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries);
 lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, const int32_t* pixels,
-                                  int32_t n, float* feat, float* tgt, int32_t cap, int32_t* n_rows);
+                                  int32_t n, uint32_t depth, float* feat, float* tgt, int32_t cap,
+                                  int32_t* n_rows);
 
 /* ---- debug / parity hooks (no counterpart in the reference: OptiX hides these) ---- */
 

@@ -136,8 +136,8 @@ def lib():
                                     C.POINTER(CacheFrameStats)]),
         "lisa_cache_trace_queries": (st, [vp, C.POINTER(RenderParams), C.c_uint32, i32p, C.c_int32,
                                           f32p, f32p, f32p, i32p, C.c_int32, i32p]),
-        "lisa_cache_trace_rows": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, f32p, f32p,
-                                       C.c_int32, i32p]),
+        "lisa_cache_trace_rows": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, C.c_uint32,
+                                       f32p, f32p, C.c_int32, i32p]),
         "lisa_camera_uvw": (st, [f32p, f32p, C.c_float, C.c_float, f32p, f32p, f32p]),
         "lisa_srgb_quantize": (st, [f32p, C.c_int64, u8p]),
         "lisa_save_ppm": (st, [C.c_char_p, u8p, C.c_int32, C.c_int32]),
@@ -433,7 +433,8 @@ class CacheFrameParams(C.Structure):
 class CacheFrameStats(C.Structure):
     _fields_ = [("trace_train_ms", C.c_float), ("train_ms", C.c_float), ("render_ms", C.c_float),
                 ("infer_ms", C.c_float), ("total_ms", C.c_float), ("loss", C.c_float),
-                ("queries", C.c_int32), ("train_samples", C.c_int32), ("fell_back", C.c_int32)]
+                ("queries", C.c_int32), ("train_samples", C.c_int32), ("fell_back", C.c_int32),
+                ("train_kernel_ms", C.c_float), ("infer_kernel_ms", C.c_float)]
 
 
 def cache_frame_params(prefix_bounces=2, train_paths=8192, train_bounces=12, train_max=1 << 16,
@@ -469,15 +470,17 @@ def cache_trace_queries(ctx, params, prefix_bounces, pixels):
     return {"accum": acc, "feat": f[:k], "att": at[:k], "item": it[:k], "n": nq.value}
 
 
-def cache_trace_rows(ctx, params, pixels, cap=None):
-    """Training rows of plain paths on a pixel list (lisa_cache_trace_rows): features, targets."""
+def cache_trace_rows(ctx, params, pixels, cap=None, depth=0):
+    """Training rows of plain paths on a pixel list (lisa_cache_trace_rows): features, targets
+    of every surface hit (depth 0) or of radiance ray number `depth`'s hits."""
     px = np.ascontiguousarray(pixels, dtype=np.int32)
     n = px.shape[0]
     cap = n * params.spp * params.bounces if cap is None else cap
     f = np.zeros((cap, CACHE_INPUTS), np.float32)
     t = np.zeros((cap, 3), np.float32)
     nr = C.c_int32()
-    ctx._ck(lib().lisa_cache_trace_rows(ctx.h, C.byref(params), _p(px, C.c_int32), n, _p(f, C.c_float),
+    ctx._ck(lib().lisa_cache_trace_rows(ctx.h, C.byref(params), _p(px, C.c_int32), n, depth,
+                                        _p(f, C.c_float),
                                         _p(t, C.c_float), cap, C.byref(nr)))
     k = min(nr.value, cap)
     return {"feat": f[:k], "tgt": t[:k], "n": nr.value}

@@ -2,7 +2,7 @@
 // (v_mfma_f32_16x16x32_bf16, FP32 accumulation) for gfx950.
 //
 //   k_mlp_infer      frequency encoding + 5-layer forward, 16 queries per
-//                    wave-tile, weights resident in LDS
+//                    wave-tile, weights and activations in registers
 //   k_mlp_train      the same forward with every activation kept, the
 //                    relative-L2 loss gradient and the backward data pass
 //                    (dZ_l = (dZ_{l+1} W_{l+1}^T) * [A_{l+1} > 0]); writes the
@@ -32,6 +32,7 @@ using namespace lisa::mlp;
 namespace {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
@@ -164,44 +165,158 @@ __device__ inline f32x4 layer_fwd(int l, const uint16_t* in, int in_stride, cons
   return acc_last;
 }
 
+// ---- inference with the weights and activations in registers ----
+// Each layer computes Z^T = W^T X^T: the weights are the A operand (held in
+// VGPRs for the whole launch), the activations the B operand.  Lane (q, r)
+// then holds D[4q .. 4q+3][r] = Z[row r][16 nt + 4q .. 4q+3], which after
+// ReLU and BF16 rounding is its own B operand of the next layer if the K
+// order of that layer's weights is permuted to match: K-step ks, lane q,
+// slot j <-> hidden feature
+//   h(ks, q, j) = 32 ks + 4 q + j           (j < 4)
+//               = 32 ks + 16 + 4 q + j - 4  (j >= 4).
+// Layer 0 uses e(ks, q, j) = 36 q + 8 ks + j (slots with 8 ks + j >= 36 are
+// zero): lane q encodes inputs 3q .. 3q+2 of row r, the (sin, cos) pairs of
+// 6 octaves, which the encoded-feature layout 12 i + 2 b + {0, 1} places at
+// 36 q .. 36 q + 35.  Activations never leave the lane.
 constexpr int kInferWaves = 4;
+
+__device__ inline uint32_t pack_bf16(float a, float b) {
+  bf16x2 v;
+  v[0] = (__bf16)a;  // round to nearest even
+  v[1] = (__bf16)b;
+  return __builtin_bit_cast(uint32_t, v);
+}
+
+// A fragment of hidden layer l (1..4): row 16 nt + r of W_l^T, K-step ks.
+__device__ inline u32x4 wfrag_hidden(const uint16_t* __restrict__ wt, int l, int nt, int ks, int q,
+                                     int r) {
+  const uint16_t* row = wt + kOffset[l] + (nt * 16 + r) * kInDim[l] + 32 * ks + 4 * q;
+  const uint2 a = *reinterpret_cast<const uint2*>(row);
+  const uint2 b = *reinterpret_cast<const uint2*>(row + 16);
+  return u32x4{a.x, a.y, b.x, b.y};
+}
+
+// A fragment of layer 0: row 16 nt + r of W_0^T at encoded features
+// 36 q + 8 ks .. + 7 (only 4 of them for ks = 4).
+__device__ inline u32x4 wfrag_in(const uint16_t* __restrict__ wt, int nt, int ks, int q, int r) {
+  const uint32_t* row =
+      reinterpret_cast<const uint32_t*>(wt + (nt * 16 + r) * kEncPad + 36 * q + 8 * ks);
+  if (ks < 4) return u32x4{row[0], row[1], row[2], row[3]};
+  return u32x4{row[0], row[1], 0u, 0u};
+}
+
+// ReLU + BF16 of two D tiles (N tiles 2 ks, 2 ks + 1) into the B operand of K-step ks.
+__device__ inline u32x4 relu_pack(f32x4 lo, f32x4 hi) {
+  return u32x4{pack_bf16(fmaxf(lo[0], 0.0f), fmaxf(lo[1], 0.0f)),
+               pack_bf16(fmaxf(lo[2], 0.0f), fmaxf(lo[3], 0.0f)),
+               pack_bf16(fmaxf(hi[0], 0.0f), fmaxf(hi[1], 0.0f)),
+               pack_bf16(fmaxf(hi[2], 0.0f), fmaxf(hi[3], 0.0f))};
+}
+
+__device__ inline bf16x8 bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __restrict__ feat,
                                                                int64_t n,
                                                                const uint16_t* __restrict__ wt,
                                                                float* __restrict__ out) {
-  __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtLds];
-  __shared__ __attribute__((aligned(16))) uint16_t s_x[kInferWaves][16 * kXStride];
-  __shared__ __attribute__((aligned(16))) uint16_t s_h[kInferWaves][2][16 * kHStride];
-  load_weights(wt, nullptr, s_wt, nullptr);
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = lane >> 4, r = lane & 15;
-  uint16_t* X = s_x[wave];
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
   const int64_t tiles = (n + 15) / 16;
-  for (int64_t t = (int64_t)blockIdx.x * kInferWaves + wave; t < tiles;
-       t += (int64_t)gridDim.x * kInferWaves) {
-    const int64_t base = t * 16;
-    encode_tile(feat, kInputs, base, n, X, lane);
-    __builtin_amdgcn_wave_barrier();
-    const uint16_t* in = X;
-    int in_stride = kXStride;
-    for (int l = 0; l < kLayers - 1; ++l) {
-      uint16_t* o = s_h[wave][l & 1];
-      layer_fwd(l, in, in_stride, s_wt, o, lane);
-      __builtin_amdgcn_wave_barrier();
-      in = o;
-      in_stride = kHStride;
-    }
-    const f32x4 y = layer_fwd(kLayers - 1, in, in_stride, s_wt, nullptr, lane);
-    if (r < kOut)
+  const int64_t first = (int64_t)blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+  const int64_t stride = (int64_t)gridDim.x * kInferWaves;
+  if (first >= tiles) return;
+  u32x4 w0[4][5], w1[4][2], w2[4][2], w3[4][2], w4[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int64_t m = base + 4 * q + i;
-        if (m < n) out[m * kOut + r] = y[i];
-      }
-    __builtin_amdgcn_wave_barrier();
+  for (int nt = 0; nt < 4; ++nt) {
+#pragma unroll
+    for (int ks = 0; ks < 5; ++ks) w0[nt][ks] = wfrag_in(wt, nt, ks, q, r);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      w1[nt][ks] = wfrag_hidden(wt, 1, nt, ks, q, r);
+      w2[nt][ks] = wfrag_hidden(wt, 2, nt, ks, q, r);
+      w3[nt][ks] = wfrag_hidden(wt, 3, nt, ks, q, r);
+    }
   }
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) w4[ks] = wfrag_hidden(wt, 4, 0, ks, q, r);
+
+  // inputs 3q .. 3q+2 of row r of a tile
+  auto load_inputs = [&](int64_t t, float* p) {
+    const int64_t row = t * 16 + r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p[k] = row < n ? feat[row * kInputs + 3 * q + k] : 0.0f;
+  };
+  float p[3];
+  load_inputs(first, p);
+  for (int64_t t = first; t < tiles; t += stride) {
+    float pn[3] = {0.0f, 0.0f, 0.0f};
+    if (t + stride < tiles) load_inputs(t + stride, pn);
+    // frequency encoding: one sincospi per input, then the double-angle
+    // recurrence sin 2x = 2 sin x cos x, cos 2x = cos^2 x - sin^2 x
+    float e[40];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float sn, cs;
+      sincospif(p[k], &sn, &cs);
+#pragma unroll
+      for (int b = 0; b < kBands; ++b) {
+        e[12 * k + 2 * b] = sn;
+        e[12 * k + 2 * b + 1] = cs;
+        const float s2 = 2.0f * sn * cs;
+        const float c2 = cs * cs - sn * sn;
+        sn = s2;
+        cs = c2;
+      }
+    }
+    e[36] = e[37] = e[38] = e[39] = 0.0f;
+    u32x4 x[5];
+#pragma unroll
+    for (int ks = 0; ks < 5; ++ks)
+      x[ks] = u32x4{pack_bf16(e[8 * ks], e[8 * ks + 1]), pack_bf16(e[8 * ks + 2], e[8 * ks + 3]),
+                    pack_bf16(e[8 * ks + 4], e[8 * ks + 5]), pack_bf16(e[8 * ks + 6], e[8 * ks + 7])};
+    f32x4 z[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      z[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 5; ++ks) z[nt] = mfma(bf(w0[nt][ks]), bf(x[ks]), z[nt]);
+    }
+    u32x4 h[2];
+    auto hidden = [&](u32x4 (&w)[4][2]) {
+      h[0] = relu_pack(z[0], z[1]);
+      h[1] = relu_pack(z[2], z[3]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        z[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) z[nt] = mfma(bf(w[nt][ks]), bf(h[ks]), z[nt]);
+      }
+    };
+    hidden(w1);
+    hidden(w2);
+    hidden(w3);
+    h[0] = relu_pack(z[0], z[1]);
+    h[1] = relu_pack(z[2], z[3]);
+    f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+    y = mfma(bf(w4[0]), bf(h[0]), y);
+    y = mfma(bf(w4[1]), bf(h[1]), y);
+    const int64_t row = t * 16 + r;
+    if (q == 0 && row < n) {
+      out[row * kOut] = y[0];
+      out[row * kOut + 1] = y[1];
+      out[row * kOut + 2] = y[2];
+    }
+    p[0] = pn[0];
+    p[1] = pn[1];
+    p[2] = pn[2];
+  }
+}
+
+// Blocks of the inference launch: enough waves for every tile, at most two
+// blocks per CU (each wave loads the 47 KB of weight fragments once).
+int infer_blocks(int cus, int64_t n) {
+  const int64_t tiles = (n + 15) / 16;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves,
+                                                     2 * (int64_t)cus));
 }
 
 constexpr int kTrainWaves = 2;
@@ -388,6 +503,7 @@ struct lisa_cache {
   float *feat = nullptr, *tgt = nullptr, *out = nullptr;
   int64_t io_cap = 0;
   float last_ms = 0.0f;
+  int cus = 256;  // compute units of the device (inference grid)
 };
 
 namespace {
@@ -532,7 +648,9 @@ lisa_status lisa_cache_create(lisa_cache** out, int32_t device, uint64_t seed, f
       (e = hipMalloc(reinterpret_cast<void**>(&c->wt), 2 * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->wb), 2 * kWbGlobal)) != hipSuccess ||
       (e = hipMemset(c->m, 0, 4 * kParams)) != hipSuccess ||
-      (e = hipMemset(c->v, 0, 4 * kParams)) != hipSuccess) {
+      (e = hipMemset(c->v, 0, 4 * kParams)) != hipSuccess ||
+      (e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device)) !=
+          hipSuccess) {
     const std::string msg = std::string("lisa_cache_create: ") + hipGetErrorString(e);
     lisa_cache_destroy(c);
     return cfail(nullptr, LISA_ERR_HIP, msg);
@@ -593,10 +711,8 @@ lisa_status lisa_cache_infer(lisa_cache* c, const float* feat, int64_t n, float*
   lisa_status rc = ensure_io(c, n);
   if (rc != LISA_OK) return rc;
   CHIP(c, hipMemcpy(c->feat, feat, 4 * kInputs * (size_t)n, hipMemcpyHostToDevice));
-  const int64_t tiles = (n + 15) / 16;
-  const int blocks = (int)std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, 4096);
   CHIP(c, hipEventRecord(c->ev0, c->stream));
-  k_mlp_infer<<<blocks, 64 * kInferWaves, 0, c->stream>>>(c->feat, n, c->wt, c->out);
+  k_mlp_infer<<<infer_blocks(c->cus, n), 64 * kInferWaves, 0, c->stream>>>(c->feat, n, c->wt, c->out);
   CHIP(c, hipGetLastError());
   CHIP(c, hipEventRecord(c->ev1, c->stream));
   CHIP(c, hipStreamSynchronize(c->stream));
@@ -670,10 +786,8 @@ lisa_status lisa::cache_infer_device(lisa_cache* c, const float* d_feat, int64_t
   if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
   if (n <= 0) return LISA_OK;
   CHIP(c, hipSetDevice(c->device));
-  const int64_t tiles = (n + 15) / 16;
-  const int blocks = (int)std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, 4096);
   CHIP(c, hipEventRecord(c->ev0, c->stream));
-  k_mlp_infer<<<blocks, 64 * kInferWaves, 0, c->stream>>>(d_feat, n, c->wt, d_out);
+  k_mlp_infer<<<infer_blocks(c->cus, n), 64 * kInferWaves, 0, c->stream>>>(d_feat, n, c->wt, d_out);
   CHIP(c, hipGetLastError());
   CHIP(c, hipEventRecord(c->ev1, c->stream));
   CHIP(c, hipStreamSynchronize(c->stream));

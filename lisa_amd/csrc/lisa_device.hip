@@ -303,7 +303,7 @@ struct PathArgs {
   uint32_t tiles_x;
   float4* accum;
   const float4* accum_prev;
-  int32_t fp64;
+  int32_t fp64;  // FP64 per-pixel sums: the launch picks the k_path<.., .., true> instantiation
   float* s_rgb;
   int32_t* s_b;
   int32_t* s_o;
@@ -324,6 +324,7 @@ struct PathArgs {
   float* t_tgt;       // kMode 2: 3 target floats per row
   uint32_t* t_count;  // kMode 2: rows appended (may pass t_cap; rows past it are dropped)
   uint32_t t_cap;
+  uint32_t t_depth;   // kMode 2: > 0 records only the hits of radiance ray number t_depth
   F3 bmin, binv;      // position normalisation for the features
 };
 
@@ -362,12 +363,13 @@ struct Job {
 // kMode 0: the plain path loop.  kMode 1: the radiance ray numbered
 // p.cache_at ends at a surface hit by queuing a cache query (features, the
 // path throughput, the pixel); misses and emitters are handled exactly.
-// kMode 2: every surface hit of the (full-length) path records its features,
+// kMode 2: every surface hit of the (full-length) path (or, with t_depth > 0,
+// the hit of radiance ray number t_depth only) records its features,
 // throughput and the radiance gathered so far; at the end of the sample the
 // suffix radiance (final - so far) / throughput of each vertex becomes a
 // training row.  Without accum_prev the launch writes its own mean (no
 // running mean over subframes; the subframe then only picks the RNG stream).
-template <bool kRecord, bool kCount, class Src, int kMode = 0>
+template <bool kRecord, bool kCount, class Src, int kMode = 0, bool kF64 = false>
 __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   Job job;
   if (!src.next(job)) return;
@@ -377,8 +379,6 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   int32_t item = job.item;
 
   uint32_t seed = tea16(y * p.width + x, p.subframe);  // shader.cu:141
-  float fx = (float)x, fy = (float)y;
-  float sx = (float)p.width, sy = (float)p.height;
 
   uint64_t c_rad = 0, c_occ = 0, c_rhit = 0, c_inv = 0, c_smp = 0, c_px = 0;
   TravCount tc;
@@ -387,7 +387,9 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0;
 
   // Per-lane path state (RayState, shader.cu:30-42, plus the NEE loop index).
-  F3 O = mk(0.0f, 0.0f, 0.0f), D = O, color = O, att = O, N = O, P = O, Din = O;
+  // O is the ray origin, and from a surface hit on the hit point (the origin
+  // of the NEE rays and of the next bounce).
+  F3 O = mk(0.0f, 0.0f, 0.0f), D = O, color = O, att = O, N = O, Din = O;
   bool hitf = false;
   int32_t qstage = 0;  // 0 radiance ray, 1 NEE emitter query, 2 NEE any-hit query
   int32_t light_mat = 0, tries = 0, matE = -1;
@@ -419,7 +421,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     }
     if (kCount) c_smp++;
     accum = accum + color;
-    if (p.fp64) {
+    if (kF64) {
       ad0 += (double)color.x;
       ad1 += (double)color.y;
       ad2 += (double)color.z;
@@ -442,8 +444,8 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     while (s < p.spp) {
       const float jx = rng(seed);
       const float jy = rng(seed);
-      const float dx = (2.0f * fx + jx) / sx - 1.0f;
-      const float dy = (2.0f * fy + jy) / sy - 1.0f;
+      const float dx = (2.0f * (float)x + jx) / (float)p.width - 1.0f;
+      const float dy = (2.0f * (float)y + jy) / (float)p.height - 1.0f;
       D = normalize(dx * p.U + dy * p.V + p.W);
       O = p.eye;
       color = mk(0.0f, 0.0f, 0.0f);
@@ -467,7 +469,6 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   auto end_nee = [&](F3 nee) -> bool {
     color = color + nee * att;
     D = Lambertian::sample(Din, N, seed, rough);
-    O = P;
     qstage = 0;
     if (b == p.bounces) {
       finish_sample();
@@ -489,7 +490,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // shader.cu:156-166: the pixel's mean (and running mean over subframes).
   auto write_pixel = [&]() {
     F3 out;
-    if (p.fp64) {
+    if (kF64) {
       const double inv = 1.0 / (double)p.spp;
       out = mk((float)(ad0 * inv), (float)(ad1 * inv), (float)(ad2 * inv));
     } else {
@@ -513,10 +514,6 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       y = job.y;
       item = job.item;
       seed = tea16(y * p.width + x, p.subframe);
-      fx = (float)x;
-      fy = (float)y;
-      sx = (float)p.width;
-      sy = (float)p.height;
       accum = mk(0.0f, 0.0f, 0.0f);
       ad0 = ad1 = ad2 = 0.0;
       s = 0;
@@ -606,18 +603,18 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           finish = true;
         } else {
           if (kCount) c_rhit++;
-          P = O + t * D;
+          O = O + t * D;
           Din = D;
           {
             const float4 t2 = sc.all.tris[3 * (size_t)k + 2];
             const float4* nq = sc.nrm + 3 * (size_t)k;
-            N = bary_normal(P, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
+            N = bary_normal(O, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
           }
           bool cached = false;
           if (kMode == 1 && b == p.cache_at) {
             const uint32_t slot = atomicAdd(p.q_count, 1u);
             if (slot < p.q_cap) {
-              cache_features(p, P, Din, N, m, p.q_feat + 12 * (size_t)slot);
+              cache_features(p, O, Din, N, m, p.q_feat + 12 * (size_t)slot);
               p.q_att[3 * (size_t)slot] = att.x;
               p.q_att[3 * (size_t)slot + 1] = att.y;
               p.q_att[3 * (size_t)slot + 2] = att.z;
@@ -626,9 +623,9 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
             cached = true;
             finish = true;
           }
-          if (kMode == 2 && t_used < p.t_slots) {
+          if (kMode == 2 && t_used < p.t_slots && (p.t_depth == 0 || b == p.t_depth)) {
             float* r = p.t_rec + 18 * ((size_t)item * p.t_slots + t_used);
-            cache_features(p, P, Din, N, m, r);
+            cache_features(p, O, Din, N, m, r);
             r[12] = att.x;
             r[13] = att.y;
             r[14] = att.z;
@@ -640,14 +637,12 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           if (cached) {
           } else if (m.diffuse_alpha.w < 1.0f) {
             D = Dielectric::scatter(D, N, m.rough_n.y, xyz(m.diffuse_alpha), att, seed);
-            O = P;
             if (b == p.bounces) finish = true;
           } else {
             att = att * xyz(m.diffuse_alpha);
             rough = m.rough_n.x;
             qstage = 1;
             tries = 0;
-            O = P;
             D = hemisphere(N, seed);
           }
         }
@@ -716,11 +711,11 @@ struct OnePixel {
   }
 };
 
-template <bool kRecord, bool kCount>
+template <bool kRecord, bool kCount, bool kF64 = false>
 __global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   OnePixel src{sc, p, false};
-  path_lane<kRecord, kCount>(src, stk + threadIdx.x);
+  path_lane<kRecord, kCount, OnePixel, 0, kF64>(src, stk + threadIdx.x);
 }
 
 // Path loop in a cache mode (1: cached render of the full image in tiles,
@@ -1496,10 +1491,11 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
 }
 
 // Training paths (path loop kMode 2) on n listed device pixels with a's spp
-// and bounces: rows (features, suffix targets) into ctx->c_tfeat / c_ttgt, at
+// and bounces: rows (features, suffix targets) of every surface hit, or of the
+// hits of radiance ray number `depth` > 0 only, into ctx->c_tfeat / c_ttgt, at
 // most `cap`; *rows = rows produced (may exceed cap).
 lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, size_t cap,
-                             uint32_t* rows) {
+                             uint32_t depth, uint32_t* rows) {
   hipStream_t st = ctx->stream;
   const size_t slots = a.bounces;
   HIP_TRY(ctx, ctx->c_trec.ensure(n * slots * 18));
@@ -1518,6 +1514,7 @@ lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, si
   a.t_tgt = ctx->c_ttgt.p;
   a.t_count = ctx->c_qcount.p + 1;
   a.t_cap = (uint32_t)cap;
+  a.t_depth = depth;
   k_path_mode<2><<<grid_for((int64_t)n, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipMemcpyAsync(rows, ctx->c_qcount.p + 1, 4, hipMemcpyDeviceToHost, st));
@@ -1735,8 +1732,12 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters, 0, LISA_NCOUNTERS * sizeof(unsigned long long), st));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
   const unsigned grid = a.tiles_x * tiles_y;
-  if (count)
+  if (count && a.fp64)
+    k_path<false, true, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
+  else if (count)
     k_path<false, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
+  else if (a.fp64)
+    k_path<false, false, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
   else
     k_path<false, false><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
   HIP_TRY(ctx, hipGetLastError());
@@ -1807,7 +1808,10 @@ lisa_status lisa_trace_subset(lisa_ctx* ctx, const lisa_render_params* p, const 
   hipStream_t st = ctx->stream;
   HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters, 0, LISA_NCOUNTERS * sizeof(unsigned long long), st));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  k_path<true, true><<<grid_for(n_pixels, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
+  if (a.fp64)
+    k_path<true, true, true><<<grid_for(n_pixels, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
+  else
+    k_path<true, true><<<grid_for(n_pixels, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -1935,6 +1939,8 @@ lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,
     }
     if (params[i].subframe > 0) return fail(nullptr, LISA_ERR_UNSUPPORTED, "batched renders use subframe 0");
     if (params[i].flags & LISA_FLAG_COUNT) return fail(nullptr, LISA_ERR_UNSUPPORTED, "no counters in batches");
+    if (params[i].flags & LISA_FLAG_FP64_ACCUM)
+      return fail(nullptr, LISA_ERR_UNSUPPORTED, "no FP64 accumulation in batches");
     for (int32_t k = 0; k < i; ++k)
       if (ctxs[k] == c) return fail(nullptr, LISA_ERR_INVALID_ARG, "a context appears twice in the batch");
     if (hipSetDevice(dev) != hipSuccess) return fail(nullptr, LISA_ERR_HIP, "hipSetDevice");
@@ -2134,8 +2140,8 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     a.spp = 1;
     a.bounces = cp->train_bounces;
     a.subframe = 0x40000000u ^ p->subframe;  // an RNG stream apart from the rendered samples
-    uint32_t rows = 0;
-    rc = trace_train_rows(ctx, a, ctx->c_tpix.p, nt, cap, &rows);
+    uint32_t rows = 0;  // rows at the query depth: targets with the queries' remaining bounces
+    rc = trace_train_rows(ctx, a, ctx->c_tpix.p, nt, cap, cp->prefix_bounces + 1, &rows);
     if (rc != LISA_OK) return rc;
     rows = std::min<uint32_t>(rows, (uint32_t)cap);
     const auto t1 = std::chrono::steady_clock::now();
@@ -2143,6 +2149,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
       float l = 0.0f;
       lisa_status r2 = lisa::cache_train_device(cache, ctx->c_tfeat.p, 12, ctx->c_ttgt.p, (int64_t)rows, &l);
       if (r2 != LISA_OK) return fail(ctx, r2, std::string("cache training: ") + lisa_cache_last_error(cache));
+      out_stats.train_kernel_ms += lisa_cache_last_ms(cache);
       loss = l;
     }
     const auto t2 = std::chrono::steady_clock::now();
@@ -2175,6 +2182,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     if (nq > 0) {
       lisa_status r2 = lisa::cache_infer_device(cache, ctx->c_qfeat.p, nq, ctx->c_qout.p);
       if (r2 != LISA_OK) return fail(ctx, r2, std::string("cache inference: ") + lisa_cache_last_error(cache));
+      out_stats.infer_kernel_ms = lisa_cache_last_ms(cache);
       k_cache_add<<<(nq + 255) / 256, 256, 0, st>>>(ctx->c_qout.p, ctx->c_qatt.p, ctx->c_qitem.p, nq,
                                                     1.0f / (float)p->spp, ctx->c_frame.p);
       HIP_TRY(ctx, hipGetLastError());
@@ -2237,7 +2245,8 @@ lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
 }
 
 lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, const int32_t* pixels,
-                                  int32_t n, float* feat, float* tgt, int32_t cap, int32_t* n_rows) {
+                                  int32_t n, uint32_t depth, float* feat, float* tgt, int32_t cap,
+                                  int32_t* n_rows) {
   if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
@@ -2248,7 +2257,7 @@ lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, co
   if (rc != LISA_OK) return rc;
   PathArgs a = cache_args(ctx, p);
   uint32_t rows = 0;
-  rc = trace_train_rows(ctx, a, ctx->c_tpix.p, (size_t)n, (size_t)cap, &rows);
+  rc = trace_train_rows(ctx, a, ctx->c_tpix.p, (size_t)n, (size_t)cap, depth, &rows);
   if (rc != LISA_OK) return rc;
   const size_t k = std::min<size_t>(rows, (size_t)cap);
   if (k > 0) {

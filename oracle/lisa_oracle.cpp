@@ -659,6 +659,7 @@ struct Tracer {
   int32_t cache_at = 0;
   const CacheBox* box = nullptr;
   std::vector<CacheVertex>* verts = nullptr;
+  int32_t vert_depth = 0;  // > 0: record only the hits of radiance ray number vert_depth
   bool queried = false;
   float qf[12] = {};
   V3 qatt{0, 0, 0};
@@ -727,7 +728,7 @@ struct Tracer {
       st.done = true;
       return;
     }
-    if (verts) {
+    if (verts && (vert_depth == 0 || rec.bounces == vert_depth)) {
       CacheVertex cv;
       cache_features(*box, st.origin, D, st.normal, m, cv.f);
       cv.att = st.att;
@@ -1059,8 +1060,8 @@ int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t p
 }
 
 int orc_cache_train_rows(const orc_scene* o, const orc_render_params* p, const int32_t* pixel_ids,
-                         int32_t n, float* feat, float* tgt, int32_t* row_path, int32_t cap,
-                         int32_t* n_rows, int threads) {
+                         int32_t n, uint32_t depth, float* feat, float* tgt, int32_t* row_path,
+                         int32_t cap, int32_t* n_rows, int threads) {
   const Scene& s = o->s;
   if (p->width == 0 || p->height == 0 || p->spp == 0 || p->bounces == 0 || n <= 0) return -1;
   const int64_t npx = (int64_t)p->width * p->height;
@@ -1088,6 +1089,7 @@ int orc_cache_train_rows(const orc_scene* o, const orc_render_params* p, const i
         Tracer tr{s, cnt, seed, rec};
         tr.box = &box;
         tr.verts = &verts;
+        tr.vert_depth = (int32_t)depth;
         const V3 c = tr.trace(eye, D, (int)p->bounces);
         for (const CacheVertex& v : verts) {  // suffix radiance per unit throughput
           rows[i].insert(rows[i].end(), v.f, v.f + 12);

@@ -124,12 +124,13 @@ int orc_cache_queries(const orc_scene* s, const orc_render_params* p, uint32_t p
                       const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
                       float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads);
 /* Training rows of plain paths (p->bounces) on the listed pixels: one row per
- * surface hit, 12 features and the target (final - gathered before the hit)
- * / throughput on arrival; in (pixel, sample, vertex) order, row_path = index
- * into pixel_ids (may be NULL); *n_rows may exceed cap. */
+ * surface hit (depth 0) or per surface hit of radiance ray number `depth`,
+ * 12 features and the target (final - gathered before the hit) / throughput
+ * on arrival; in (pixel, sample, vertex) order, row_path = index into
+ * pixel_ids (may be NULL); *n_rows may exceed cap. */
 int orc_cache_train_rows(const orc_scene* s, const orc_render_params* p, const int32_t* pixel_ids,
-                         int32_t n, float* feat, float* tgt, int32_t* row_path, int32_t cap,
-                         int32_t* n_rows, int threads);
+                         int32_t n, uint32_t depth, float* feat, float* tgt, int32_t* row_path,
+                         int32_t cap, int32_t* n_rows, int threads);
 
 /* Render pixels (all if pixel_ids == NULL, else n_pixels listed ids).
  * accum_out: float4 per listed pixel (rgb, 1).  accum_prev (same layout), when

@@ -97,7 +97,7 @@ def lib():
                                         C.c_int32, f32p, f32p, f32p, i32p, C.c_int32, i32p, C.c_int]
         L.orc_cache_train_rows.restype = C.c_int
         L.orc_cache_train_rows.argtypes = [C.c_void_p, C.POINTER(RenderParams), i32p, C.c_int32,
-                                           f32p, f32p, i32p, C.c_int32, i32p, C.c_int]
+                                           C.c_uint32, f32p, f32p, i32p, C.c_int32, i32p, C.c_int]
         _lib = L
     return _lib
 
@@ -331,8 +331,9 @@ class Scene:
         return {"accum": acc, "feat": f[:k], "att": at[:k], "item": it[:k], "n": k}
 
     def cache_train_rows(self, width, height, spp, bounces, eye, U, V, W, pixels, subframe=0,
-                         threads=1):
-        """Training rows of plain paths (orc_cache_train_rows)."""
+                         depth=0, threads=1):
+        """Training rows of plain paths (orc_cache_train_rows): every surface hit, or the hits
+        of radiance ray number `depth` only."""
         p = self._params(width, height, spp, bounces, eye, U, V, W, subframe)
         px = np.ascontiguousarray(pixels, dtype=np.int32)
         n = px.shape[0]
@@ -341,7 +342,8 @@ class Scene:
         t = np.zeros((cap, 3), np.float32)
         path = np.zeros(cap, np.int32)
         nr = C.c_int32()
-        rc = lib().orc_cache_train_rows(self.h, C.byref(p), _p(px, C.c_int32), n, _p(f, C.c_float),
+        rc = lib().orc_cache_train_rows(self.h, C.byref(p), _p(px, C.c_int32), n, depth,
+                                        _p(f, C.c_float),
                                         _p(t, C.c_float), _p(path, C.c_int32), cap, C.byref(nr),
                                         threads)
         if rc != 0:

@@ -83,6 +83,14 @@ def test_training_rows_match_oracle(capi, oracle_mod, ctx):
     # a cap keeps the first rows only and reports the full count
     g2 = capi.cache_trace_rows(ctx, p, px, cap=100)
     assert g2["n"] == g["n"] and g2["feat"].shape[0] == 100
+    # rows at one depth (the query depth of lisa_render_cached)
+    g3 = capi.cache_trace_rows(ctx, p, px, depth=3)
+    r3 = o.cache_train_rows(W, H, 2, 7, eye, U, V, Wv, px, subframe=5, depth=3, threads=8)
+    assert g3["n"] == r3["n"] > 0
+    gi = np.lexsort([g3["tgt"][:, 0], g3["feat"][:, 2], g3["feat"][:, 1], g3["feat"][:, 0]])
+    ri = np.lexsort([r3["tgt"][:, 0], r3["feat"][:, 2], r3["feat"][:, 1], r3["feat"][:, 0]])
+    assert np.array_equal(g3["feat"][gi][:, EXACT], r3["feat"][ri][:, EXACT])
+    assert np.array_equal(g3["tgt"][gi], r3["tgt"][ri], equal_nan=True)
 
 
 def test_forced_fallback_is_the_plain_progressive_render(capi, oracle_mod, ctx):
@@ -123,7 +131,7 @@ def test_trained_cache_gains_over_truncated_paths(capi, oracle_mod, ctx):
     """What the trained network gains: the same prefix paths (2 bounces, then
     the cache) with the network's answer replaced by 0 are plain paths cut
     after 3 radiance rays, and the Cornell box loses its indirect light.
-    Trained online for 48 frames, the network recovers most of it: the RMSE
+    Trained online for 96 frames, the network recovers most of it: the RMSE
     against an 8192-spp plain render drops well below the cut paths' and the
     untrained network's."""
     W = H = 64
@@ -141,14 +149,14 @@ def test_trained_cache_gains_over_truncated_paths(capi, oracle_mod, ctx):
     def params_of(f):
         return capi.render_params(W, H, 4, 7, eye, U, V, Wv, subframe=f)
 
-    frames = 48
+    frames = 96
     trained = capi.Cache(0, seed=1, lr=1e-3)
     untrained = capi.Cache(0, seed=1, lr=1e-3)
     zero = capi.Cache(0, seed=1, lr=1e-3)
     try:
         zero.set_weights(np.zeros(capi.CACHE_PARAMS, np.float32))
-        cp = capi.cache_frame_params(prefix_bounces=2, train_paths=4096, train_bounces=7,
-                                     train_steps=2, seed=2)
+        cp = capi.cache_frame_params(prefix_bounces=2, train_paths=32768, train_bounces=7,
+                                     train_steps=4, seed=2)
         off = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_bounces=7,
                                       train_steps=0, seed=2)
         e_tr, st = _cached_rmse(capi, ctx, trained, cp, params_of, frames, ref)
@@ -157,8 +165,8 @@ def test_trained_cache_gains_over_truncated_paths(capi, oracle_mod, ctx):
         e_0, _ = _cached_rmse(capi, ctx, zero, off, params_of, frames, ref)
         print(f"RMSE vs 8192-spp plain: trained {e_tr:.5f}, untrained {e_un:.5f}, "
               f"cut paths (zero cache) {e_0:.5f}", flush=True)
-        assert e_tr < 0.5 * e_0
-        assert e_tr < 0.5 * e_un
+        assert e_tr < 0.6 * e_0
+        assert e_tr < 0.25 * e_un
     finally:
         for c in (trained, untrained, zero):
             c.close()

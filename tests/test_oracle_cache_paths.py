@@ -152,3 +152,22 @@ def test_training_rows_follow_plain_samples(oracle_mod):
     nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
     assert np.abs(np.einsum("ij,ij->i", pos - tri[:, 0], nrm)).max() < 1e-5
+
+
+def test_depth_rows_are_the_kth_vertex(oracle_mod):
+    """Rows at depth d are, path by path, the d-th row of the every-vertex
+    rows (each radiance ray before the end of a path ends at a recorded
+    surface hit)."""
+    sc, o, (U, V, Wv), _ = _setup(oracle_mod)
+    eye = sc["camera"]["eye"]
+    px = _pixels(256, seed=13)
+    allr = o.cache_train_rows(W, H, 1, 7, eye, U, V, Wv, px, subframe=2, threads=4)
+    for d in (1, 3):
+        t = o.cache_train_rows(W, H, 1, 7, eye, U, V, Wv, px, subframe=2, depth=d, threads=4)
+        starts = np.r_[0, np.nonzero(allr["path"][1:] != allr["path"][:-1])[0] + 1]
+        counts = np.diff(np.r_[starts, allr["n"]])
+        pick = starts[counts >= d] + d - 1
+        assert t["n"] == pick.size > 0
+        assert np.array_equal(t["path"], allr["path"][pick])
+        assert np.array_equal(t["feat"], allr["feat"][pick])
+        assert np.array_equal(t["tgt"], allr["tgt"][pick], equal_nan=True)
