@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=2)
     ap.add_argument("--rmse-limit-s", type=float, default=30.0,
                     help="give up a method after this much render time")
+    ap.add_argument("--cache-frames", type=int, default=0,
+                    help="island / cornell_cache: only run this many cached frames (no "
+                         "reference, no plain sweep) and report their timings, for profiling")
     return ap.parse_args()
 
 
@@ -137,7 +140,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profile(a.workload, spp),
-                "kernel": "k_path<false,false>", "alg_bytes_per_launch": alg_bytes,
+                "kernel": "k_path<false,false,false>", "alg_bytes_per_launch": alg_bytes,
                 "avg_kernel_ms": round(avg_kernel_ms, 3)}
 
     cpu = None
@@ -201,6 +204,8 @@ def bench_rmse(a):
     U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
     ctx = capi.Context(0)
     ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    if a.cache_frames > 0:
+        return bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv)
     pix = scenes.seeded_pixels(W, H, REF_PIXELS, seed=7)
     t = time.perf_counter()
     ref = np.zeros((REF_PIXELS, 3))
@@ -269,22 +274,7 @@ def bench_rmse(a):
     e_cache = rmse(ctx.copy_accum(W * H))
     print(f"[bench] cache: {cache_t} s after {frames} frames, rmse {e_cache:.5f}, last {last}",
           file=sys.stderr, flush=True)
-    mlp = None
-    if last is not None and last["queries"] > 0:
-        # algorithmic FLOPs: 2 per real weight per query forward, 3x that per
-        # training row (forward + backward data + weight gradients)
-        f_q = 2 * MLP_WEIGHTS
-        inf_tf = last["queries"] * f_q / (last["infer_kernel_ms"] * 1e-3) / 1e12
-        tr_tf = (last["train_samples"] * 3 * f_q * a.train_steps
-                 / max(last["train_kernel_ms"] * 1e-3, 1e-12) / 1e12)
-        mlp = {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-               "infer_queries": last["queries"], "infer_kernel_ms": round(last["infer_kernel_ms"], 4),
-               "infer_achieved": round(inf_tf, 2),
-               "infer_frac": round(inf_tf / MFMA_BF16_PEAK_TFLOPS, 5),
-               "train_rows": last["train_samples"], "train_steps": a.train_steps,
-               "train_kernel_ms": round(last["train_kernel_ms"], 4),
-               "train_achieved": round(tr_tf, 2),
-               "train_frac": round(tr_tf / MFMA_BF16_PEAK_TFLOPS, 5)}
+    mlp = mlp_roofline(last, a.train_steps)
     out = {"metric": "time to RMSE<=1e-2 vs 16k-spp plain reference (s)",
            "value": None if cache_t is None else round(cache_t, 4), "unit": "s", "n_gpus": 1,
            "steps": frames, "warmup": 0,
@@ -306,6 +296,64 @@ def bench_rmse(a):
            "reference": {"spp": 16384, "pixels": REF_PIXELS, "fp64": True,
                          "seconds": round(ref_s, 2)},
            "last_frame": last, "mlp_roofline": mlp, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+    cache.close()
+    ctx.close()
+
+
+def mlp_roofline(st, train_steps):
+    """MFMA roofline of the cache MLP from one frame's HIP-event kernel times.
+    Algorithmic FLOPs: 2 per real weight per query forward; 3x that per
+    training row (forward, backward data pass, weight gradients)."""
+    if st is None or st["queries"] <= 0 or st["infer_kernel_ms"] <= 0:
+        return None
+    f_q = 2 * MLP_WEIGHTS
+    inf_tf = st["queries"] * f_q / (st["infer_kernel_ms"] * 1e-3) / 1e12
+    out = {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "infer_queries": st["queries"], "infer_kernel_ms": round(st["infer_kernel_ms"], 4),
+           "infer_achieved": round(inf_tf, 2),
+           "infer_frac": round(inf_tf / MFMA_BF16_PEAK_TFLOPS, 5)}
+    if st["train_kernel_ms"] > 0:
+        tr_tf = (st["train_samples"] * 3 * f_q * train_steps / (st["train_kernel_ms"] * 1e-3)
+                 / 1e12)
+        out.update({"train_rows": st["train_samples"], "train_steps": train_steps,
+                    "train_kernel_ms": round(st["train_kernel_ms"], 4),
+                    "train_achieved": round(tr_tf, 2),
+                    "train_frac": round(tr_tf / MFMA_BF16_PEAK_TFLOPS, 5)})
+    return out
+
+
+def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
+    """`--cache-frames N`: N cached frames after `--warmup` untimed ones; per-frame
+    wall time and the MLP's MFMA roofline (kernel times from HIP events)."""
+    cam = sc["camera"]
+    cache = capi.Cache(0, seed=1, lr=1e-3)
+    cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
+                                 train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
+                                 seed=3)
+    stats = []
+    for f in range(a.warmup + a.cache_frames):
+        p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
+        t = time.perf_counter()
+        st = capi.render_cached(ctx, cache, p, cp)
+        st["wall_ms"] = (time.perf_counter() - t) * 1e3
+        if f >= a.warmup:
+            stats.append(st)
+        print(f"[bench] frame {f}: {st['wall_ms']:.2f} ms, {st['queries']} queries, "
+              f"{st['train_samples']} rows", file=sys.stderr, flush=True)
+    med = sorted(stats, key=lambda x: x["infer_kernel_ms"])[len(stats) // 2]
+    ms = sum(x["wall_ms"] for x in stats) / len(stats)
+    out = {"metric": "cached frame time (ms)", "value": round(ms, 3), "unit": "ms",
+           "n_gpus": 1, "steps": a.cache_frames, "warmup": a.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": f"{sc['name']} learned cache frames", "width": W, "height": H,
+                      "bounces": B, "cache_prefix_bounces": a.prefix,
+                      "cache_spp_per_frame": a.cache_spp, "train_paths": a.train_paths,
+                      "train_steps": a.train_steps, "parallelism": "single GPU"},
+           "frame": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in med.items()},
+           "mlp_roofline": mlp_roofline(med, a.train_steps), "roofline": None,
+           "cpu_baseline": None}
     print(json.dumps(out), flush=True)
     cache.close()
     ctx.close()

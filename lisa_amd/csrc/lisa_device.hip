@@ -357,9 +357,10 @@ struct Job {
   int32_t item;
 };
 
-// The per-lane path loop over the pixels a source hands out.  `src.sc` and
-// `src.p` describe the scene and launch of the current pixel; `src.next(j)`
-// gives the lane its next pixel (false: no work left).
+// The per-lane path loop over the pixels a source hands out.  `src.view()`
+// and `src.args()` point at the scene and launch of the current pixel (read
+// again after every `src.next(j)`, which gives the lane its next pixel;
+// false: no work left).
 // kMode 0: the plain path loop.  kMode 1: the radiance ray numbered
 // p.cache_at ends at a surface hit by queuing a cache query (features, the
 // path throughput, the pixel); misses and emitters are handled exactly.
@@ -373,12 +374,12 @@ template <bool kRecord, bool kCount, class Src, int kMode = 0, bool kF64 = false
 __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   Job job;
   if (!src.next(job)) return;
-  const SceneView& sc = src.sc;
-  const PathArgs& p = src.p;
+  const SceneView* scp = src.view();  // the current pixel's scene and launch
+  const PathArgs* pp = src.args();
   uint32_t x = job.x, y = job.y;
   int32_t item = job.item;
 
-  uint32_t seed = tea16(y * p.width + x, p.subframe);  // shader.cu:141
+  uint32_t seed = tea16(y * pp->width + x, pp->subframe);  // shader.cu:141
 
   uint64_t c_rad = 0, c_occ = 0, c_rhit = 0, c_inv = 0, c_smp = 0, c_px = 0;
   TravCount tc;
@@ -399,20 +400,20 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   uint32_t b = 0, s = 0;
   uint32_t hash = 0;
   int32_t occl_cnt = 0;
-  size_t rec_base = (size_t)item * p.spp;
+  size_t rec_base = (size_t)item * pp->spp;
 
   uint32_t t_used = 0;  // kMode 2: vertex slots recorded in this sample
   // accum_color += trace(...) (shader.cu:154) and the per-sample records.
   auto finish_sample = [&]() {
     if (kMode == 2 && t_used > 0) {
       // one reservation per path; suffix radiance per unit throughput
-      const uint32_t row0 = atomicAdd(p.t_count, t_used);
-      const size_t base = (size_t)item * p.t_slots;
-      for (uint32_t v = 0; v < t_used && row0 + v < p.t_cap; ++v) {
-        const float* r = p.t_rec + 18 * (base + v);
-        float* f = p.t_feat + 12 * (size_t)(row0 + v);
+      const uint32_t row0 = atomicAdd(pp->t_count, t_used);
+      const size_t base = (size_t)item * pp->t_slots;
+      for (uint32_t v = 0; v < t_used && row0 + v < pp->t_cap; ++v) {
+        const float* r = pp->t_rec + 18 * (base + v);
+        float* f = pp->t_feat + 12 * (size_t)(row0 + v);
         for (int q = 0; q < 12; ++q) f[q] = r[q];
-        float* tg = p.t_tgt + 3 * (size_t)(row0 + v);
+        float* tg = pp->t_tgt + 3 * (size_t)(row0 + v);
         tg[0] = (color.x - r[15]) / r[12];
         tg[1] = (color.y - r[16]) / r[13];
         tg[2] = (color.z - r[17]) / r[14];
@@ -427,27 +428,27 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       ad2 += (double)color.z;
     }
     if (kRecord) {
-      if (p.s_rgb) {
-        p.s_rgb[3 * (rec_base + s)] = color.x;
-        p.s_rgb[3 * (rec_base + s) + 1] = color.y;
-        p.s_rgb[3 * (rec_base + s) + 2] = color.z;
+      if (pp->s_rgb) {
+        pp->s_rgb[3 * (rec_base + s)] = color.x;
+        pp->s_rgb[3 * (rec_base + s) + 1] = color.y;
+        pp->s_rgb[3 * (rec_base + s) + 2] = color.z;
       }
-      if (p.s_b) p.s_b[rec_base + s] = (int32_t)b;
-      if (p.s_o) p.s_o[rec_base + s] = occl_cnt;
-      if (p.s_h) p.s_h[rec_base + s] = hash;
+      if (pp->s_b) pp->s_b[rec_base + s] = (int32_t)b;
+      if (pp->s_o) pp->s_o[rec_base + s] = occl_cnt;
+      if (pp->s_h) pp->s_h[rec_base + s] = hash;
     }
     ++s;
   };
   // The camera ray of the next sample (shader.cu:147-152); samples with no
   // bounce finish at once.  False when the pixel has no sample left.
   auto begin_sample = [&]() -> bool {
-    while (s < p.spp) {
+    while (s < pp->spp) {
       const float jx = rng(seed);
       const float jy = rng(seed);
-      const float dx = (2.0f * (float)x + jx) / (float)p.width - 1.0f;
-      const float dy = (2.0f * (float)y + jy) / (float)p.height - 1.0f;
-      D = normalize(dx * p.U + dy * p.V + p.W);
-      O = p.eye;
+      const float dx = (2.0f * (float)x + jx) / (float)pp->width - 1.0f;
+      const float dy = (2.0f * (float)y + jy) / (float)pp->height - 1.0f;
+      D = normalize(dx * pp->U + dy * pp->V + pp->W);
+      O = pp->eye;
       color = mk(0.0f, 0.0f, 0.0f);
       att = mk(1.0f, 1.0f, 1.0f);
       hitf = false;
@@ -456,10 +457,10 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       if (kRecord) {
         hash = 2166136261u;
         occl_cnt = 0;
-        if (p.rad_hits)
-          for (int32_t q = 0; q < p.rb; ++q) p.rad_hits[(rec_base + s) * p.rb + q] = -3;
+        if (pp->rad_hits)
+          for (int32_t q = 0; q < pp->rb; ++q) pp->rad_hits[(rec_base + s) * pp->rb + q] = -3;
       }
-      if (p.bounces > 0) return true;
+      if (pp->bounces > 0) return true;
       finish_sample();
     }
     return false;
@@ -470,7 +471,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     color = color + nee * att;
     D = Lambertian::sample(Din, N, seed, rough);
     qstage = 0;
-    if (b == p.bounces) {
+    if (b == pp->bounces) {
       finish_sample();
       return begin_sample();
     }
@@ -480,7 +481,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // or give up after kNeeTries (shader.cu:200-208).
   auto nee_after = [&](int32_t code) -> bool {
     if (kRecord) hash = (hash ^ ((uint32_t)code ^ 0x5bd1e995u)) * 16777619u;
-    if (hitf) return end_nee(xyz(sc.mats[light_mat].emission_emit) * Lambertian::eval(N, D));
+    if (hitf) return end_nee(xyz(scp->mats[light_mat].emission_emit) * Lambertian::eval(N, D));
     if (++tries == kNeeTries) return end_nee(mk(0.0f, 0.0f, 0.0f));
     D = hemisphere(N, seed);
     qstage = 1;
@@ -491,17 +492,17 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   auto write_pixel = [&]() {
     F3 out;
     if (kF64) {
-      const double inv = 1.0 / (double)p.spp;
+      const double inv = 1.0 / (double)pp->spp;
       out = mk((float)(ad0 * inv), (float)(ad1 * inv), (float)(ad2 * inv));
     } else {
-      const float inv = 1.0f / (float)p.spp;
+      const float inv = 1.0f / (float)pp->spp;
       out = accum * inv;
     }
-    if (kMode == 0 && p.subframe > 0 && p.accum_prev) {
-      const float a = 1.0f / (float)(p.subframe + 1);
-      out = lerp(xyz(p.accum_prev[item]), out, a);
+    if (kMode == 0 && pp->subframe > 0 && pp->accum_prev) {
+      const float a = 1.0f / (float)(pp->subframe + 1);
+      out = lerp(xyz(pp->accum_prev[item]), out, a);
     }
-    p.accum[item] = make_float4(out.x, out.y, out.z, 1.0f);
+    pp->accum[item] = make_float4(out.x, out.y, out.z, 1.0f);
     if (kCount) c_px++;
   };
 
@@ -510,14 +511,16 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     if (!alive) {
       write_pixel();
       if (!src.next(job)) break;
+      scp = src.view();
+      pp = src.args();
       x = job.x;
       y = job.y;
       item = job.item;
-      seed = tea16(y * p.width + x, p.subframe);
+      seed = tea16(y * pp->width + x, pp->subframe);
       accum = mk(0.0f, 0.0f, 0.0f);
       ad0 = ad1 = ad2 = 0.0;
       s = 0;
-      rec_base = (size_t)item * p.spp;
+      rec_base = (size_t)item * pp->spp;
       alive = begin_sample();
       continue;
     }
@@ -535,16 +538,16 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       if (kRecord) occl_cnt++;
       int32_t k = -1;
       float t = 0.0f;
-      if (valid && sc.emit.T > 0) {
+      if (valid && scp->emit.T > 0) {
         const RayT r = make_ray(O, D);
-        k = closest<kCount, kPathBlock>(sc.emit, r, kTmin, kTmax, 0xFFFFFFFFu, false, t, my_stk, tc);
+        k = closest<kCount, kPathBlock>(scp->emit, r, kTmin, kTmax, 0xFFFFFFFFu, false, t, my_stk, tc);
       }
       if (!valid) {  // counts as a ray, misses without traversal
         const int32_t code = hitf ? -1 : -5;
         hitf = false;
         alive = nee_after(code);
       } else if (k >= 0) {  // an emitter: is anything in front of it?
-        const float4 e0 = sc.emit.tris[3 * (size_t)k], e1 = sc.emit.tris[3 * (size_t)k + 1];
+        const float4 e0 = scp->emit.tris[3 * (size_t)k], e1 = scp->emit.tris[3 * (size_t)k + 1];
         tE = t;
         primE = __float_as_uint(e0.w);
         matE = (int32_t)__float_as_uint(e1.w);
@@ -566,10 +569,10 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     const bool valid = ray_valid(D);
     int32_t k = -1;
     float t = 0.0f;
-    if (valid && sc.all.T > 0) {
+    if (valid && scp->all.T > 0) {
       const RayT r = make_ray(O, D);
       const bool any = qstage == 2;
-      k = closest<kCount, kPathBlock>(sc.all, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
+      k = closest<kCount, kPathBlock>(scp->all, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
                                       any, t, my_stk, tc);
     }
 
@@ -582,14 +585,14 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       float4 t0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), t1 = t0;
       int32_t prim = valid ? -1 : -2, mat = -1;
       if (k >= 0) {
-        t0 = sc.all.tris[3 * (size_t)k];
-        t1 = sc.all.tris[3 * (size_t)k + 1];
+        t0 = scp->all.tris[3 * (size_t)k];
+        t1 = scp->all.tris[3 * (size_t)k + 1];
         prim = (int32_t)__float_as_uint(t0.w);
         mat = (int32_t)__float_as_uint(t1.w);
       }
       if (kRecord) {
         hash = (hash ^ (uint32_t)prim) * 16777619u;
-        if (p.rad_hits && (int32_t)b < p.rb) p.rad_hits[(rec_base + s) * p.rb + b] = prim;
+        if (pp->rad_hits && (int32_t)b < pp->rb) pp->rad_hits[(rec_base + s) * pp->rb + b] = prim;
       }
       ++b;
       bool finish = false;
@@ -597,7 +600,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
         color = color + mk(0.0f, 0.0f, 0.0f);
         finish = true;
       } else {
-        const DevMat m = sc.mats[mat];
+        const DevMat m = scp->mats[mat];
         if (m.emission_emit.w != 0.0f) {
           color = color + xyz(m.emission_emit) * att;
           finish = true;
@@ -606,26 +609,26 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           O = O + t * D;
           Din = D;
           {
-            const float4 t2 = sc.all.tris[3 * (size_t)k + 2];
-            const float4* nq = sc.nrm + 3 * (size_t)k;
+            const float4 t2 = scp->all.tris[3 * (size_t)k + 2];
+            const float4* nq = scp->nrm + 3 * (size_t)k;
             N = bary_normal(O, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
           }
           bool cached = false;
-          if (kMode == 1 && b == p.cache_at) {
-            const uint32_t slot = atomicAdd(p.q_count, 1u);
-            if (slot < p.q_cap) {
-              cache_features(p, O, Din, N, m, p.q_feat + 12 * (size_t)slot);
-              p.q_att[3 * (size_t)slot] = att.x;
-              p.q_att[3 * (size_t)slot + 1] = att.y;
-              p.q_att[3 * (size_t)slot + 2] = att.z;
-              p.q_item[slot] = item;
+          if (kMode == 1 && b == pp->cache_at) {
+            const uint32_t slot = atomicAdd(pp->q_count, 1u);
+            if (slot < pp->q_cap) {
+              cache_features(*pp, O, Din, N, m, pp->q_feat + 12 * (size_t)slot);
+              pp->q_att[3 * (size_t)slot] = att.x;
+              pp->q_att[3 * (size_t)slot + 1] = att.y;
+              pp->q_att[3 * (size_t)slot + 2] = att.z;
+              pp->q_item[slot] = item;
             }
             cached = true;
             finish = true;
           }
-          if (kMode == 2 && t_used < p.t_slots && (p.t_depth == 0 || b == p.t_depth)) {
-            float* r = p.t_rec + 18 * ((size_t)item * p.t_slots + t_used);
-            cache_features(p, O, Din, N, m, r);
+          if (kMode == 2 && t_used < pp->t_slots && (pp->t_depth == 0 || b == pp->t_depth)) {
+            float* r = pp->t_rec + 18 * ((size_t)item * pp->t_slots + t_used);
+            cache_features(*pp, O, Din, N, m, r);
             r[12] = att.x;
             r[13] = att.y;
             r[14] = att.z;
@@ -637,7 +640,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           if (cached) {
           } else if (m.diffuse_alpha.w < 1.0f) {
             D = Dielectric::scatter(D, N, m.rough_n.y, xyz(m.diffuse_alpha), att, seed);
-            if (b == p.bounces) finish = true;
+            if (b == pp->bounces) finish = true;
           } else {
             att = att * xyz(m.diffuse_alpha);
             rough = m.rough_n.x;
@@ -674,14 +677,14 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   }
 
   if (kCount) {
-    atomicAdd(&p.counters[0], (unsigned long long)c_rad);
-    atomicAdd(&p.counters[1], (unsigned long long)c_occ);
-    atomicAdd(&p.counters[2], (unsigned long long)tc.visits);
-    atomicAdd(&p.counters[3], (unsigned long long)tc.tests);
-    atomicAdd(&p.counters[4], (unsigned long long)c_rhit);
-    atomicAdd(&p.counters[5], (unsigned long long)c_smp);
-    atomicAdd(&p.counters[6], (unsigned long long)c_inv);
-    atomicAdd(&p.counters[7], (unsigned long long)c_px);
+    atomicAdd(&pp->counters[0], (unsigned long long)c_rad);
+    atomicAdd(&pp->counters[1], (unsigned long long)c_occ);
+    atomicAdd(&pp->counters[2], (unsigned long long)tc.visits);
+    atomicAdd(&pp->counters[3], (unsigned long long)tc.tests);
+    atomicAdd(&pp->counters[4], (unsigned long long)c_rhit);
+    atomicAdd(&pp->counters[5], (unsigned long long)c_smp);
+    atomicAdd(&pp->counters[6], (unsigned long long)c_inv);
+    atomicAdd(&pp->counters[7], (unsigned long long)c_px);
   }
 }
 
@@ -691,6 +694,8 @@ struct OnePixel {
   SceneView sc;
   PathArgs p;
   bool given;
+  __device__ const SceneView* view() const { return &sc; }
+  __device__ const PathArgs* args() const { return &p; }
   __device__ bool next(Job& j) {
     if (given) return false;
     given = true;
@@ -775,9 +780,9 @@ struct QueueSrc {
   uint32_t total;
   uint32_t chunk;
   uint32_t cur, end;
-  int32_t scene;
-  SceneView sc;
-  PathArgs p;
+  const BatchScene* bs;  // the scene of the current pixel (device table entry)
+  __device__ const SceneView* view() const { return &bs->sc; }
+  __device__ const PathArgs* args() const { return &bs->p; }
   __device__ bool next(Job& j) {
     for (;;) {
       if (cur >= end) {
@@ -795,18 +800,15 @@ struct QueueSrc {
         else
           hi = mid - 1;
       }
-      if (lo != scene) {
-        sc = scenes[lo].sc;
-        p = scenes[lo].p;
-        scene = lo;
-      }
-      const uint32_t local = idx - scenes[lo].first;
+      bs = scenes + lo;
+      const uint32_t local = idx - bs->first;
       const uint32_t tile = local >> 6, w = local & 63u;
-      const uint32_t x = (tile % p.tiles_x) * 8 + (w & 7u), y = (tile / p.tiles_x) * 8 + (w >> 3);
-      if (x >= p.width || y >= p.height) continue;  // outside a partial edge tile
+      const uint32_t tx = bs->p.tiles_x, width = bs->p.width;
+      const uint32_t x = (tile % tx) * 8 + (w & 7u), y = (tile / tx) * 8 + (w >> 3);
+      if (x >= width || y >= bs->p.height) continue;  // outside a partial edge tile
       j.x = x;
       j.y = y;
-      j.item = (int32_t)(y * p.width + x);
+      j.item = (int32_t)(y * width + x);
       return true;
     }
   }
@@ -823,7 +825,7 @@ __global__ __launch_bounds__(kPathBlock) void k_path_queue(const BatchScene* sce
   src.total = total;
   src.chunk = chunk;
   src.cur = src.end = 0;
-  src.scene = -1;
+  src.bs = scenes;
   path_lane<false, false>(src, stk + threadIdx.x);
 }
 
