@@ -63,6 +63,10 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=2)
     ap.add_argument("--rmse-limit-s", type=float, default=30.0,
                     help="give up a method after this much render time")
+    ap.add_argument("--persistent", action="store_true",
+                    help="time the persistent launch (LISA_FLAG_PERSISTENT) of the plain path")
+    ap.add_argument("--cache-persistent", action="store_true",
+                    help="cached frames' paths in one persistent launch (LISA_CACHE_PERSISTENT)")
     ap.add_argument("--cache-frames", type=int, default=0,
                     help="island / cornell_cache: only run this many cached frames (no "
                          "reference, no plain sweep) and report their timings, for profiling")
@@ -118,7 +122,7 @@ def main():
     rays = counts["rays_radiance"] + counts["rays_occlusion"]
     print(f"[bench] counted step: {counts}", file=sys.stderr, flush=True)
 
-    p.flags = 0
+    p.flags = capi.FLAG_PERSISTENT if a.persistent else 0
     for _ in range(a.warmup):
         ctx.render(p, want_accum=False)
     ctx.synchronize()
@@ -140,7 +144,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profile(a.workload, spp),
-                "kernel": "k_path<false,false,false>", "alg_bytes_per_launch": alg_bytes,
+                "kernel": "k_path_pq<0>" if a.persistent else "k_path<false,false,false>",
+                "alg_bytes_per_launch": alg_bytes,
                 "avg_kernel_ms": round(avg_kernel_ms, 3)}
 
     cpu = None
@@ -163,7 +168,9 @@ def main():
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 2),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
-           "config": {"workload": f"{sc['name']} plain megakernel", "width": W, "height": H,
+           "config": {"workload": f"{sc['name']} plain megakernel"
+                                  + (" (persistent launch)" if a.persistent else ""),
+                      "width": W, "height": H,
                       "spp": spp, "bounces": B, "triangles": ctx.T, "parallelism": "single GPU"},
            "samples_per_s": round(W * H * spp * a.steps / wall, 1),
            "rays_per_step": rays, "bvh_build_ms": round(bvh_ms, 3),
@@ -256,7 +263,7 @@ def bench_rmse(a):
     cache = capi.Cache(0, seed=1, lr=1e-3)
     cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                  train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
-                                 seed=3)
+                                 seed=3, flags=capi.CACHE_PERSISTENT if a.cache_persistent else 0)
     cache_t, tot, frames, last = None, 0.0, 0, None
     for f in range(1 << 20):
         p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
@@ -285,7 +292,7 @@ def bench_rmse(a):
                       "bounces": B, "triangles": ctx.T, "cache_prefix_bounces": a.prefix,
                       "cache_spp_per_frame": a.cache_spp, "train_paths": a.train_paths,
                       "train_rows_max": 1 << 16, "train_steps": a.train_steps, "lr": 1e-3,
-                      "parallelism": "single GPU"},
+                      "persistent_frames": a.cache_persistent, "parallelism": "single GPU"},
            "plain_time_s": None if plain_t is None else round(plain_t, 4),
            "plain_progressive_time_s": None if prog_t is None else round(prog_t, 4),
            "speedup_vs_plain": (None if (plain_t is None or cache_t is None)
@@ -330,7 +337,7 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
     cache = capi.Cache(0, seed=1, lr=1e-3)
     cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                  train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
-                                 seed=3)
+                                 seed=3, flags=capi.CACHE_PERSISTENT if a.cache_persistent else 0)
     stats = []
     for f in range(a.warmup + a.cache_frames):
         p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)

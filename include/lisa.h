@@ -60,6 +60,8 @@ typedef struct lisa_render_params {
 #define LISA_MODE_PLAIN 0        /* one work-item per pixel megakernel (the drop-in) */
 #define LISA_FLAG_FP64_ACCUM 1u  /* per-pixel FP64 accumulation (reference renders >= 16k spp) */
 #define LISA_FLAG_COUNT 2u       /* also count rays / node visits / triangle tests */
+#define LISA_FLAG_PERSISTENT 4u  /* one persistent launch whose lanes pull pixels from a queue
+                                    (same image; not with COUNT or FP64_ACCUM) */
 
 /* Counters (lisa_get_counters), in order. */
 #define LISA_NCOUNTERS 8
@@ -204,8 +206,10 @@ typedef struct lisa_cache_frame_params {
   uint32_t train_max;      /* at most this many training rows (suffixes) per frame; 0: no cap */
   uint32_t train_steps;    /* Adam steps per frame over the frame's rows */
   float fallback_loss;     /* > 0: plain frame when the last training loss exceeds it */
+  uint32_t flags;          /* LISA_CACHE_PERSISTENT: the frame's paths in a persistent launch */
   uint64_t seed;           /* draws the training pixels */
 } lisa_cache_frame_params;
+#define LISA_CACHE_PERSISTENT 1u
 
 typedef struct lisa_cache_frame_stats {
   float trace_train_ms, train_ms, render_ms, infer_ms, total_ms;  /* host wall time */

@@ -28,6 +28,7 @@ ERR_UNSUPPORTED = -7
 MODE_PLAIN = 0
 FLAG_FP64_ACCUM = 1
 FLAG_COUNT = 2
+FLAG_PERSISTENT = 4
 NCOUNTERS = 8
 COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
                  "radiance_hits", "samples", "invalid_dir_rays", "pixels")
@@ -424,10 +425,14 @@ CACHE_PARAMS = 23552
 CACHE_INPUTS = 12
 
 
+CACHE_PERSISTENT = 1
+
+
 class CacheFrameParams(C.Structure):
     _fields_ = [("prefix_bounces", C.c_uint32), ("train_paths", C.c_uint32),
                 ("train_bounces", C.c_uint32), ("train_max", C.c_uint32),
-                ("train_steps", C.c_uint32), ("fallback_loss", C.c_float), ("seed", C.c_uint64)]
+                ("train_steps", C.c_uint32), ("fallback_loss", C.c_float),
+                ("flags", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class CacheFrameStats(C.Structure):
@@ -438,9 +443,9 @@ class CacheFrameStats(C.Structure):
 
 
 def cache_frame_params(prefix_bounces=2, train_paths=8192, train_bounces=12, train_max=1 << 16,
-                       train_steps=1, fallback_loss=0.0, seed=1):
+                       train_steps=1, fallback_loss=0.0, seed=1, flags=0):
     return CacheFrameParams(prefix_bounces, train_paths, train_bounces, train_max, train_steps,
-                            fallback_loss, seed)
+                            fallback_loss, flags, seed)
 
 
 def render_cached(ctx, cache, params, cparams):

@@ -740,20 +740,25 @@ lisa_status lisa_cache_gradients(lisa_cache* c, const float* feat, const float* 
 
 }  // extern "C"
 
-// One Adam step on n device-resident samples (features n x 12, targets n x 3).
+// `steps` Adam steps on the same n device-resident samples (features with
+// row stride feat_stride, targets n x 3), enqueued back to back; one wait at
+// the end; *loss = the last step's loss, last_ms = all steps (HIP events).
 lisa_status lisa::cache_train_device(lisa_cache* c, const float* d_feat, int feat_stride,
-                                     const float* d_tgt, int64_t n, float* loss) {
+                                     const float* d_tgt, int64_t n, float* loss, int steps) {
   if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
-  if (n <= 0) return cfail(c, LISA_ERR_INVALID_ARG, "n must be > 0");
+  if (n <= 0 || steps <= 0) return cfail(c, LISA_ERR_INVALID_ARG, "n and steps must be > 0");
   CHIP(c, hipSetDevice(c->device));
   CHIP(c, hipEventRecord(c->ev0, c->stream));
-  lisa_status rc = grads_device(c, d_feat, feat_stride, d_tgt, n);
-  if (rc != LISA_OK) return rc;
-  ++c->step;
-  const float c1 = 1.0f - std::pow(c->b1, (float)c->step), c2 = 1.0f - std::pow(c->b2, (float)c->step);
-  k_mlp_adam<<<(kParams + 255) / 256, 256, 0, c->stream>>>(c->w, c->m, c->v, c->g, c->lr, c->b1,
-                                                          c->b2, c->eps, c1, c2, c->wt, c->wb);
-  CHIP(c, hipGetLastError());
+  for (int k = 0; k < steps; ++k) {
+    lisa_status rc = grads_device(c, d_feat, feat_stride, d_tgt, n);
+    if (rc != LISA_OK) return rc;
+    ++c->step;
+    const float c1 = 1.0f - std::pow(c->b1, (float)c->step),
+                c2 = 1.0f - std::pow(c->b2, (float)c->step);
+    k_mlp_adam<<<(kParams + 255) / 256, 256, 0, c->stream>>>(c->w, c->m, c->v, c->g, c->lr, c->b1,
+                                                            c->b2, c->eps, c1, c2, c->wt, c->wb);
+    CHIP(c, hipGetLastError());
+  }
   CHIP(c, hipEventRecord(c->ev1, c->stream));
   if (loss) {
     CHIP(c, hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));

@@ -732,6 +732,63 @@ __global__ __launch_bounds__(kPathBlock) void k_path_mode(SceneView sc, PathArgs
   path_lane<false, false, OnePixel, kMode>(src, stk + threadIdx.x);
 }
 
+// One scene's pixels from a device counter, for persistent launches: the
+// listed pixels (p.pixels) or the full image in 8x8-tile order.  A lane takes
+// `chunk` work indices when it runs out; the lanes of a wave that ask
+// together share one atomic.  Lanes whose paths end early pull more pixels
+// instead of idling behind the wave's longest path.
+struct PixelQueue {
+  SceneView sc;
+  PathArgs p;
+  uint32_t* counter;
+  uint32_t total, chunk, cur, end;
+  __device__ const SceneView* view() const { return &sc; }
+  __device__ const PathArgs* args() const { return &p; }
+  __device__ bool next(Job& j) {
+    for (;;) {
+      if (cur >= end) {
+        const uint64_t req = __ballot(1);
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)req) - 1u;
+        const uint32_t rank = (uint32_t)__popcll(req & ((1ull << lane) - 1ull));
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(req) * chunk);
+        base = __shfl(base, (int)leader);
+        const uint32_t mine = base + rank * chunk;
+        if (mine >= total) return false;
+        cur = mine;
+        end = min(mine + chunk, total);
+      }
+      const uint32_t idx = cur++;
+      if (p.pixels) {
+        const uint32_t pix = (uint32_t)p.pixels[idx];
+        j.x = pix % p.width;
+        j.y = pix / p.width;
+        j.item = (int32_t)idx;
+        return true;
+      }
+      const uint32_t tile = idx >> 6, w = idx & 63u;
+      const uint32_t x = (tile % p.tiles_x) * 8 + (w & 7u), y = (tile / p.tiles_x) * 8 + (w >> 3);
+      if (x >= p.width || y >= p.height) continue;  // outside a partial edge tile
+      j.x = x;
+      j.y = y;
+      j.item = (int32_t)(y * p.width + x);
+      return true;
+    }
+  }
+};
+
+// Persistent form of the path loop (kMode as in k_path_mode, 0 = plain):
+// `total` work indices (pixels of the list, or 64 per 8x8 tile) handed out
+// by `counter`, which must be zero at launch.
+template <int kMode>
+__global__ __launch_bounds__(kPathBlock) void k_path_pq(SceneView sc, PathArgs p, uint32_t* counter,
+                                                        uint32_t total, uint32_t chunk) {
+  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  PixelQueue src{sc, p, counter, total, chunk, 0u, 0u};
+  path_lane<false, false, PixelQueue, kMode>(src, stk + threadIdx.x);
+}
+
 // frame[item] += throughput * max(network output, 0) / spp for every query.
 __global__ __launch_bounds__(256) void k_cache_add(const float* __restrict__ pred,
                                                    const float* __restrict__ att,
@@ -1253,6 +1310,7 @@ struct DevBuf {
 
 struct lisa_ctx {
   int32_t device = 0;
+  int32_t cus = 256;  // compute units (persistent grids)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -1353,6 +1411,8 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 
 lisa_status check_params(lisa_ctx* ctx, const lisa_render_params* p) {
   if (!p) return fail(ctx, LISA_ERR_INVALID_ARG, "render params is NULL");
+  if ((p->flags & LISA_FLAG_PERSISTENT) && (p->flags & (LISA_FLAG_COUNT | LISA_FLAG_FP64_ACCUM)))
+    return fail(ctx, LISA_ERR_UNSUPPORTED, "the persistent launch has no counting or FP64 form");
   if (p->width == 0 || p->height == 0) return fail(ctx, LISA_ERR_INVALID_ARG, "width/height must be > 0");
   if ((uint64_t)p->width * p->height > (1ull << 31))
     return fail(ctx, LISA_ERR_INVALID_ARG, "image larger than 2^31 pixels");
@@ -1466,6 +1526,29 @@ lisa_status refit_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
 
 }  // namespace
 
+// Persistent launch of the path loop in mode kMode over `a`'s pixel list or
+// full image: 20 workgroups of 64 per CU (the occupancy of the path kernels),
+// work handed out by counter slot `slot` of the context's scratch.
+constexpr int kPersistentBlocksPerCu = 20;
+template <int kMode>
+lisa_status launch_persistent(lisa_ctx* ctx, PathArgs a, int slot) {
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, ctx->c_qcount.ensure(64));
+  uint32_t* counter = ctx->c_qcount.p + slot;
+  HIP_TRY(ctx, hipMemsetAsync(counter, 0, 4, st));
+  uint32_t total;
+  if (a.pixels) {
+    total = (uint32_t)a.n_items;
+  } else {
+    a.tiles_x = (a.width + 7) / 8;
+    total = a.tiles_x * ((a.height + 7) / 8) * 64u;
+  }
+  k_path_pq<kMode><<<(unsigned)(ctx->cus * kPersistentBlocksPerCu), kPathBlock, 0, st>>>(
+      view(ctx), a, counter, total, 1u);
+  HIP_TRY(ctx, hipGetLastError());
+  return LISA_OK;
+}
+
 // Scene box of the cache features: union of the root's two child boxes.
 lisa_status read_scene_box(lisa_ctx* ctx) {
   const int64_t T = ctx->all.T;
@@ -1528,7 +1611,7 @@ lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, si
 // n listed device pixels; radiance gathered before the cache into `frame`
 // (per pixel mean), queries into ctx->c_q* (capacity n * spp); *nq = queries.
 lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, uint32_t prefix,
-                          float4* frame, uint32_t* nq) {
+                          float4* frame, uint32_t* nq, bool persistent = false) {
   hipStream_t st = ctx->stream;
   const size_t qcap = n * a.spp;
   HIP_TRY(ctx, ctx->c_qfeat.ensure(qcap * 12));
@@ -1548,15 +1631,20 @@ lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_
   a.q_item = ctx->c_qitem.p;
   a.q_count = ctx->c_qcount.p;
   a.q_cap = (uint32_t)qcap;
-  unsigned blocks;
-  if (d_pix) {
-    blocks = grid_for((int64_t)n, kPathBlock);
+  if (persistent) {
+    lisa_status rc = launch_persistent<1>(ctx, a, 2);
+    if (rc != LISA_OK) return rc;
   } else {
-    a.tiles_x = (a.width + kTileW - 1) / kTileW;
-    blocks = a.tiles_x * ((a.height + kTileH - 1) / kTileH);
+    unsigned blocks;
+    if (d_pix) {
+      blocks = grid_for((int64_t)n, kPathBlock);
+    } else {
+      a.tiles_x = (a.width + kTileW - 1) / kTileW;
+      blocks = a.tiles_x * ((a.height + kTileH - 1) / kTileH);
+    }
+    k_path_mode<1><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
+    HIP_TRY(ctx, hipGetLastError());
   }
-  k_path_mode<1><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
-  HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipMemcpyAsync(nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   return LISA_OK;
@@ -1604,7 +1692,9 @@ lisa_status lisa_create(lisa_ctx** out, int32_t device) {
   if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&c->stream)) != hipSuccess ||
       (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->d_counters),
-                     LISA_NCOUNTERS * sizeof(unsigned long long))) != hipSuccess) {
+                     LISA_NCOUNTERS * sizeof(unsigned long long))) != hipSuccess ||
+      (e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device)) !=
+          hipSuccess) {
     const std::string msg = std::string("lisa_create: ") + hipGetErrorString(e);
     lisa_destroy(c);
     return fail(nullptr, LISA_ERR_HIP, msg);
@@ -1734,7 +1824,10 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters, 0, LISA_NCOUNTERS * sizeof(unsigned long long), st));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
   const unsigned grid = a.tiles_x * tiles_y;
-  if (count && a.fp64)
+  if (p->flags & LISA_FLAG_PERSISTENT) {
+    lisa_status rc2 = launch_persistent<0>(ctx, a, 4);
+    if (rc2 != LISA_OK) return rc2;
+  } else if (count && a.fp64)
     k_path<false, true, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
   else if (count)
     k_path<false, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
@@ -2147,12 +2240,11 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     if (rc != LISA_OK) return rc;
     rows = std::min<uint32_t>(rows, (uint32_t)cap);
     const auto t1 = std::chrono::steady_clock::now();
-    for (uint32_t k = 0; k < cp->train_steps && rows > 0; ++k) {
-      float l = 0.0f;
-      lisa_status r2 = lisa::cache_train_device(cache, ctx->c_tfeat.p, 12, ctx->c_ttgt.p, (int64_t)rows, &l);
+    if (rows > 0) {
+      lisa_status r2 = lisa::cache_train_device(cache, ctx->c_tfeat.p, 12, ctx->c_ttgt.p, (int64_t)rows,
+                                                &loss, (int)cp->train_steps);
       if (r2 != LISA_OK) return fail(ctx, r2, std::string("cache training: ") + lisa_cache_last_error(cache));
-      out_stats.train_kernel_ms += lisa_cache_last_ms(cache);
-      loss = l;
+      out_stats.train_kernel_ms = lisa_cache_last_ms(cache);
     }
     const auto t2 = std::chrono::steady_clock::now();
     out_stats.trace_train_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
@@ -2170,14 +2262,20 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     a.n_items = (int32_t)npx;
     a.accum = ctx->c_frame.p;
     a.accum_prev = nullptr;
-    const unsigned tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
-    k_path<false, false><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
-    HIP_TRY(ctx, hipGetLastError());
+    if (cp->flags & LISA_CACHE_PERSISTENT) {
+      rc = launch_persistent<0>(ctx, a, 3);
+      if (rc != LISA_OK) return rc;
+    } else {
+      const unsigned tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
+      k_path<false, false><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
+      HIP_TRY(ctx, hipGetLastError());
+    }
     HIP_TRY(ctx, hipStreamSynchronize(st));
     out_stats.fell_back = 1;
   } else {
     uint32_t nq = 0;
-    rc = trace_queries(ctx, base, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, &nq);
+    rc = trace_queries(ctx, base, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, &nq,
+                       (cp->flags & LISA_CACHE_PERSISTENT) != 0);
     if (rc != LISA_OK) return rc;
     nq = std::min<uint32_t>(nq, (uint32_t)(npx * p->spp));
     const auto t4 = std::chrono::steady_clock::now();

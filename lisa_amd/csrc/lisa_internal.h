@@ -10,11 +10,12 @@ namespace lisa {
 void set_global_error(const std::string& msg);
 lisa_status global_fail(lisa_status code, const std::string& msg);
 const char* global_error();
-// One Adam step of the radiance cache on device-resident samples (feature
-// rows of feat_stride floats, the first 12 used; targets 3 per row,
-// non-finite targets ignored).
+// `steps` Adam steps of the radiance cache on the same device-resident
+// samples (feature rows of feat_stride floats, the first 12 used; targets 3
+// per row, non-finite targets ignored), enqueued back to back; *loss is the
+// last step's.
 lisa_status cache_train_device(lisa_cache* c, const float* d_feat, int feat_stride,
-                               const float* d_tgt, int64_t n, float* loss);
+                               const float* d_tgt, int64_t n, float* loss, int steps = 1);
 // Network outputs (3 per row) for n device-resident feature rows (12 floats).
 lisa_status cache_infer_device(lisa_cache* c, const float* d_feat, int64_t n, float* d_out);
 }  // namespace lisa

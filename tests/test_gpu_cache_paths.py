@@ -170,3 +170,32 @@ def test_trained_cache_gains_over_truncated_paths(capi, oracle_mod, ctx):
     finally:
         for c in (trained, untrained, zero):
             c.close()
+
+
+def test_persistent_cached_frames_same_image(capi, oracle_mod, ctx):
+    """LISA_CACHE_PERSISTENT runs a frame's paths in one persistent launch.
+    With a network that answers 0 (and no training) the frame is the prefix
+    radiance alone, which both launch forms compute identically."""
+    W, H = 100, 60
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    other = capi.Context(0)
+    a, b = capi.Cache(0, seed=1), capi.Cache(0, seed=1)
+    try:
+        other.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        for c in (a, b):
+            c.set_weights(np.zeros(capi.CACHE_PARAMS, np.float32))
+        tiled = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0)
+        pers = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0,
+                                       flags=capi.CACHE_PERSISTENT)
+        for f in range(3):
+            p = capi.render_params(W, H, 2, 7, eye, U, V, Wv, subframe=f)
+            s1 = capi.render_cached(ctx, a, p, tiled)
+            s2 = capi.render_cached(other, b, p, pers)
+            assert s1["queries"] == s2["queries"] > 0
+            g1, g2 = ctx.copy_accum(W * H), other.copy_accum(W * H)
+            assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32)), f
+    finally:
+        other.close()
+        a.close()
+        b.close()

@@ -334,3 +334,27 @@ def test_rmse_against_16k_spp_oracle(capi, oracle_mod, ctx):
         expect = np.sqrt(var / spp + var / 16384)
         assert 0.5 * expect < e < 1.6 * expect, (spp, e, expect)
     assert errs[1024] < errs[64] / 2.5
+
+
+def test_persistent_launch_same_image(capi, oracle_mod, ctx):
+    """LISA_FLAG_PERSISTENT (lanes pull pixels from a queue) renders the
+    same image as the tiled launch, bit for bit, on both progressive
+    subframes and a partial edge tile (100 x 60)."""
+    sc = cornell(32, 16, 100, 60, 4, 7)
+    _pair(capi, oracle_mod, ctx, sc)
+    U, V, W = camera(sc, capi)
+    eye = sc["camera"]["eye"]
+    plain = capi.Context(0)
+    try:
+        plain.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        for f in range(2):
+            p = capi.render_params(100, 60, 4, 7, eye, U, V, W, subframe=f)
+            want, _ = plain.render(p)
+            p.flags = capi.FLAG_PERSISTENT
+            got, _ = ctx.render(p)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f
+        p = capi.render_params(100, 60, 4, 7, eye, U, V, W, flags=capi.FLAG_PERSISTENT | capi.FLAG_COUNT)
+        with pytest.raises(capi.LisaError):
+            ctx.render(p)
+    finally:
+        plain.close()
