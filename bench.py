@@ -401,7 +401,7 @@ def bench_batch(a):
     seq_s = time.perf_counter() - t
     batch = capi.Batch(ctxs, params)
     cfg = batch.autotune(tune_spp=2)
-    print(f"[bench] autotuned grid/chunk: {cfg}", file=sys.stderr, flush=True)
+    print(f"[bench] autotuned grid/chunk/form: {cfg}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         batch.render()
     t0 = time.perf_counter()
@@ -413,7 +413,9 @@ def bench_batch(a):
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "batch64 wavefront (persistent queue, hipGraph)", "scenes": n,
                       "triangles": tris, "spp": params[0].spp, "bounces": params[0].bounces,
-                      "grid_blocks": cfg[0], "chunk": cfg[1], "parallelism": "single GPU"},
+                      "grid_blocks": cfg[0], "chunk": cfg[1],
+                      "form": "wave tiles" if cfg[2] == 1 else "lane queue",
+                      "parallelism": "single GPU"},
            "rays_per_step": rays, "graph_replay_ms": [round(x, 2) for x in ms],
            "sequential_plain_ms": round(seq_s * 1e3, 2),
            "speedup_vs_sequential_plain": round(seq_s / (wall / a.steps), 3)}

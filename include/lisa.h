@@ -140,7 +140,8 @@ void* lisa_get_stream(const lisa_ctx* ctx);
 typedef struct lisa_batch lisa_batch;
 typedef struct lisa_batch_config {
   int32_t grid_blocks; /* persistent workgroups (64 work-items each) */
-  int32_t chunk;       /* pixels taken from the queue per atomic */
+  int32_t chunk;       /* form 0: pixels a lane takes from the queue per atomic */
+  int32_t form;        /* 0: every lane pulls pixels (any scene); 1: every wave pulls 8x8 tiles */
 } lisa_batch_config;
 
 lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,

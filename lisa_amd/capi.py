@@ -76,7 +76,7 @@ class Stats(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("grid_blocks", C.c_int32), ("chunk", C.c_int32)]
+    _fields_ = [("grid_blocks", C.c_int32), ("chunk", C.c_int32), ("form", C.c_int32)]
 
 
 class SceneInfo(C.Structure):
@@ -398,12 +398,13 @@ class Batch:
             pass
 
     def config(self):
+        """(grid_blocks, chunk, form)."""
         c = BatchConfig()
         _check(lib().lisa_batch_get_config(self.h, C.byref(c)))
-        return c.grid_blocks, c.chunk
+        return c.grid_blocks, c.chunk, c.form
 
-    def set_config(self, grid_blocks, chunk):
-        c = BatchConfig(grid_blocks, chunk)
+    def set_config(self, grid_blocks, chunk, form=0):
+        c = BatchConfig(grid_blocks, chunk, form)
         _check(lib().lisa_batch_set_config(self.h, C.byref(c)))
 
     def render(self):
@@ -412,9 +413,10 @@ class Batch:
         return ms.value
 
     def autotune(self, tune_spp=2):
+        """Measured pick of (grid_blocks, chunk, form)."""
         c = BatchConfig()
         _check(lib().lisa_batch_autotune(self.h, tune_spp, C.byref(c)))
-        return c.grid_blocks, c.chunk
+        return c.grid_blocks, c.chunk, c.form
 
     def accum(self, i):
         p = self.params[i]

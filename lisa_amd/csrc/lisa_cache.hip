@@ -205,12 +205,15 @@ __device__ inline u32x4 wfrag_in(const uint16_t* __restrict__ wt, int nt, int ks
   return u32x4{row[0], row[1], 0u, 0u};
 }
 
+// ReLU as a signed integer max on the float's bits (v_max_i32 with 0): a
+// float with its sign bit set becomes +0, others are kept; one instruction,
+// where fmaxf under IEEE mode needs a quieting step as well.
+__device__ inline float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 // ReLU + BF16 of two D tiles (N tiles 2 ks, 2 ks + 1) into the B operand of K-step ks.
 __device__ inline u32x4 relu_pack(f32x4 lo, f32x4 hi) {
-  return u32x4{pack_bf16(fmaxf(lo[0], 0.0f), fmaxf(lo[1], 0.0f)),
-               pack_bf16(fmaxf(lo[2], 0.0f), fmaxf(lo[3], 0.0f)),
-               pack_bf16(fmaxf(hi[0], 0.0f), fmaxf(hi[1], 0.0f)),
-               pack_bf16(fmaxf(hi[2], 0.0f), fmaxf(hi[3], 0.0f))};
+  return u32x4{pack_bf16(relu(lo[0]), relu(lo[1])), pack_bf16(relu(lo[2]), relu(lo[3])),
+               pack_bf16(relu(hi[0]), relu(hi[1])), pack_bf16(relu(hi[2]), relu(hi[3]))};
 }
 
 __device__ inline bf16x8 bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }

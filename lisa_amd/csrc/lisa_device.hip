@@ -53,6 +53,13 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #ifndef LISA_PATH_MIN_BLOCKS
 #define LISA_PATH_MIN_BLOCKS 1
 #endif
+// The BSDF of opaque materials, chosen at compile time as the reference
+// chooses it with an #include (shader.cu:4): a type with
+// sample(wi, N, seed&, roughness) -> wo and eval(N, L) -> the NEE weight
+// (lisa_math.h: Lambertian, bsdfs/lambertian.cu).
+#ifndef LISA_BSDF
+#define LISA_BSDF Lambertian
+#endif
 constexpr int kStackLds = LISA_STACK_LDS;
 constexpr int kStackMax = 64;
 // Path workgroup: kPathBlock work-items on a kTileW x kTileH pixel tile
@@ -370,7 +377,8 @@ struct Job {
 // suffix radiance (final - so far) / throughput of each vertex becomes a
 // training row.  Without accum_prev the launch writes its own mean (no
 // running mean over subframes; the subframe then only picks the RNG stream).
-template <bool kRecord, bool kCount, class Src, int kMode = 0, bool kF64 = false>
+template <bool kRecord, bool kCount, class Src, int kMode = 0, bool kF64 = false,
+          class Bsdf = LISA_BSDF>
 __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   Job job;
   if (!src.next(job)) return;
@@ -469,7 +477,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // next bounce (shader.cu:251-252), end the sample after the last bounce.
   auto end_nee = [&](F3 nee) -> bool {
     color = color + nee * att;
-    D = Lambertian::sample(Din, N, seed, rough);
+    D = Bsdf::sample(Din, N, seed, rough);
     qstage = 0;
     if (b == pp->bounces) {
       finish_sample();
@@ -481,7 +489,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // or give up after kNeeTries (shader.cu:200-208).
   auto nee_after = [&](int32_t code) -> bool {
     if (kRecord) hash = (hash ^ ((uint32_t)code ^ 0x5bd1e995u)) * 16777619u;
-    if (hitf) return end_nee(xyz(scp->mats[light_mat].emission_emit) * Lambertian::eval(N, D));
+    if (hitf) return end_nee(xyz(scp->mats[light_mat].emission_emit) * Bsdf::eval(N, D));
     if (++tries == kNeeTries) return end_nee(mk(0.0f, 0.0f, 0.0f));
     D = hemisphere(N, seed);
     qstage = 1;
@@ -884,6 +892,52 @@ __global__ __launch_bounds__(kPathBlock) void k_path_queue(const BatchScene* sce
   src.cur = src.end = 0;
   src.bs = scenes;
   path_lane<false, false>(src, stk + threadIdx.x);
+}
+
+// The wave-tile form of the batched launch: each wave takes one 8x8 tile of
+// the concatenated work list at a time and renders it like k_path.  The
+// tile's scene is the same for the whole wave, so its pointers and launch
+// constants are scalar; lanes whose pixel is done wait for the tile.
+struct TileSrc {
+  SceneView sc;
+  PathArgs p;
+  uint32_t tile;
+  bool given;
+  __device__ const SceneView* view() const { return &sc; }
+  __device__ const PathArgs* args() const { return &p; }
+  __device__ bool next(Job& j) {
+    if (given) return false;
+    given = true;
+    const uint32_t w = threadIdx.x & 63u;
+    j.x = (tile % p.tiles_x) * 8 + (w & 7u);
+    j.y = (tile / p.tiles_x) * 8 + (w >> 3);
+    if (j.x >= p.width || j.y >= p.height) return false;
+    j.item = (int32_t)(j.y * p.width + j.x);
+    return true;
+  }
+};
+
+__global__ __launch_bounds__(kPathBlock) void k_path_tiles(const BatchScene* scenes, int32_t n,
+                                                           uint32_t* counter, uint32_t total) {
+  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  const uint32_t n_tiles = total / 64u;
+  for (;;) {
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+    if (t >= n_tiles) break;
+    int32_t lo = 0, hi = n - 1;  // the scene whose work range holds tile t
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (scenes[mid].first / 64u <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    TileSrc src{scenes[lo].sc, scenes[lo].p, t - scenes[lo].first / 64u, false};
+    path_lane<false, false, TileSrc>(src, stk + threadIdx.x);
+  }
 }
 
 // Queue reset inside the captured graph (a kernel, not a memset node).
@@ -1345,7 +1399,7 @@ struct lisa_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  lisa_batch_config cfg{0, 1};
+  lisa_batch_config cfg{0, 1, 0};
   float last_ms = 0.0f;
 };
 
@@ -2102,9 +2156,10 @@ void lisa_batch_destroy(lisa_batch* b) {
 
 lisa_status lisa_batch_set_config(lisa_batch* b, const lisa_batch_config* cfg) {
   if (!b || !cfg) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL argument");
-  if (cfg->grid_blocks <= 0 || cfg->chunk <= 0 || cfg->chunk > 4096)
-    return fail(nullptr, LISA_ERR_INVALID_ARG, "grid_blocks and chunk must be positive (chunk <= 4096)");
-  if (cfg->grid_blocks != b->cfg.grid_blocks || cfg->chunk != b->cfg.chunk) {
+  if (cfg->grid_blocks <= 0 || cfg->chunk <= 0 || cfg->chunk > 4096 || cfg->form < 0 || cfg->form > 1)
+    return fail(nullptr, LISA_ERR_INVALID_ARG,
+                "grid_blocks and chunk must be positive (chunk <= 4096), form 0 or 1");
+  if (cfg->grid_blocks != b->cfg.grid_blocks || cfg->chunk != b->cfg.chunk || cfg->form != b->cfg.form) {
     b->cfg = *cfg;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -2131,8 +2186,12 @@ lisa_status lisa_batch_render(lisa_batch* b, float* elapsed_ms) {
     if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return fail(nullptr, LISA_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
     k_queue_reset<<<1, 1, 0, b->stream>>>(b->d_counter);
-    k_path_queue<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter,
-                                                                   b->total, (uint32_t)b->cfg.chunk);
+    if (b->cfg.form == 1)
+      k_path_tiles<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter,
+                                                                     b->total);
+    else
+      k_path_queue<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter,
+                                                                     b->total, (uint32_t)b->cfg.chunk);
     e = hipStreamEndCapture(b->stream, &b->graph);
     if (e != hipSuccess) return fail(nullptr, LISA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
@@ -2163,21 +2222,24 @@ lisa_status lisa_batch_autotune(lisa_batch* b, int32_t tune_spp, lisa_batch_conf
   const int32_t chunks[] = {1, 4, 16};
   float best = 1e30f;
   lisa_batch_config best_cfg = b->cfg;
-  for (int32_t pc : per_cu)
-    for (int32_t ch : chunks) {
-      lisa_batch_config c{b->cus * pc, ch};
-      float ms = 0.0f, ms2 = 0.0f;
-      if ((rc = lisa_batch_set_config(t, &c)) != LISA_OK || (rc = lisa_batch_render(t, &ms)) != LISA_OK ||
-          (rc = lisa_batch_render(t, &ms2)) != LISA_OK) {
-        lisa_batch_destroy(t);
-        return rc;
-      }
-      ms = std::min(ms, ms2);
-      if (ms < best) {
-        best = ms;
-        best_cfg = c;
-      }
+  std::vector<lisa_batch_config> cands;
+  for (int32_t pc : per_cu) {
+    for (int32_t ch : chunks) cands.push_back(lisa_batch_config{b->cus * pc, ch, 0});
+    cands.push_back(lisa_batch_config{b->cus * pc, 1, 1});
+  }
+  for (const lisa_batch_config& c : cands) {
+    float ms = 0.0f, ms2 = 0.0f;
+    if ((rc = lisa_batch_set_config(t, &c)) != LISA_OK || (rc = lisa_batch_render(t, &ms)) != LISA_OK ||
+        (rc = lisa_batch_render(t, &ms2)) != LISA_OK) {
+      lisa_batch_destroy(t);
+      return rc;
     }
+    ms = std::min(ms, ms2);
+    if (ms < best) {
+      best = ms;
+      best_cfg = c;
+    }
+  }
   lisa_batch_destroy(t);
   rc = lisa_batch_set_config(b, &best_cfg);
   if (rc != LISA_OK) return rc;

@@ -249,7 +249,8 @@ def test_refit_update_bit_exact(capi, oracle_mod, ctx):
 def test_batched_wavefront_bit_exact(capi, oracle_mod):
     """Three scenes of different sizes, spp and bounces in one persistent,
     graph-captured launch: every image equals lisa_render's bit for bit,
-    for several queue configurations and after autotuning."""
+    for both forms (lanes pulling pixels, waves pulling tiles), several
+    queue configurations and after autotuning."""
     specs = [(32, 16, 64, 48, 4, 7), (64, 32, 40, 72, 3, 5), (16, 8, 96, 40, 2, 9)]
     ctxs, params, refs = [], [], []
     for seg, rings, W, H, spp, B in specs:
@@ -262,7 +263,7 @@ def test_batched_wavefront_bit_exact(capi, oracle_mod):
         ctxs.append(c)
         params.append(p)
     batch = capi.Batch(ctxs, params)
-    for cfg in [None, (64, 1), (1024, 4), (300, 16)]:
+    for cfg in [None, (64, 1), (1024, 4), (300, 16), (64, 1, 1), (1024, 1, 1), (7, 1, 1)]:
         if cfg:
             batch.set_config(*cfg)
         for rep in range(2):  # graph replays
@@ -271,7 +272,7 @@ def test_batched_wavefront_bit_exact(capi, oracle_mod):
                 got = batch.accum(i)
                 assert np.array_equal(got.view(np.uint32), refs[i].view(np.uint32)), (cfg, rep, i)
     best = batch.autotune(tune_spp=1)
-    assert best[0] > 0 and best[1] > 0
+    assert best[0] > 0 and best[1] > 0 and best[2] in (0, 1)
     batch.render()
     for i in range(3):
         assert np.array_equal(batch.accum(i).view(np.uint32), refs[i].view(np.uint32))
