@@ -36,6 +36,7 @@ S_NODE, S_TRI, S_SHADE, S_PIXEL = 64, 48, 96, 16
 
 # name -> (scene factory, default spp, default CPU-baseline pixel count)
 WORKLOADS = {
+    "cornell": ("cornell_small", 16, 65536),
     "cornell65k": ("cornell_65k", 256, 65536),
     "icosphere": ("displaced_icosphere", 64, 2048),
     "batch64": ("batch_scene", 32, 0),
@@ -53,14 +54,17 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-pixels", type=int, default=None)
     ap.add_argument("--workload", default="cornell65k", choices=sorted(WORKLOADS),
-                    help="cornell65k: the headline configuration; icosphere: the "
+                    help="cornell: the 256^2 case (CPU oracle on the whole image); "
+                         "cornell65k: the headline configuration; icosphere: the "
                          "traversal-bound 5.2M-triangle scene; batch64: the batched "
                          "wavefront form; island / cornell_cache: time to RMSE <= 1e-2, "
                          "learned cache vs plain")
-    ap.add_argument("--cache-spp", type=int, default=4, help="spp per cached frame")
+    ap.add_argument("--cache-spp", type=int, default=16, help="spp per cached frame")
     ap.add_argument("--prefix", type=int, default=2, help="radiance rays before the cache")
-    ap.add_argument("--train-paths", type=int, default=16384)
-    ap.add_argument("--train-steps", type=int, default=2)
+    ap.add_argument("--train-paths", type=int, default=262144,
+                    help="training paths per frame (their rows at the query depth: ~2^16 on "
+                         "the island)")
+    ap.add_argument("--train-steps", type=int, default=4)
     ap.add_argument("--rmse-limit-s", type=float, default=30.0,
                     help="give up a method after this much render time")
     ap.add_argument("--persistent", action="store_true",
@@ -158,10 +162,21 @@ def main():
         r = o.render(W, H, spp, B, cam["eye"], U, V, Wv, pixels=px, threads=threads)
         dt = time.perf_counter() - t
         crays = int(r["counters"][0] + r["counters"][1])
+        # one thread on a sixteenth of the sample (about the same wall time)
+        px1 = px[:: max(threads, 1)]
+        t = time.perf_counter()
+        r1 = o.render(W, H, spp, B, cam["eye"], U, V, Wv, pixels=px1, threads=1)
+        dt1 = time.perf_counter() - t
+        crays1 = int(r1["counters"][0] + r1["counters"][1])
+        whole = " (the whole image)" if len(px) == W * H else ""
         cpu = {"value": round(crays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                "kind": "port",
-               "sample": f"{cpu_pixels} seeded pixels of the same {W}x{H} render at "
-                         f"{spp} spp x {B} bounces ({crays} rays, {dt:.1f} s, OpenMP over pixels)"}
+               "sample": f"{len(px)} seeded pixels of the same {W}x{H} render{whole} at "
+                         f"{spp} spp x {B} bounces ({crays} rays, {dt:.1f} s, OpenMP over pixels)",
+               "samples_per_s": round(len(px) * spp / dt, 1),
+               "single_thread": {"value": round(crays1 / dt1 / 1e6, 3), "unit": "Mrays/s",
+                                 "cores": 1, "sample": f"every {max(threads, 1)}th pixel of that "
+                                 f"sample ({len(px1)} pixels, {crays1} rays, {dt1:.1f} s)"}}
         print(f"[bench] cpu oracle: {cpu}", file=sys.stderr, flush=True)
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,

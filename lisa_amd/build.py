@@ -47,8 +47,9 @@ def build(verbose_resources=False, defines=(), out_dir=OUT, cli=True):
     extra += [f"-D{d}" for d in defines]
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, *FP_FLAGS, *HIP_FLAGS, *extra, "-c",
           os.path.join(SRC, "lisa_device.hip"), "-o", dev_o])
-    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, *FP_FLAGS, *HIP_FLAGS, *extra, "-c",
-          os.path.join(SRC, "lisa_cache.hip"), "-o", cache_o])
+    # the network's MFMA accumulators in VGPRs (no AGPR copies in the forward loop)
+    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, *FP_FLAGS, *HIP_FLAGS, *extra,
+          "-mllvm", "-amdgpu-mfma-vgpr-form", "-c", os.path.join(SRC, "lisa_cache.hip"), "-o", cache_o])
     _run(["g++", *COMMON, *FP_FLAGS, "-c", os.path.join(SRC, "lisa_host.cpp"), "-o", host_o])
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", dev_o, cache_o, host_o, "-o", tmp,

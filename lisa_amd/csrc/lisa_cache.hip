@@ -166,8 +166,9 @@ __device__ inline f32x4 layer_fwd(int l, const uint16_t* in, int in_stride, cons
 }
 
 // ---- inference with the weights and activations in registers ----
-// Each layer computes Z^T = W^T X^T: the weights are the A operand (held in
-// VGPRs for the whole launch), the activations the B operand.  Lane (q, r)
+// Each layer computes Z^T = W^T X^T: the weights are the A operand (the
+// hidden layers' held in VGPRs for the whole launch, layer 0's in LDS), the
+// activations the B operand.  Lane (q, r)
 // then holds D[4q .. 4q+3][r] = Z[row r][16 nt + 4q .. 4q+3], which after
 // ReLU and BF16 rounding is its own B operand of the next layer if the K
 // order of that layer's weights is permuted to match: K-step ks, lane q,
@@ -205,6 +206,33 @@ __device__ inline u32x4 wfrag_in(const uint16_t* __restrict__ wt, int nt, int ks
   return u32x4{row[0], row[1], 0u, 0u};
 }
 
+// sin(pi x) and cos(pi x) for the encoder's inputs (x in [0, 1]): x = n + r
+// with n = rint(x), r in [-1/2, 1/2]; the Taylor series of sin(pi r) to r^13
+// and of cos(pi r) to r^12 in Horner form (|error| < 3e-7 in FP32, against a
+// BF16 rounding of 2^-9 for the features); (-1)^n for the integer part.
+__device__ inline void sincospi_unit(float x, float& s, float& c) {
+  const float n = rintf(x);
+  const float r = x - n;
+  const float z = r * r;
+  float ps = 4.6630279e-4f;
+  ps = fmaf(ps, z, -7.3704310e-3f);
+  ps = fmaf(ps, z, 8.2145885e-2f);
+  ps = fmaf(ps, z, -5.9926450e-1f);
+  ps = fmaf(ps, z, 2.5501640e+0f);
+  ps = fmaf(ps, z, -5.1677127e+0f);
+  ps = fmaf(ps, z, 3.1415927e+0f);
+  float pc = 1.9295743e-3f;
+  pc = fmaf(pc, z, -2.5806891e-2f);
+  pc = fmaf(pc, z, 2.3533063e-1f);
+  pc = fmaf(pc, z, -1.3352628e+0f);
+  pc = fmaf(pc, z, 4.0587121e+0f);
+  pc = fmaf(pc, z, -4.9348021e+0f);
+  pc = fmaf(pc, z, 1.0f);
+  const float sign = ((int)n & 1) ? -1.0f : 1.0f;
+  s = sign * (ps * r);
+  c = sign * pc;
+}
+
 // ReLU as a signed integer max on the float's bits (v_max_i32 with 0): a
 // float with its sign bit set becomes +0, others are kept; one instruction,
 // where fmaxf under IEEE mode needs a quieting step as well.
@@ -222,16 +250,23 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
                                                                int64_t n,
                                                                const uint16_t* __restrict__ wt,
                                                                float* __restrict__ out) {
+  // layer-0 A fragments (80 VGPRs' worth) in LDS, [nt][ks][lane]: one
+  // conflict-free 16-byte read per lane per MFMA; the hidden layers' stay in
+  // registers
+  __shared__ u32x4 s_w0[4 * 5 * 64];
+  for (int e = threadIdx.x; e < 4 * 5 * 64; e += blockDim.x) {
+    const int l = e & 63, f = e >> 6;
+    s_w0[e] = wfrag_in(wt, f / 5, f % 5, l >> 4, l & 15);
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
   const int64_t tiles = (n + 15) / 16;
   const int64_t first = (int64_t)blockIdx.x * kInferWaves + (threadIdx.x >> 6);
   const int64_t stride = (int64_t)gridDim.x * kInferWaves;
   if (first >= tiles) return;
-  u32x4 w0[4][5], w1[4][2], w2[4][2], w3[4][2], w4[2];
+  u32x4 w1[4][2], w2[4][2], w3[4][2], w4[2];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-#pragma unroll
-    for (int ks = 0; ks < 5; ++ks) w0[nt][ks] = wfrag_in(wt, nt, ks, q, r);
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       w1[nt][ks] = wfrag_hidden(wt, 1, nt, ks, q, r);
@@ -259,7 +294,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float sn, cs;
-      sincospif(p[k], &sn, &cs);
+      sincospi_unit(p[k], sn, cs);
 #pragma unroll
       for (int b = 0; b < kBands; ++b) {
         e[12 * k + 2 * b] = sn;
@@ -276,12 +311,18 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
     for (int ks = 0; ks < 5; ++ks)
       x[ks] = u32x4{pack_bf16(e[8 * ks], e[8 * ks + 1]), pack_bf16(e[8 * ks + 2], e[8 * ks + 3]),
                     pack_bf16(e[8 * ks + 4], e[8 * ks + 5]), pack_bf16(e[8 * ks + 6], e[8 * ks + 7])};
+    // the lane's LDS offset, opaque to the compiler on every tile: the layer-0
+    // fragments are read here each time instead of being hoisted into
+    // registers (which would push the accumulators into AGPRs)
+    int w0_lane = lane;
+    asm volatile("" : "+v"(w0_lane));
+    const u32x4* w0 = s_w0 + w0_lane;
     f32x4 z[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
       z[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int ks = 0; ks < 5; ++ks) z[nt] = mfma(bf(w0[nt][ks]), bf(x[ks]), z[nt]);
+      for (int ks = 0; ks < 5; ++ks) z[nt] = mfma(bf(w0[(nt * 5 + ks) * 64]), bf(x[ks]), z[nt]);
     }
     u32x4 h[2];
     auto hidden = [&](u32x4 (&w)[4][2]) {
@@ -314,12 +355,13 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
   }
 }
 
-// Blocks of the inference launch: enough waves for every tile, at most two
-// blocks per CU (each wave loads the 47 KB of weight fragments once).
+// Blocks of the inference launch: enough waves for every tile, at most three
+// 4-wave blocks per CU (the kernel's occupancy; each wave loads its 27 KB of
+// hidden-layer fragments and each block 20 KB of layer-0 fragments once).
 int infer_blocks(int cus, int64_t n) {
   const int64_t tiles = (n + 15) / 16;
   return (int)std::max<int64_t>(1, std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves,
-                                                     2 * (int64_t)cus));
+                                                     3 * (int64_t)cus));
 }
 
 constexpr int kTrainWaves = 2;

@@ -1636,7 +1636,8 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
 lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, size_t cap,
                              uint32_t depth, uint32_t* rows) {
   hipStream_t st = ctx->stream;
-  const size_t slots = a.bounces;
+  // vertex slots per path: one hit per sample at a fixed depth, else up to one per bounce
+  const size_t slots = depth > 0 ? 1 : a.bounces;
   HIP_TRY(ctx, ctx->c_trec.ensure(n * slots * 18));
   HIP_TRY(ctx, ctx->c_tfeat.ensure(cap * 12));
   HIP_TRY(ctx, ctx->c_ttgt.ensure(cap * 3));
