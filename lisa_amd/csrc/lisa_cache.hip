@@ -844,7 +844,3 @@ lisa_status lisa::cache_infer_device(lisa_cache* c, const float* d_feat, int64_t
   CHIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
   return LISA_OK;
 }
-
-extern "C" {
-
-}  // extern "C"
