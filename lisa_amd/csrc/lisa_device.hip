@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -90,6 +91,7 @@ struct SceneView {
   BvhView emit;  // emitting triangles only (NEE light queries)
   const float4* __restrict__ nrm;  // vertex normals, Morton order of `all`
   const DevMat* __restrict__ mats;
+  float4 ecull_c, ecull_h;  // emitter cull box: centre, inflated half extents (emit_culled)
 };
 
 // ---------------------------------------------------------------------------
@@ -135,7 +137,7 @@ __device__ __forceinline__ RayT make_ray(F3 O, F3 D) {
   r.kz = kz;
   r.Sx = sel(D, kx) / dz;
   r.Sy = sel(D, ky) / dz;
-  r.Sz = 1.0f / dz;
+  r.Sz = sel(r.inv, kz);  // 1 / D[kz], already rounded once for the slab tests
   return r;
 }
 
@@ -175,6 +177,31 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
   const float tf = fminf(tfb, tmax);
   tn_out = tn;
   return tn <= tf;
+}
+
+// Division-free cull of NEE's emitter query.  The box (c, H) holds every
+// emitter, inflated by twice a margin m = S * 2^-12 per side, where S is the
+// largest |coordinate| of the scene box.  A ray whose origin lies in the
+// scene box and that misses the inflated box by a separating axis test
+// evaluated in FP32 misses the box inflated by m in exact arithmetic; the
+// slab tests of the emitter root's child boxes (each inside the box) then
+// miss in FP32 too -- their rounding error is below 16 S u / |D_i| (u = 2^-24)
+// per t value against a gap of at least m / |D_i| + m / |D_j| -- so the query
+// has the traversal's answer, "no emitter", without the ray's divisions.
+// Tested axes: the box behind the origin along an axis the ray moves away
+// on, and the line's separating axes D x e_x, D x e_y, D x e_z.  A direction
+// with a zero component is never culled.
+__device__ __forceinline__ bool emit_culled(float4 c, float4 H, F3 O, F3 D) {
+  const float ax = fabsf(D.x), ay = fabsf(D.y), az = fabsf(D.z);
+  if (ax == 0.0f || ay == 0.0f || az == 0.0f) return false;
+  const float vx = c.x - O.x, vy = c.y - O.y, vz = c.z - O.z;
+  if (D.x > 0.0f ? vx < -H.x : vx > H.x) return true;
+  if (D.y > 0.0f ? vy < -H.y : vy > H.y) return true;
+  if (D.z > 0.0f ? vz < -H.z : vz > H.z) return true;
+  if (fabsf(vy * D.z - vz * D.y) > H.y * az + H.z * ay) return true;
+  if (fabsf(vz * D.x - vx * D.z) > H.z * ax + H.x * az) return true;
+  if (fabsf(vx * D.y - vy * D.x) > H.x * ay + H.y * ax) return true;
+  return false;
 }
 
 struct TravCount {
@@ -546,7 +573,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       if (kRecord) occl_cnt++;
       int32_t k = -1;
       float t = 0.0f;
-      if (valid && scp->emit.T > 0) {
+      if (valid && scp->emit.T > 0 && !emit_culled(scp->ecull_c, scp->ecull_h, O, D)) {
         const RayT r = make_ray(O, D);
         k = closest<kCount, kPathBlock>(scp->emit, r, kTmin, kTmax, 0xFFFFFFFFu, false, t, my_stk, tc);
       }
@@ -1380,6 +1407,7 @@ struct lisa_ctx {
   float render_ms = 0.0f, bvh_ms = 0.0f;
   uint64_t scene_bytes = 0, accum_bytes = 0;
   float bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};  // scene box (cache features)
+  float ecull_c[3] = {0, 0, 0}, ecull_h[3] = {INFINITY, INFINITY, INFINITY};  // never culls
   // radiance-cache frame scratch (lisa_render_cached), grown on demand
   DevBuf<float4> c_frame, c_tacc;
   DevBuf<float> c_qfeat, c_qatt, c_qout, c_trec, c_tfeat, c_ttgt;
@@ -1444,7 +1472,11 @@ void free_scene(lisa_ctx* c) {
 
 SceneView view(const lisa_ctx* c) {
   return SceneView{BvhView{c->all.nodes, c->all.tris, c->all.T},
-                   BvhView{c->emit.nodes, c->emit.tris, c->emit.T}, c->d_nrm, c->d_mats};
+                   BvhView{c->emit.nodes, c->emit.tris, c->emit.T},
+                   c->d_nrm,
+                   c->d_mats,
+                   make_float4(c->ecull_c[0], c->ecull_c[1], c->ecull_c[2], 0.0f),
+                   make_float4(c->ecull_h[0], c->ecull_h[1], c->ecull_h[2], 0.0f)};
 }
 
 // Device scratch freed at scope exit.
@@ -1604,15 +1636,46 @@ lisa_status launch_persistent(lisa_ctx* ctx, PathArgs a, int slot) {
 }
 
 // Scene box of the cache features: union of the root's two child boxes.
+// Union of the child boxes of a root node (its 16 words).
+void root_box(const float* rb, float lo[3], float hi[3]) {
+  uint32_t ref[2];
+  std::memcpy(ref, rb + 12, 8);
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = INFINITY;
+    hi[a] = -INFINITY;
+  }
+  for (int ch = 0; ch < 2; ++ch) {
+    if (ref[ch] == kEmpty) continue;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], rb[6 * ch + a]);
+      hi[a] = std::max(hi[a], rb[6 * ch + 3 + a]);
+    }
+  }
+}
+
+// The scene box (root of `all`) and the emitter cull box (emit_culled): the
+// emitter root box inflated by S * 2^-11 per side, S the largest |coordinate|
+// of the scene box.
 lisa_status read_scene_box(lisa_ctx* ctx) {
   const int64_t T = ctx->all.T;
+  for (int a = 0; a < 3; ++a) {
+    ctx->ecull_c[a] = 0.0f;
+    ctx->ecull_h[a] = INFINITY;
+  }
   if (T <= 0) return LISA_OK;
   float rb[16];
   HIP_TRY(ctx, hipMemcpy(rb, ctx->all.nodes, 64, hipMemcpyDeviceToHost));
-  const bool two = T > 1;
+  root_box(rb, ctx->bmin, ctx->bmax);
+  if (ctx->emit.T <= 0) return LISA_OK;
+  float S = 0.0f;
+  for (int a = 0; a < 3; ++a) S = std::max(S, std::max(std::fabs(ctx->bmin[a]), std::fabs(ctx->bmax[a])));
+  const float m2 = S * 0x1p-11f;
+  float lo[3], hi[3];
+  HIP_TRY(ctx, hipMemcpy(rb, ctx->emit.nodes, 64, hipMemcpyDeviceToHost));
+  root_box(rb, lo, hi);
   for (int a = 0; a < 3; ++a) {
-    ctx->bmin[a] = two ? std::min(rb[a], rb[6 + a]) : rb[a];
-    ctx->bmax[a] = two ? std::max(rb[3 + a], rb[9 + a]) : rb[3 + a];
+    ctx->ecull_c[a] = 0.5f * (lo[a] + hi[a]);
+    ctx->ecull_h[a] = 0.5f * (hi[a] - lo[a]) + m2;
   }
   return LISA_OK;
 }

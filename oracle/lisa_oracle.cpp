@@ -175,6 +175,22 @@ struct Bvh {
   std::vector<float> tv;               // 9 per sorted triangle (traversal copy)
 };
 
+// Division-free cull of NEE's emitter query (this implementation's, not the
+// reference's): a box around every emitter, centre c and half extents H,
+// inflated by twice a margin m = S * 2^-12 per side, where S is the largest
+// |coordinate| of the scene box.  A ray whose origin lies in the scene box
+// and that misses the inflated box by a separating axis test evaluated in
+// FP32 misses the box inflated by m in exact arithmetic; the slab tests of
+// the emitter root's two child boxes (each inside the box) then miss in FP32
+// too, since their rounding error is below 16 S u / |D_i| (u = 2^-24) per
+// t value against a gap of at least m / |D_i| + m / |D_j| -- so the query
+// answers "no emitter" exactly as the traversal would, without the ray's
+// divisions.  Counted as no node visit.
+struct EmitCull {
+  float c[3] = {0.0f, 0.0f, 0.0f};
+  float H[3] = {INFINITY, INFINITY, INFINITY};  // never culls
+};
+
 struct Scene {
   int32_t T = 0;
   std::vector<float> v, n;     // 9T each, original order
@@ -182,6 +198,7 @@ struct Scene {
   std::vector<orc_material> mats;
   Bvh all;   // every triangle
   Bvh emit;  // the triangles whose material emits (light queries of NEE)
+  EmitCull ecull;
 };
 
 inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
@@ -536,6 +553,53 @@ bool nee_loop(RayState& st, Sample sample_dir, Occlude occlude, Emits emits, V3&
 
 constexpr int32_t kNonEmitter = 0x7FFFFFFF;  // a material id that never emits
 
+// Union of the root's child boxes.
+void root_box(const Bvh& b, float lo[3], float hi[3]) {
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = INFINITY;
+    hi[a] = -INFINITY;
+  }
+  for (int ch = 0; ch < 2; ++ch) {
+    if (b.refs[ch] == kEmpty) continue;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], b.boxes[6 * ch + a]);
+      hi[a] = std::max(hi[a], b.boxes[6 * ch + 3 + a]);
+    }
+  }
+}
+
+EmitCull emit_cull(const Scene& s) {
+  EmitCull e;
+  if (s.T == 0 || s.emit.T == 0) return e;
+  float lo[3], hi[3], S = 0.0f;
+  root_box(s.all, lo, hi);
+  for (int a = 0; a < 3; ++a) S = std::max(S, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+  const float m2 = S * 0x1p-11f;
+  root_box(s.emit, lo, hi);
+  for (int a = 0; a < 3; ++a) {
+    e.c[a] = 0.5f * (lo[a] + hi[a]);
+    e.H[a] = 0.5f * (hi[a] - lo[a]) + m2;
+  }
+  return e;
+}
+
+// True when the ray (O, D) certainly misses every emitter: the box lies
+// behind the origin along an axis the ray moves away on, or the ray's line
+// passes outside the box on one of the axes D x e_x, D x e_y, D x e_z.  A
+// direction with a zero component is never culled.
+bool emit_culled(const EmitCull& e, V3 O, V3 D) {
+  const float ax = std::fabs(D.x), ay = std::fabs(D.y), az = std::fabs(D.z);
+  if (ax == 0.0f || ay == 0.0f || az == 0.0f) return false;
+  const float vx = e.c[0] - O.x, vy = e.c[1] - O.y, vz = e.c[2] - O.z;
+  if (D.x > 0.0f ? vx < -e.H[0] : vx > e.H[0]) return true;
+  if (D.y > 0.0f ? vy < -e.H[1] : vy > e.H[1]) return true;
+  if (D.z > 0.0f ? vz < -e.H[2] : vz > e.H[2]) return true;
+  if (std::fabs(vy * D.z - vz * D.y) > e.H[1] * az + e.H[2] * ay) return true;
+  if (std::fabs(vz * D.x - vx * D.z) > e.H[2] * ax + e.H[0] * az) return true;
+  if (std::fabs(vx * D.y - vy * D.x) > e.H[0] * ay + e.H[1] * ax) return true;
+  return false;
+}
+
 // The occlusion query of NEE.  Returns the material of the closest hit when
 // it emits, -1 on a miss, kNonEmitter when the closest hit does not emit --
 // exactly the outcomes shader.cu:172-184 distinguish, computed as:
@@ -555,7 +619,9 @@ int32_t occlusion_query(const Scene& s, V3 O, V3 D, bool hit_before, int32_t& co
     return -1;
   }
   float tE = 0.0f, tb = 0.0f;
-  const int32_t e = closest<false>(s.emit, r, kTmin, kTmax, 0xFFFFFFFFu, tE, cnt);
+  const int32_t e = emit_culled(s.ecull, O, D)
+                        ? -1
+                        : closest<false>(s.emit, r, kTmin, kTmax, 0xFFFFFFFFu, tE, cnt);
   if (e >= 0) {
     if (closest<true>(s.all, r, kTmin, tE, (uint32_t)e, tb, cnt) < 0) {
       code = e;
@@ -918,6 +984,7 @@ orc_scene* orc_scene_create(const float* verts, const float* normals, const int3
   }
   build_bvh(s.all, s.v, gid_all);
   build_bvh(s.emit, s.v, gid_emit);
+  s.ecull = emit_cull(s);
   return o;
 }
 void orc_scene_destroy(orc_scene* s) { delete s; }
@@ -928,6 +995,7 @@ void orc_scene_update_vertices(orc_scene* o, const float* verts, const float* no
   if (normals) s.n.assign(normals, normals + 9 * (size_t)s.T);
   update_bvh(s.all, s.v);
   update_bvh(s.emit, s.v);
+  s.ecull = emit_cull(s);
 }
 int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
 

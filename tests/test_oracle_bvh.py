@@ -208,6 +208,56 @@ def test_occlusion_query_equals_closest_hit(oracle_mod, seed):
             assert (fast == -4).sum() > 1000 and (fast == -1).sum() > 100
 
 
+@pytest.mark.parametrize("scale,shift", [(1.0, (0.0, 0.0, 0.0)), (40.0, (3000.0, -700.0, 250.0))])
+def test_emitter_cull_is_conservative(oracle_mod, scale, shift):
+    """The division-free cull in front of NEE's emitter query never drops a
+    ray that the full query would answer with an emitter: rays aimed at the
+    light quad's corners and at points within 1e-7 .. 1e-2 (relative) of its
+    edges, inside and outside, from points all over the box, in the unit
+    scene and in a scaled, translated copy (the cull's margin scales with
+    the largest coordinate).  Rays in random directions are culled: they
+    cost no node visit unless they pass near a light."""
+    sc = cornell(32, 16, 8, 8, 1, 1)
+    shift = np.asarray(shift, np.float64)
+    V = (np.asarray(sc["verts"], np.float64).reshape(-1, 3) * scale + shift)
+    verts = V.astype(np.float32).reshape(np.asarray(sc["verts"]).shape)
+    o = oracle_mod.Scene(verts, sc["normals"], sc["mat_idx"], sc["materials"])
+    emits = np.array([bool(m.get("emit", False)) for m in sc["materials"]])
+    light = verts.reshape(-1, 3, 3)[emits[np.asarray(sc["mat_idx"])]].reshape(-1, 3).astype(np.float64)
+    lo, hi = light.min(0), light.max(0)
+    rs = np.random.default_rng(7)
+    n = 40000
+    O = (rs.uniform(-0.49, 0.49, size=(n, 3)) * scale + shift)
+    # targets on the light's edges and corners, pushed in or out along the
+    # edge normal in the light's plane by a relative offset
+    flat = int(np.argmin(hi - lo))
+    ax = [a for a in range(3) if a != flat]
+    T = np.empty((n, 3))
+    T[:, flat] = lo[flat]
+    u = rs.uniform(0, 1, n)
+    side = rs.integers(0, 4, n)
+    off = 10.0 ** rs.uniform(-7, -2, n) * scale * rs.choice([-1.0, 1.0], n)
+    for k in range(4):
+        m = side == k
+        a_run, a_fix = (ax[0], ax[1]) if k < 2 else (ax[1], ax[0])
+        T[m, a_run] = lo[a_run] + u[m] * (hi[a_run] - lo[a_run])
+        T[m, a_fix] = (lo[a_fix] if k % 2 == 0 else hi[a_fix]) + off[m] * (1 if k % 2 else -1)
+    corners = rs.random(n) < 0.1
+    T[corners, ax[0]] = np.where(rs.random(corners.sum()) < 0.5, lo[ax[0]], hi[ax[0]])
+    T[corners, ax[1]] = np.where(rs.random(corners.sum()) < 0.5, lo[ax[1]], hi[ax[1]])
+    O32 = O.astype(np.float32)
+    D = (T - O32).astype(np.float32)
+    for hb in (0, 1):
+        fast, ref, _ = o.occlusion(O32, D, np.full(n, hb))
+        assert np.array_equal(fast, ref)
+        assert (fast >= 0).sum() > 5000
+    # random directions: culled unless they pass near the light
+    R = rs.normal(size=(n, 3)).astype(np.float32)
+    fast, ref, cnt = o.occlusion(O32, R, np.zeros(n, np.int32))
+    assert np.array_equal(fast, ref)
+    assert int(cnt[2]) < 0.25 * n
+
+
 def test_refit_update_keeps_topology(oracle_mod):
     """Refit-only update: new positions, same Morton order and hierarchy;
     boxes are the unions of the new triangle boxes and traversal still
