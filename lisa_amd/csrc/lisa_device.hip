@@ -188,20 +188,31 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
 // miss in FP32 too -- their rounding error is below 16 S u / |D_i| (u = 2^-24)
 // per t value against a gap of at least m / |D_i| + m / |D_j| -- so the query
 // has the traversal's answer, "no emitter", without the ray's divisions.
-// Tested axes: the box behind the origin along an axis the ray moves away
-// on, and the line's separating axes D x e_x, D x e_y, D x e_z.  A direction
+//
+// line_culled: the line through O along d misses the box (separating axes
+// d x e_x, d x e_y, d x e_z).  It does not depend on the length or sign of
+// d, so it can run on hemisphere()'s raw draws before they are normalised:
+// the normalised direction is parallel to them up to a few ulps, an angle
+// whose offset over the scene (< 16 S u) the margin absorbs.  A direction
 // with a zero component is never culled.
-__device__ __forceinline__ bool emit_culled(float4 c, float4 H, F3 O, F3 D) {
-  const float ax = fabsf(D.x), ay = fabsf(D.y), az = fabsf(D.z);
+__device__ __forceinline__ bool line_culled(float4 c, float4 H, F3 O, F3 d) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
   if (ax == 0.0f || ay == 0.0f || az == 0.0f) return false;
+  const float vx = c.x - O.x, vy = c.y - O.y, vz = c.z - O.z;
+  if (fabsf(vy * d.z - vz * d.y) > H.y * az + H.z * ay) return true;
+  if (fabsf(vz * d.x - vx * d.z) > H.z * ax + H.x * az) return true;
+  if (fabsf(vx * d.y - vy * d.x) > H.x * ay + H.y * ax) return true;
+  return false;
+}
+// emit_culled: the ray misses the box -- the box lies behind the origin along
+// an axis the ray moves away on, or the ray's line misses it.
+__device__ __forceinline__ bool emit_culled(float4 c, float4 H, F3 O, F3 D) {
+  if (D.x == 0.0f || D.y == 0.0f || D.z == 0.0f) return false;
   const float vx = c.x - O.x, vy = c.y - O.y, vz = c.z - O.z;
   if (D.x > 0.0f ? vx < -H.x : vx > H.x) return true;
   if (D.y > 0.0f ? vy < -H.y : vy > H.y) return true;
   if (D.z > 0.0f ? vz < -H.z : vz > H.z) return true;
-  if (fabsf(vy * D.z - vz * D.y) > H.y * az + H.z * ay) return true;
-  if (fabsf(vz * D.x - vx * D.z) > H.z * ax + H.x * az) return true;
-  if (fabsf(vx * D.y - vy * D.x) > H.x * ay + H.y * ax) return true;
-  return false;
+  return line_culled(c, H, O, D);
 }
 
 struct TravCount {
@@ -518,7 +529,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     if (kRecord) hash = (hash ^ ((uint32_t)code ^ 0x5bd1e995u)) * 16777619u;
     if (hitf) return end_nee(xyz(scp->mats[light_mat].emission_emit) * Bsdf::eval(N, D));
     if (++tries == kNeeTries) return end_nee(mk(0.0f, 0.0f, 0.0f));
-    D = hemisphere(N, seed);
+    D = hemisphere_raw(seed);  // finished in the emitter-query loop
     qstage = 1;
     return true;
   };
@@ -564,13 +575,19 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     // false (shader.cu:172-184), so with the flag false a ray that hits no
     // emitter is resolved here; these cheap tries loop without waiting for
     // the wave's full traversals.
+    // D holds hemisphere()'s raw draws; with the flag false a try whose line
+    // misses every emitter ends here without normalising them.
     while (qstage == 1) {
-      const bool valid = ray_valid(D);
-      if (kCount) {
-        c_occ++;
-        if (!valid) c_inv++;
-      }
+      if (kCount) c_occ++;
       if (kRecord) occl_cnt++;
+      if (!hitf && line_culled(scp->ecull_c, scp->ecull_h, O, D)) {
+        alive = nee_after(-5);
+        if (!alive) break;
+        continue;
+      }
+      D = hemisphere_dir(D, N);
+      const bool valid = ray_valid(D);
+      if (kCount && !valid) c_inv++;
       int32_t k = -1;
       float t = 0.0f;
       if (valid && scp->emit.T > 0 && !emit_culled(scp->ecull_c, scp->ecull_h, O, D)) {
@@ -681,7 +698,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
             rough = m.rough_n.x;
             qstage = 1;
             tries = 0;
-            D = hemisphere(N, seed);
+            D = hemisphere_raw(seed);
           }
         }
       }

@@ -57,13 +57,19 @@ __device__ __forceinline__ float rnd(uint32_t& s) {
 __device__ __forceinline__ float rng(uint32_t& s) { return rnd(s) * 2.0f - 1.0f; }
 
 // ---- maths.cu ----
-__device__ __forceinline__ F3 hemisphere(F3 N, uint32_t& s) {
+// shoot_ray_hemisphere in two steps: the three draws (x, y, z), then the
+// normalised direction turned to the side of N.
+__device__ __forceinline__ F3 hemisphere_raw(uint32_t& s) {
   const float x = rng(s);
   const float y = rng(s);
   const float z = rng(s);
-  const F3 r = normalize(mk(x, y, z));
+  return mk(x, y, z);
+}
+__device__ __forceinline__ F3 hemisphere_dir(F3 raw, F3 N) {
+  const F3 r = normalize(raw);
   return faceforward(r, N, r);
 }
+__device__ __forceinline__ F3 hemisphere(F3 N, uint32_t& s) { return hemisphere_dir(hemisphere_raw(s), N); }
 __device__ __forceinline__ float fresnel(float cosT, float eta) {
   const float r = (1.0f - eta) / (1.0f + eta);
   const float R0 = r * r;

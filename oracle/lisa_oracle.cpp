@@ -74,13 +74,19 @@ inline float rnd(uint32_t& s) { return (float)lcg(s) / (float)0x01000000; }
 inline float rng(uint32_t& s) { return rnd(s) * 2.0f - 1.0f; }
 
 // maths.cu:10-15 -- x, then y, then z.
-inline V3 hemisphere(V3 N, uint32_t& s) {
+// in two steps: the three draws, then the normalised direction turned to
+// the side of N (the NEE loop culls on the raw draws first).
+inline V3 hemisphere_raw(uint32_t& s) {
   const float x = rng(s);
   const float y = rng(s);
   const float z = rng(s);
-  const V3 r = normalize(mk(x, y, z));
+  return mk(x, y, z);
+}
+inline V3 hemisphere_dir(V3 raw, V3 N) {
+  const V3 r = normalize(raw);
   return faceforward(r, N, r);
 }
+inline V3 hemisphere(V3 N, uint32_t& s) { return hemisphere_dir(hemisphere_raw(s), N); }
 // maths.cu:17-20 (Schlick); pow(r,2) = r*r, pow(x,5) = ((x*x)*(x*x))*x.
 inline float fresnel(float cosT, float eta) {
   const float r = (1.0f - eta) / (1.0f + eta);
@@ -534,7 +540,7 @@ template <class Sample, class Occlude, class Emits>
 bool nee_loop(RayState& st, Sample sample_dir, Occlude occlude, Emits emits, V3& dir_out,
               int& tries) {
   for (int i = 0; i < kNeeTries; ++i) {
-    const V3 dir = sample_dir();
+    V3 dir = sample_dir();  // occlude may finish it in place
     ++tries;
     const int32_t m = occlude(dir, st.hit);
     if (m < 0) {
@@ -583,21 +589,31 @@ EmitCull emit_cull(const Scene& s) {
   return e;
 }
 
-// True when the ray (O, D) certainly misses every emitter: the box lies
-// behind the origin along an axis the ray moves away on, or the ray's line
-// passes outside the box on one of the axes D x e_x, D x e_y, D x e_z.  A
-// direction with a zero component is never culled.
-bool emit_culled(const EmitCull& e, V3 O, V3 D) {
-  const float ax = std::fabs(D.x), ay = std::fabs(D.y), az = std::fabs(D.z);
+// True when the line through O along d certainly misses every emitter: it
+// passes outside the box on one of the axes d x e_x, d x e_y, d x e_z.  The
+// test does not depend on the length or sign of d, so the NEE loop runs it
+// on hemisphere()'s raw draws (parallel to the normalised direction up to a
+// few ulps, an offset over the scene below 16 S u that the margin absorbs).
+// A direction with a zero component is never culled.
+bool line_culled(const EmitCull& e, V3 O, V3 d) {
+  const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
   if (ax == 0.0f || ay == 0.0f || az == 0.0f) return false;
+  const float vx = e.c[0] - O.x, vy = e.c[1] - O.y, vz = e.c[2] - O.z;
+  if (std::fabs(vy * d.z - vz * d.y) > e.H[1] * az + e.H[2] * ay) return true;
+  if (std::fabs(vz * d.x - vx * d.z) > e.H[2] * ax + e.H[0] * az) return true;
+  if (std::fabs(vx * d.y - vy * d.x) > e.H[0] * ay + e.H[1] * ax) return true;
+  return false;
+}
+
+// True when the ray (O, D) certainly misses every emitter: the box lies
+// behind the origin along an axis the ray moves away on, or its line misses.
+bool emit_culled(const EmitCull& e, V3 O, V3 D) {
+  if (D.x == 0.0f || D.y == 0.0f || D.z == 0.0f) return false;
   const float vx = e.c[0] - O.x, vy = e.c[1] - O.y, vz = e.c[2] - O.z;
   if (D.x > 0.0f ? vx < -e.H[0] : vx > e.H[0]) return true;
   if (D.y > 0.0f ? vy < -e.H[1] : vy > e.H[1]) return true;
   if (D.z > 0.0f ? vz < -e.H[2] : vz > e.H[2]) return true;
-  if (std::fabs(vy * D.z - vz * D.y) > e.H[1] * az + e.H[2] * ay) return true;
-  if (std::fabs(vz * D.x - vx * D.z) > e.H[2] * ax + e.H[0] * az) return true;
-  if (std::fabs(vx * D.y - vy * D.x) > e.H[0] * ay + e.H[1] * ax) return true;
-  return false;
+  return line_culled(e, O, D);
 }
 
 // The occlusion query of NEE.  Returns the material of the closest hit when
@@ -746,12 +762,18 @@ struct Tracer {
     V3 dir;
     int tries = 0;
     const bool found = nee_loop(
-        st, [&]() { return hemisphere(st.normal, seed); },
-        [&](V3 d, bool hit_before) {
+        st, [&]() { return hemisphere_raw(seed); },
+        [&](V3& d, bool hit_before) {
           cnt.c[1]++;
           rec.occl++;
-          int32_t code;
-          const int32_t m = occlusion_query(s, st.origin, d, hit_before, code, cnt);
+          int32_t code = -5;
+          int32_t m = kNonEmitter;
+          // with the flag false, a line that misses every emitter is resolved
+          // on the raw draws (a non-emitter and a miss have the same effect)
+          if (hit_before || !line_culled(s.ecull, st.origin, d)) {
+            d = hemisphere_dir(d, st.normal);
+            m = occlusion_query(s, st.origin, d, hit_before, code, cnt);
+          }
           rec.ray(code, true);
           return m;
         },
@@ -957,7 +979,7 @@ int32_t orc_nee_scripted(const int32_t* outcomes, int32_t n, int32_t hit_in, int
   V3 dir;
   const bool found = nee_loop(
       st, [&]() { return mk((float)i, 0.0f, 0.0f); },
-      [&](V3, bool) {
+      [&](V3&, bool) {
         const int32_t m = i < n ? outcomes[i] : 0;
         ++i;
         return m;
@@ -1000,6 +1022,10 @@ void orc_scene_update_vertices(orc_scene* o, const float* verts, const float* no
 int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
 
 int32_t orc_num_emitters(const orc_scene* s) { return s->s.emit.T; }
+
+void orc_scene_set_emitter_cull(orc_scene* o, int32_t enabled) {
+  o->s.ecull = enabled ? emit_cull(o->s) : EmitCull();
+}
 
 int orc_get_bvh(const orc_scene* o, int32_t which, uint32_t* morton_sorted, uint32_t* prim_sorted,
                 uint32_t* child_refs, float* child_boxes) {

@@ -83,6 +83,7 @@ def lib():
         L.orc_get_bvh.argtypes = [C.c_void_p, C.c_int32, u32p, u32p, u32p, f32p]
         L.orc_num_emitters.restype = C.c_int32
         L.orc_num_emitters.argtypes = [C.c_void_p]
+        L.orc_scene_set_emitter_cull.argtypes = [C.c_void_p, C.c_int32]
         L.orc_occlusion.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, i32p, i32p, i32p, u64p]
         L.orc_closest_hit.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, C.c_float, C.c_float,
                                       i32p, f32p, u64p, C.c_int]
@@ -256,6 +257,10 @@ class Scene:
         if n is not None:
             self.normals = n
         lib().orc_scene_update_vertices(self.h, _p(self.verts, C.c_float), _p(n, C.c_float))
+
+    def set_emitter_cull(self, enabled):
+        """Switch the emitter-query culls off or on (results unchanged)."""
+        lib().orc_scene_set_emitter_cull(self.h, 1 if enabled else 0)
 
     def bvh(self, which=0):
         T = self.T if which == 0 else lib().orc_num_emitters(self.h)

@@ -114,3 +114,37 @@ def test_tir_zero_direction_ends_sample(oracle_mod):
     idx = np.nonzero(rh == -2)
     for p, s, b in zip(*idx):
         assert b == nb[p, s] - 1  # the zero-direction ray is the sample's last
+
+
+def _scene_arrays(sc):
+    return sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"]
+
+
+def test_emitter_culls_change_no_result(oracle_mod):
+    """The culls in front of NEE's emitter query (on the raw hemisphere draws
+    with the flag false, and on the ray for the query itself) only skip
+    work: with them switched off the same pixels give the same samples,
+    hit ids, occlusion hashes and ray counts, bit for bit, and more node
+    visits.  Scenes: the Cornell box (closed, ceiling quad), a small island
+    (open, sun quad, dielectric water) and a batch scene whose materials
+    include random emitters."""
+    from lisa_amd import scenes
+
+    cases = [cornell(32, 16, 64, 64, 4, 7), scenes.island(n=96, width=64, height=48),
+             scenes.batch_scene(5)]
+    for sc in cases:
+        W, H = 64, 48
+        s = oracle_mod.Scene(*_scene_arrays(sc))
+        cam = sc["camera"]
+        U, V, Wv = oracle_mod.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+        px = np.arange(0, W * H, 3, dtype=np.int32)
+        args = (W, H, 4, 7, cam["eye"], U, V, Wv)
+        on = s.render(*args, pixels=px, record=True, threads=4)
+        s.set_emitter_cull(False)
+        off = s.render(*args, pixels=px, record=True, threads=4)
+        for key in ("sample_rgb", "sample_bounces", "sample_occl", "sample_hash", "rad_hits",
+                    "accum"):
+            assert np.array_equal(on[key].view(np.uint8), off[key].view(np.uint8)), (sc["name"], key)
+        c_on, c_off = on["counters"], off["counters"]
+        assert np.array_equal(c_on[[0, 1, 4, 5, 6]], c_off[[0, 1, 4, 5, 6]]), sc["name"]
+        assert c_on[2] < c_off[2], sc["name"]
