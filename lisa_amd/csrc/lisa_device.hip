@@ -247,8 +247,10 @@ __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, flo
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
     const uint32_t r0 = __float_as_uint(d.x), r1 = __float_as_uint(d.y);
     float tn0 = 0.0f, tn1 = 0.0f;
-    bool h0 = (r0 != kEmpty) && slab(a.x, a.y, a.z, a.w, b.x, b.y, r, tmin, cull_t, tn0);
-    bool h1 = (r1 != kEmpty) && slab(b.z, b.w, c.x, c.y, c.z, c.w, r, tmin, cull_t, tn1);
+    // both slab tests unconditionally (no short circuit), so the node's four
+    // 16-byte loads are issued together and waited for once
+    bool h0 = (r0 != kEmpty) & slab(a.x, a.y, a.z, a.w, b.x, b.y, r, tmin, cull_t, tn0);
+    bool h1 = (r1 != kEmpty) & slab(b.z, b.w, c.x, c.y, c.z, c.w, r, tmin, cull_t, tn1);
 #pragma unroll
     for (int ch = 0; ch < 2; ++ch) {
       const bool hit = ch == 0 ? h0 : h1;
@@ -778,7 +780,7 @@ __global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(Scene
 // Path loop in a cache mode (1: cached render of the full image in tiles,
 // 2: training paths on a pixel list).
 template <int kMode>
-__global__ __launch_bounds__(kPathBlock) void k_path_mode(SceneView sc, PathArgs p) {
+__global__ __launch_bounds__(kPathBlock, 5) void k_path_mode(SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   OnePixel src{sc, p, false};
   path_lane<false, false, OnePixel, kMode>(src, stk + threadIdx.x);
