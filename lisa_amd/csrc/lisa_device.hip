@@ -231,7 +231,7 @@ template <bool kCount, int kStride>
 __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, float tmin,
                                            float tlim, uint32_t lim_prim, bool any, float& t_out,
                                            uint32_t* stk, TravCount& cnt) {
-  uint32_t spill[kStackMax - kStackLds];
+  volatile uint32_t spill[kStackMax - kStackLds];
   float best_t = tlim;
   float cull_t = tlim * kWiden;
   uint32_t best_prim = lim_prim;
@@ -296,7 +296,13 @@ __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, flo
     } else {
       if (sp == 0) break;
       --sp;
-      node = sp < kStackLds ? stk[sp * kStride] : spill[sp - kStackLds];
+      // separate reads: the spill array is volatile, so the compiler neither
+      // keeps it in registers nor merges the two reads into one generic
+      // (flat) load through a selected pointer
+      if (sp < kStackLds)
+        node = stk[sp * kStride];
+      else
+        node = spill[sp - kStackLds];
     }
   }
   t_out = best_t;
