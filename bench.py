@@ -212,9 +212,12 @@ def bench_rmse(a):
     subframes, time at the first subframe that reaches it.  Cache: frames of
     lisa_render_cached (online training + `--cache-spp` samples of paths cut
     after `--prefix` radiance rays), time summed over frames up to the first
-    frame count 2^k that reaches it.  Times are wall time around the
-    synchronous render calls; reading the image back for the RMSE is not
-    timed."""
+    frame that reaches it (checked after every frame, like the progressive
+    plain render).  `plain_time_fit_s` is the sweep's time scaled to the spp
+    that reaches the target exactly, taking RMSE proportional to
+    1/sqrt(spp) from the last sweep point above the target.  Times are wall
+    time around the synchronous render calls; reading the image back for the
+    RMSE is not timed."""
     import numpy as np
     from lisa_amd import capi, scenes
 
@@ -286,17 +289,20 @@ def bench_rmse(a):
         last = capi.render_cached(ctx, cache, p, cp)
         tot += time.perf_counter() - t
         frames = f + 1
-        if frames & (frames - 1) == 0 or frames % 16 == 0:
-            e = rmse(ctx.copy_accum(W * H))
-            if e <= RMSE_TARGET:
-                cache_t = tot
-                break
+        if rmse(ctx.copy_accum(W * H)) <= RMSE_TARGET:
+            cache_t = tot
+            break
         if tot > a.rmse_limit_s:
             break
     e_cache = rmse(ctx.copy_accum(W * H))
     print(f"[bench] cache: {cache_t} s after {frames} frames, rmse {e_cache:.5f}, last {last}",
           file=sys.stderr, flush=True)
     mlp = mlp_roofline(last, a.train_steps)
+    fit_t = None
+    above = [r for r in sweep if r["rmse"] > RMSE_TARGET]
+    if plain_t is not None and above:
+        r = above[-1]
+        fit_t = r["s"] * (r["rmse"] / RMSE_TARGET) ** 2
     out = {"metric": "time to RMSE<=1e-2 vs 16k-spp plain reference (s)",
            "value": None if cache_t is None else round(cache_t, 4), "unit": "s", "n_gpus": 1,
            "steps": frames, "warmup": 0,
@@ -310,6 +316,9 @@ def bench_rmse(a):
                       "persistent_frames": a.cache_persistent, "parallelism": "single GPU"},
            "plain_time_s": None if plain_t is None else round(plain_t, 4),
            "plain_progressive_time_s": None if prog_t is None else round(prog_t, 4),
+           "plain_time_fit_s": None if fit_t is None else round(fit_t, 4),
+           "speedup_vs_plain_fit": (None if (fit_t is None or cache_t is None)
+                                    else round(fit_t / cache_t, 3)),
            "speedup_vs_plain": (None if (plain_t is None or cache_t is None)
                                 else round(plain_t / cache_t, 3)),
            "speedup_vs_plain_progressive": (None if (prog_t is None or cache_t is None)
