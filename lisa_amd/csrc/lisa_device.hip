@@ -46,13 +46,18 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 // a private spill array.  The depth of a Karras tree over index-augmented
 // 30-bit keys is below 64, so kStackMax entries always suffice.
 #ifndef LISA_STACK_LDS
-#define LISA_STACK_LDS 32
+#define LISA_STACK_LDS 24
 #endif
 #ifndef LISA_PATH_BLOCK
 #define LISA_PATH_BLOCK 64
 #endif
+// Waves per SIMD the plain path kernel k_path is compiled for: 6 with a
+// 24-entry LDS stack (6 KB per wave) at 80 VGPRs and a few spills in cold
+// code.  The cache-mode paths run at 5 (their extra state spills more at
+// 6), the counting and recording builds at 4, and the batched forms keep
+// the registers they need.
 #ifndef LISA_PATH_MIN_BLOCKS
-#define LISA_PATH_MIN_BLOCKS 1
+#define LISA_PATH_MIN_BLOCKS 6
 #endif
 // The BSDF of opaque materials, chosen at compile time as the reference
 // chooses it with an #include (shader.cu:4): a type with
@@ -381,6 +386,20 @@ struct PathArgs {
   F3 bmin, binv;      // position normalisation for the features
 };
 
+// One slot of a device counter for every active lane, with one atomic per
+// wave: the lowest active lane reserves the whole group, each lane takes its
+// rank in it.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter) {
+  const uint64_t act = __ballot(1);
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+  const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(act));
+  base = __shfl(base, (int)leader);
+  return base + rank;
+}
+
 // Cache features of a surface hit (lisa.h, lisa_cache_*): position in the
 // scene box, incoming direction and shading normal as (theta/pi, phi/2pi),
 // albedo, roughness, dielectric flag -- each in [0, 1].
@@ -675,7 +694,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           }
           bool cached = false;
           if (kMode == 1 && b == pp->cache_at) {
-            const uint32_t slot = atomicAdd(pp->q_count, 1u);
+            const uint32_t slot = wave_reserve(pp->q_count);
             if (slot < pp->q_cap) {
               cache_features(*pp, O, Din, N, m, pp->q_feat + 12 * (size_t)slot);
               pp->q_att[3 * (size_t)slot] = att.x;
@@ -777,7 +796,7 @@ struct OnePixel {
 };
 
 template <bool kRecord, bool kCount, bool kF64 = false>
-__global__ __launch_bounds__(kPathBlock, LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
+__global__ __launch_bounds__(kPathBlock, (kRecord || kCount) ? 4 : LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   OnePixel src{sc, p, false};
   path_lane<kRecord, kCount, OnePixel, 0, kF64>(src, stk + threadIdx.x);
