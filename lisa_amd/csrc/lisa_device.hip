@@ -59,6 +59,9 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #ifndef LISA_PATH_MIN_BLOCKS
 #define LISA_PATH_MIN_BLOCKS 6
 #endif
+#ifndef LISA_TILE_WAVES
+#define LISA_TILE_WAVES 5  // the batched wave-tile form
+#endif
 // The BSDF of opaque materials, chosen at compile time as the reference
 // chooses it with an #include (shader.cu:4): a type with
 // sample(wi, N, seed&, roughness) -> wo and eval(N, L) -> the NEE weight
@@ -970,25 +973,25 @@ __global__ __launch_bounds__(kPathBlock) void k_path_queue(const BatchScene* sce
 // tile's scene is the same for the whole wave, so its pointers and launch
 // constants are scalar; lanes whose pixel is done wait for the tile.
 struct TileSrc {
-  SceneView sc;
-  PathArgs p;
+  const BatchScene* bs;  // wave-uniform: its fields are read through scalar loads
   uint32_t tile;
   bool given;
-  __device__ const SceneView* view() const { return &sc; }
-  __device__ const PathArgs* args() const { return &p; }
+  __device__ const SceneView* view() const { return &bs->sc; }
+  __device__ const PathArgs* args() const { return &bs->p; }
   __device__ bool next(Job& j) {
     if (given) return false;
     given = true;
     const uint32_t w = threadIdx.x & 63u;
-    j.x = (tile % p.tiles_x) * 8 + (w & 7u);
-    j.y = (tile / p.tiles_x) * 8 + (w >> 3);
-    if (j.x >= p.width || j.y >= p.height) return false;
-    j.item = (int32_t)(j.y * p.width + j.x);
+    const uint32_t tx = bs->p.tiles_x;
+    j.x = (tile % tx) * 8 + (w & 7u);
+    j.y = (tile / tx) * 8 + (w >> 3);
+    if (j.x >= bs->p.width || j.y >= bs->p.height) return false;
+    j.item = (int32_t)(j.y * bs->p.width + j.x);
     return true;
   }
 };
 
-__global__ __launch_bounds__(kPathBlock) void k_path_tiles(const BatchScene* scenes, int32_t n,
+__global__ __launch_bounds__(kPathBlock, LISA_TILE_WAVES) void k_path_tiles(const BatchScene* scenes, int32_t n,
                                                            uint32_t* counter, uint32_t total) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   const uint32_t n_tiles = total / 64u;
@@ -1006,7 +1009,7 @@ __global__ __launch_bounds__(kPathBlock) void k_path_tiles(const BatchScene* sce
         hi = mid - 1;
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
-    TileSrc src{scenes[lo].sc, scenes[lo].p, t - scenes[lo].first / 64u, false};
+    TileSrc src{scenes + lo, t - scenes[lo].first / 64u, false};
     path_lane<false, false, TileSrc>(src, stk + threadIdx.x);
   }
 }
