@@ -406,11 +406,11 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter) {
 // Cache features of a surface hit (lisa.h, lisa_cache_*): position in the
 // scene box, incoming direction and shading normal as (theta/pi, phi/2pi),
 // albedo, roughness, dielectric flag -- each in [0, 1].
-__device__ __forceinline__ void cache_features(const PathArgs& p, F3 P, F3 D, F3 N, const DevMat& m,
+__device__ __forceinline__ void cache_features(F3 bmin, F3 binv, F3 P, F3 D, F3 N, const DevMat& m,
                                                float* f) {
-  f[0] = fminf(fmaxf((P.x - p.bmin.x) * p.binv.x, 0.0f), 1.0f);
-  f[1] = fminf(fmaxf((P.y - p.bmin.y) * p.binv.y, 0.0f), 1.0f);
-  f[2] = fminf(fmaxf((P.z - p.bmin.z) * p.binv.z, 0.0f), 1.0f);
+  f[0] = fminf(fmaxf((P.x - bmin.x) * binv.x, 0.0f), 1.0f);
+  f[1] = fminf(fmaxf((P.y - bmin.y) * binv.y, 0.0f), 1.0f);
+  f[2] = fminf(fmaxf((P.z - bmin.z) * binv.z, 0.0f), 1.0f);
   const float dl = sqrtf(dot(D, D));
   const float dy = dl > 0.0f ? D.y / dl : 0.0f;
   f[3] = acosf(fminf(fmaxf(dy, -1.0f), 1.0f)) * 0.31830988618f;
@@ -699,7 +699,18 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           if (kMode == 1 && b == pp->cache_at) {
             const uint32_t slot = wave_reserve(pp->q_count);
             if (slot < pp->q_cap) {
-              cache_features(*pp, O, Din, N, m, pp->q_feat + 12 * (size_t)slot);
+              // the hit itself; k_cache_features turns it into the features
+              float* r = pp->q_feat + 12 * (size_t)slot;
+              r[0] = O.x;
+              r[1] = O.y;
+              r[2] = O.z;
+              r[3] = Din.x;
+              r[4] = Din.y;
+              r[5] = Din.z;
+              r[6] = N.x;
+              r[7] = N.y;
+              r[8] = N.z;
+              r[9] = __uint_as_float((uint32_t)mat);
               pp->q_att[3 * (size_t)slot] = att.x;
               pp->q_att[3 * (size_t)slot + 1] = att.y;
               pp->q_att[3 * (size_t)slot + 2] = att.z;
@@ -710,7 +721,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           }
           if (kMode == 2 && t_used < pp->t_slots && (pp->t_depth == 0 || b == pp->t_depth)) {
             float* r = pp->t_rec + 18 * ((size_t)item * pp->t_slots + t_used);
-            cache_features(*pp, O, Din, N, m, r);
+            cache_features(pp->bmin, pp->binv, O, Din, N, m, r);
             r[12] = att.x;
             r[13] = att.y;
             r[14] = att.z;
@@ -869,6 +880,19 @@ __global__ __launch_bounds__(kPathBlock) void k_path_pq(SceneView sc, PathArgs p
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   PixelQueue src{sc, p, counter, total, chunk, 0u, 0u};
   path_lane<false, false, PixelQueue, kMode>(src, stk + threadIdx.x);
+}
+
+// The features of the queries of a cached frame, in place: each 12-float
+// record holds the hit point, incoming direction, shading normal and the
+// material index as written by the path loop (path_lane's kMode 1).
+__global__ __launch_bounds__(256) void k_cache_features(float* __restrict__ q, uint32_t n, F3 bmin,
+                                                        F3 binv, const DevMat* __restrict__ mats) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float* r = q + 12 * (size_t)i;
+  const F3 P = mk(r[0], r[1], r[2]), D = mk(r[3], r[4], r[5]), N = mk(r[6], r[7], r[8]);
+  const DevMat m = mats[__float_as_uint(r[9])];
+  cache_features(bmin, binv, P, D, N, m, r);
 }
 
 // frame[item] += throughput * max(network output, 0) / spp for every query.
@@ -1812,6 +1836,12 @@ lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_
   }
   HIP_TRY(ctx, hipMemcpyAsync(nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
+  const uint32_t nf = std::min<uint32_t>(*nq, (uint32_t)qcap);
+  if (nf > 0) {
+    k_cache_features<<<(nf + 255) / 256, 256, 0, st>>>(ctx->c_qfeat.p, nf, a.bmin, a.binv, ctx->d_mats);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(st));  // the network reads them on the cache's stream
+  }
   return LISA_OK;
 }
 
