@@ -95,6 +95,31 @@ def test_path_subset_small_cornell(capi, oracle_mod, ctx):
     _subset_compare(g, r)
 
 
+def test_path_subset_scaled_translated_and_island(capi, oracle_mod, ctx):
+    """Per-sample parity away from the unit scene: the Cornell box scaled by 40
+    and moved to (3000, -700, 250) (the emitter-query culls' margin scales
+    with the largest coordinate, computed on the host from the BVH roots),
+    and a small island (open scene, sun quad, dielectric water)."""
+    base = cornell(32, 16, 128, 128, 8, 7)
+    shift = np.array([3000.0, -700.0, 250.0])
+    v = (np.asarray(base["verts"], np.float64).reshape(-1, 3) * 40.0 + shift).astype(np.float32)
+    sc = dict(base, verts=v.reshape(np.asarray(base["verts"]).shape),
+              camera=dict(base["camera"],
+                          eye=tuple(np.asarray(base["camera"]["eye"]) * 40.0 + shift),
+                          look_at=tuple(np.asarray(base["camera"]["look_at"]) * 40.0 + shift)))
+    isl = scenes.island(n=128, width=96, height=64)
+    for sc_, W, H in ((sc, 128, 128), (isl, 96, 64)):
+        o = _pair(capi, oracle_mod, ctx, sc_)
+        cam = sc_["camera"]
+        U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+        px = scenes.seeded_pixels(W, H, 768, seed=4)
+        p = capi.render_params(W, H, 8, 7, cam["eye"], U, V, Wv)
+        g = ctx.trace_subset(p, px)
+        r = o.render(W, H, 8, 7, cam["eye"], U, V, Wv, pixels=px, record=True, threads=8)
+        assert int(r["counters"][4]) > 1000  # surface hits, so NEE runs
+        _subset_compare(g, r)
+
+
 def test_path_subset_65k_cornell(capi, oracle_mod, ctx):
     sc = cornell(256, 128, 1024, 1024, 256, 7)
     o = _pair(capi, oracle_mod, ctx, sc)
