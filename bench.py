@@ -37,7 +37,7 @@ S_NODE, S_TRI, S_SHADE, S_PIXEL = 64, 48, 96, 16
 # name -> (scene factory, default spp, default CPU-baseline pixel count)
 WORKLOADS = {
     "cornell": ("cornell_small", 16, 65536),
-    "cornell65k": ("cornell_65k", 256, 65536),
+    "cornell65k": ("cornell_65k", 256, 131072),
     "icosphere": ("displaced_icosphere", 64, 2048),
     "batch64": ("batch_scene", 32, 0),
     "island": ("island", 0, 0),
