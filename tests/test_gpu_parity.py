@@ -120,6 +120,22 @@ def test_path_subset_scaled_translated_and_island(capi, oracle_mod, ctx):
         _subset_compare(g, r)
 
 
+def test_path_subset_displaced_icosphere(capi, oracle_mod, ctx):
+    """The traversal-bound scene's kind at a smaller level: a displaced
+    dielectric icosphere (level 6, 81,920 triangles) in the Cornell walls,
+    12 bounces, so most paths refract through the glass many times."""
+    sc = scenes.displaced_icosphere(level=6, width=96, height=64, spp=4, bounces=12)
+    o = _pair(capi, oracle_mod, ctx, sc)
+    cam = sc["camera"]
+    U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], 96 / 64)
+    px = scenes.seeded_pixels(96, 64, 1024, seed=6)
+    p = capi.render_params(96, 64, 4, 12, cam["eye"], U, V, Wv)
+    g = ctx.trace_subset(p, px)
+    r = o.render(96, 64, 4, 12, cam["eye"], U, V, Wv, pixels=px, record=True, threads=8)
+    assert int(r["counters"][0]) > 4 * 1024 * 3  # several radiance rays per sample
+    _subset_compare(g, r)
+
+
 def test_path_subset_65k_cornell(capi, oracle_mod, ctx):
     sc = cornell(256, 128, 1024, 1024, 256, 7)
     o = _pair(capi, oracle_mod, ctx, sc)
