@@ -1023,6 +1023,11 @@ int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
 
 int32_t orc_num_emitters(const orc_scene* s) { return s->s.emit.T; }
 
+void orc_raw_cull(const orc_scene* o, const float* O, const float* d, int32_t n, int32_t* culled) {
+  for (int32_t i = 0; i < n; ++i)
+    culled[i] = line_culled(o->s.ecull, ld3(O + 3 * (size_t)i), ld3(d + 3 * (size_t)i)) ? 1 : 0;
+}
+
 void orc_scene_set_emitter_cull(orc_scene* o, int32_t enabled) {
   o->s.ecull = enabled ? emit_cull(o->s) : EmitCull();
 }

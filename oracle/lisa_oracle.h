@@ -96,6 +96,9 @@ int32_t orc_num_emitters(const orc_scene* s);
 // Switch the emitter-query culls off (enabled = 0) or back on: the culls
 // change no result, only the node-visit counters (a test of that).
 void orc_scene_set_emitter_cull(orc_scene* s, int32_t enabled);
+// The NEE loop's cull of a try on its raw draws (line_culled):
+// culled[i] = 1 when the line through O[i] along d[i] certainly misses every emitter.
+void orc_raw_cull(const orc_scene* s, const float* O, const float* d, int32_t n, int32_t* culled);
 /* which = 0: the BVH over all triangles, 1: over the emitting triangles (N of them).
  * morton_sorted[N], prim_sorted[N] (global triangle ids), child refs [2*max(N-1,1)],
  * child boxes [12*max(N-1,1)] as (c0.lo.xyz, c0.hi.xyz, c1.lo.xyz, c1.hi.xyz) */

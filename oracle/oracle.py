@@ -84,6 +84,7 @@ def lib():
         L.orc_num_emitters.restype = C.c_int32
         L.orc_num_emitters.argtypes = [C.c_void_p]
         L.orc_scene_set_emitter_cull.argtypes = [C.c_void_p, C.c_int32]
+        L.orc_raw_cull.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, i32p]
         L.orc_occlusion.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, i32p, i32p, i32p, u64p]
         L.orc_closest_hit.argtypes = [C.c_void_p, f32p, f32p, C.c_int32, C.c_float, C.c_float,
                                       i32p, f32p, u64p, C.c_int]
@@ -257,6 +258,14 @@ class Scene:
         if n is not None:
             self.normals = n
         lib().orc_scene_update_vertices(self.h, _p(self.verts, C.c_float), _p(n, C.c_float))
+
+    def raw_cull(self, O, d):
+        """The NEE loop's cull of raw hemisphere draws: 1 = the line certainly
+        misses every emitter."""
+        O, d = f32(O).reshape(-1, 3), f32(d).reshape(-1, 3)
+        out = np.zeros(O.shape[0], np.int32)
+        lib().orc_raw_cull(self.h, _p(O, C.c_float), _p(d, C.c_float), O.shape[0], _p(out, C.c_int32))
+        return out
 
     def set_emitter_cull(self, enabled):
         """Switch the emitter-query culls off or on (results unchanged)."""

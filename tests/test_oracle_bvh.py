@@ -251,11 +251,22 @@ def test_emitter_cull_is_conservative(oracle_mod, scale, shift):
         fast, ref, _ = o.occlusion(O32, D, np.full(n, hb))
         assert np.array_equal(fast, ref)
         assert (fast >= 0).sum() > 5000
+    # the raw-draw culls (any length, either sign) never drop a line that the
+    # full query answers with an emitter in one of its two directions
+    scale_d = (10.0 ** rs.uniform(-0.3, 0.3, n))[:, None] * rs.choice([-1.0, 1.0], n)[:, None]
+    raw = (D * scale_d).astype(np.float32)
+    cull = o.raw_cull(O32, raw)
+    _, fwd, _ = o.occlusion(O32, D, np.zeros(n, np.int32))
+    _, bwd, _ = o.occlusion(O32, -D, np.zeros(n, np.int32))
+    assert not np.any(cull.astype(bool) & ((fwd >= 0) | (bwd >= 0)))
     # random directions: culled unless they pass near the light
     R = rs.normal(size=(n, 3)).astype(np.float32)
     fast, ref, cnt = o.occlusion(O32, R, np.zeros(n, np.int32))
     assert np.array_equal(fast, ref)
     assert int(cnt[2]) < 0.25 * n
+    _, bwd, _ = o.occlusion(O32, -R, np.zeros(n, np.int32))
+    cull = o.raw_cull(O32, R).astype(bool)
+    assert cull.sum() > 0.5 * n and not np.any(cull & ((fast >= 0) | (bwd >= 0)))
 
 
 def test_refit_update_keeps_topology(oracle_mod):
