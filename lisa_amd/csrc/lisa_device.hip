@@ -59,6 +59,9 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #ifndef LISA_PATH_MIN_BLOCKS
 #define LISA_PATH_MIN_BLOCKS 6
 #endif
+#ifndef LISA_MODE_WAVES
+#define LISA_MODE_WAVES 5  // the cache-mode path kernels
+#endif
 #ifndef LISA_TILE_WAVES
 #define LISA_TILE_WAVES 5  // the batched wave-tile form
 #endif
@@ -819,7 +822,7 @@ __global__ __launch_bounds__(kPathBlock, (kRecord || kCount) ? 4 : LISA_PATH_MIN
 // Path loop in a cache mode (1: cached render of the full image in tiles,
 // 2: training paths on a pixel list).
 template <int kMode>
-__global__ __launch_bounds__(kPathBlock, 5) void k_path_mode(SceneView sc, PathArgs p) {
+__global__ __launch_bounds__(kPathBlock, LISA_MODE_WAVES) void k_path_mode(SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   OnePixel src{sc, p, false};
   path_lane<false, false, OnePixel, kMode>(src, stk + threadIdx.x);
