@@ -54,7 +54,13 @@ __device__ __forceinline__ float rnd(uint32_t& s) {
   s = 1664525u * s + 1013904223u;
   return (float)(s & 0x00FFFFFFu) / (float)0x01000000;
 }
-__device__ __forceinline__ float rng(uint32_t& s) { return rnd(s) * 2.0f - 1.0f; }
+// rnd(s) * 2 - 1 (maths.cu:6-8) as one fused multiply-add: k * 2^-24 * 2 - 1
+// is exact for every 24-bit k (a multiple of 2^-23 in [-1, 1)), so the fused
+// form gives the same bits with two instructions fewer.
+__device__ __forceinline__ float rng(uint32_t& s) {
+  s = 1664525u * s + 1013904223u;
+  return __builtin_fmaf((float)(s & 0x00FFFFFFu), 0x1p-23f, -1.0f);
+}
 
 // ---- maths.cu ----
 // shoot_ray_hemisphere in two steps: the three draws (x, y, z), then the
