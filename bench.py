@@ -59,7 +59,7 @@ def parse():
                          "traversal-bound 5.2M-triangle scene; batch64: the batched "
                          "wavefront form; island / cornell_cache: time to RMSE <= 1e-2, "
                          "learned cache vs plain")
-    ap.add_argument("--cache-spp", type=int, default=16, help="spp per cached frame")
+    ap.add_argument("--cache-spp", type=int, default=8, help="spp per cached frame")
     ap.add_argument("--prefix", type=int, default=2, help="radiance rays before the cache")
     ap.add_argument("--train-paths", type=int, default=262144,
                     help="training paths per frame (their rows at the query depth: ~2^16 on "
@@ -71,6 +71,10 @@ def parse():
                     help="time the persistent launch (LISA_FLAG_PERSISTENT) of the plain path")
     ap.add_argument("--cache-persistent", action="store_true",
                     help="cached frames' paths in one persistent launch (LISA_CACHE_PERSISTENT)")
+    ap.add_argument("--cache-emission", action=argparse.BooleanOptionalAction, default=True,
+                    help="prefix bounces' emission in expectation (LISA_CACHE_EMISSION)")
+    ap.add_argument("--compare-emission", action="store_true",
+                    help="island / cornell_cache: also time the cache with --cache-emission flipped")
     ap.add_argument("--cache-frames", type=int, default=0,
                     help="island / cornell_cache: only run this many cached frames (no "
                          "reference, no plain sweep) and report their timings, for profiling")
@@ -211,7 +215,9 @@ def bench_rmse(a):
     is the first render that reaches the target.  Plain progressive: 16-spp
     subframes, time at the first subframe that reaches it.  Cache: frames of
     lisa_render_cached (online training + `--cache-spp` samples of paths cut
-    after `--prefix` radiance rays), time summed over frames up to the first
+    after `--prefix` radiance rays, whose diffuse bounces' emission is taken
+    in expectation unless `--no-cache-emission`; `--compare-emission` also
+    times the other form), time summed over frames up to the first
     frame that reaches it (checked after every frame, like the progressive
     plain render).  `plain_time_fit_s` is the sweep's time scaled to the spp
     that reaches the target exactly, taking RMSE proportional to
@@ -278,25 +284,36 @@ def bench_rmse(a):
             break
     print(f"[bench] plain progressive: {prog_t}", file=sys.stderr, flush=True)
     # cache
-    cache = capi.Cache(0, seed=1, lr=1e-3)
-    cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
-                                 train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
-                                 seed=3, flags=capi.CACHE_PERSISTENT if a.cache_persistent else 0)
-    cache_t, tot, frames, last = None, 0.0, 0, None
-    for f in range(1 << 20):
-        p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
-        t = time.perf_counter()
-        last = capi.render_cached(ctx, cache, p, cp)
-        tot += time.perf_counter() - t
-        frames = f + 1
-        if rmse(ctx.copy_accum(W * H)) <= RMSE_TARGET:
-            cache_t = tot
-            break
-        if tot > a.rmse_limit_s:
-            break
-    e_cache = rmse(ctx.copy_accum(W * H))
-    print(f"[bench] cache: {cache_t} s after {frames} frames, rmse {e_cache:.5f}, last {last}",
-          file=sys.stderr, flush=True)
+    def run_cache(flags):
+        cache = capi.Cache(0, seed=1, lr=1e-3)
+        cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
+                                     train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
+                                     seed=3, flags=flags)
+        cache_t, tot, frames, last = None, 0.0, 0, None
+        for f in range(1 << 20):
+            p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
+            t = time.perf_counter()
+            last = capi.render_cached(ctx, cache, p, cp)
+            tot += time.perf_counter() - t
+            frames = f + 1
+            if rmse(ctx.copy_accum(W * H)) <= RMSE_TARGET:
+                cache_t = tot
+                break
+            if tot > a.rmse_limit_s:
+                break
+        e_cache = rmse(ctx.copy_accum(W * H))
+        cache.close()
+        print(f"[bench] cache (flags {flags}): {cache_t} s after {frames} frames, "
+              f"rmse {e_cache:.5f}, last {last}", file=sys.stderr, flush=True)
+        return cache_t, tot, frames, last, e_cache
+
+    other = None
+    if a.compare_emission:
+        o_t, _, o_frames, _, o_e = run_cache(cache_flags(a, capi) ^ capi.CACHE_EMISSION)
+        other = {"cache_emission": not a.cache_emission,
+                 "time_s": None if o_t is None else round(o_t, 4), "frames": o_frames,
+                 "rmse": round(o_e, 6)}
+    cache_t, tot, frames, last, e_cache = run_cache(cache_flags(a, capi))
     mlp = mlp_roofline(last, a.train_steps)
     fit_t = None
     above = [r for r in sweep if r["rmse"] > RMSE_TARGET]
@@ -313,7 +330,8 @@ def bench_rmse(a):
                       "bounces": B, "triangles": ctx.T, "cache_prefix_bounces": a.prefix,
                       "cache_spp_per_frame": a.cache_spp, "train_paths": a.train_paths,
                       "train_rows_max": 1 << 16, "train_steps": a.train_steps, "lr": 1e-3,
-                      "persistent_frames": a.cache_persistent, "parallelism": "single GPU"},
+                      "persistent_frames": a.cache_persistent,
+                      "cache_emission": a.cache_emission, "parallelism": "single GPU"},
            "plain_time_s": None if plain_t is None else round(plain_t, 4),
            "plain_progressive_time_s": None if prog_t is None else round(prog_t, 4),
            "plain_time_fit_s": None if fit_t is None else round(fit_t, 4),
@@ -323,12 +341,12 @@ def bench_rmse(a):
                                 else round(plain_t / cache_t, 3)),
            "speedup_vs_plain_progressive": (None if (prog_t is None or cache_t is None)
                                             else round(prog_t / cache_t, 3)),
-           "cache_frames": frames, "cache_rmse": round(e_cache, 6), "plain_sweep": sweep,
+           "cache_frames": frames, "cache_rmse": round(e_cache, 6), "cache_other_form": other,
+           "plain_sweep": sweep,
            "reference": {"spp": 16384, "pixels": REF_PIXELS, "fp64": True,
                          "seconds": round(ref_s, 2)},
            "last_frame": last, "mlp_roofline": mlp, "roofline": None, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
-    cache.close()
     ctx.close()
 
 
@@ -354,6 +372,11 @@ def mlp_roofline(st, train_steps):
     return out
 
 
+def cache_flags(a, capi):
+    return ((capi.CACHE_PERSISTENT if a.cache_persistent else 0)
+            | (capi.CACHE_EMISSION if a.cache_emission else 0))
+
+
 def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
     """`--cache-frames N`: N cached frames after `--warmup` untimed ones; per-frame
     wall time and the MLP's MFMA roofline (kernel times from HIP events)."""
@@ -361,7 +384,7 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
     cache = capi.Cache(0, seed=1, lr=1e-3)
     cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                  train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
-                                 seed=3, flags=capi.CACHE_PERSISTENT if a.cache_persistent else 0)
+                                 seed=3, flags=cache_flags(a, capi))
     stats = []
     for f in range(a.warmup + a.cache_frames):
         p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)

@@ -60,6 +60,7 @@ typedef struct lisa_render_params {
 #define LISA_MODE_PLAIN 0        /* one work-item per pixel megakernel (the drop-in) */
 #define LISA_FLAG_FP64_ACCUM 1u  /* per-pixel FP64 accumulation (reference renders >= 16k spp) */
 #define LISA_FLAG_COUNT 2u       /* also count rays / node visits / triangle tests */
+#define LISA_FLAG_CACHE_EMISSION 8u  /* lisa_cache_trace_queries only: the LISA_CACHE_EMISSION prefix */
 #define LISA_FLAG_PERSISTENT 4u  /* one persistent launch whose lanes pull pixels from a queue
                                     (same image; not with COUNT or FP64_ACCUM) */
 
@@ -211,6 +212,11 @@ typedef struct lisa_cache_frame_params {
   uint64_t seed;           /* draws the training pixels */
 } lisa_cache_frame_params;
 #define LISA_CACHE_PERSISTENT 1u
+/* The prefix paths add the expected emission of each Lambertian (roughness 1)
+ * bounce from one light sample and draw the bounce on the condition that it
+ * hits no emitter, weighted by 1 - (the sampled chance that it would): the
+ * same expectation as the plain tracer's bounces, with less variance. */
+#define LISA_CACHE_EMISSION 2u
 
 typedef struct lisa_cache_frame_stats {
   float trace_train_ms, train_ms, render_ms, infer_ms, total_ms;  /* host wall time */
@@ -231,7 +237,8 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
  * dielectric flag.
  *
  * lisa_cache_trace_queries: p->spp samples per listed pixel (seeded by
- * p->subframe), at most prefix_bounces + 1 radiance rays; the surface hit of
+ * p->subframe; LISA_FLAG_CACHE_EMISSION in p->flags selects the prefix form
+ * of LISA_CACHE_EMISSION), at most prefix_bounces + 1 radiance rays; the surface hit of
  * radiance ray prefix_bounces + 1 ends the path in a query (features,
  * throughput, index into `pixels`).  accum_out (4n) = per-pixel mean of the
  * radiance gathered before the queries.  Queries come in no fixed order;

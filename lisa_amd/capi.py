@@ -29,6 +29,7 @@ MODE_PLAIN = 0
 FLAG_FP64_ACCUM = 1
 FLAG_COUNT = 2
 FLAG_PERSISTENT = 4
+FLAG_CACHE_EMISSION = 8  # lisa_cache_trace_queries: the CACHE_EMISSION prefix
 NCOUNTERS = 8
 COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
                  "radiance_hits", "samples", "invalid_dir_rays", "pixels")
@@ -428,6 +429,7 @@ CACHE_INPUTS = 12
 
 
 CACHE_PERSISTENT = 1
+CACHE_EMISSION = 2  # prefix bounces' emission in expectation (same mean, less variance)
 
 
 class CacheFrameParams(C.Structure):

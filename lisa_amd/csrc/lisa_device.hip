@@ -40,6 +40,7 @@ constexpr float kRobust = 0x1.000006p+0f;
 // is still tested and equal-t hits resolve to the lower triangle id.
 constexpr float kWiden = 0x1.0001p+0f;
 constexpr float kTmin = 1e-4f;  // shader.cu:114, :202
+constexpr int kMaxRedraws = 64;  // conditioned bounces drawn again at most this often
 constexpr float kTmax = 1e16f;  // shader.cu:115, :202
 constexpr int kNeeTries = 30;   // shader.cu:199
 // Per-lane traversal stack: the first kStackLds entries in LDS, the rest in
@@ -61,6 +62,9 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #endif
 #ifndef LISA_MODE_WAVES
 #define LISA_MODE_WAVES 5  // the cache-mode path kernels
+#endif
+#ifndef LISA_EMIT_WAVES
+#define LISA_EMIT_WAVES 5  // the query kernel with the prefix-emission light samples
 #endif
 #ifndef LISA_TILE_WAVES
 #define LISA_TILE_WAVES 5  // the batched wave-tile form
@@ -389,6 +393,9 @@ struct PathArgs {
   uint32_t* t_count;  // kMode 2: rows appended (may pass t_cap; rows past it are dropped)
   uint32_t t_cap;
   uint32_t t_depth;   // kMode 2: > 0 records only the hits of radiance ray number t_depth
+  int32_t emit_expect;      // queries: path_lane's kMode 3 instead of kMode 1
+  const float* emit_cdf;    // area prefix sums of the emitting triangles (emitter tree order)
+  float emit_area;          // their total area
   F3 bmin, binv;      // position normalisation for the features
 };
 
@@ -442,6 +449,9 @@ struct Job {
 // kMode 0: the plain path loop.  kMode 1: the radiance ray numbered
 // p.cache_at ends at a surface hit by queuing a cache query (features, the
 // path throughput, the pixel); misses and emitters are handled exactly.
+// kMode 3: kMode 1 with the emission that the prefix's diffuse bounces would
+// hit taken in expectation (a light sample at each Lambertian vertex, the
+// bounce drawn on the condition that it hits no emitter).
 // kMode 2: every surface hit of the (full-length) path (or, with t_depth > 0,
 // the hit of radiance ray number t_depth only) records its features,
 // throughput and the radiance gathered so far; at the end of the sample the
@@ -481,7 +491,15 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   int32_t occl_cnt = 0;
   size_t rec_base = (size_t)item * pp->spp;
 
+  constexpr bool kQuery = kMode == 1 || kMode == 3;  // ends at the cache query
+  constexpr bool kEmit = kMode == 3;                 // prefix emission in expectation
   uint32_t t_used = 0;  // kMode 2: vertex slots recorded in this sample
+  // kMode 3: the current radiance ray is a bounce drawn on
+  // the condition that it hits no emitter (redrawn when it does); lw is the
+  // pending light sample's weight
+  bool cond = false;
+  int32_t rejects = 0, ls_k = 0;  // ls_k: the light sample's emitter triangle
+  float lw = 0.0f;
   // accum_color += trace(...) (shader.cu:154) and the per-sample records.
   auto finish_sample = [&]() {
     if (kMode == 2 && t_used > 0) {
@@ -533,6 +551,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       hitf = false;
       qstage = 0;
       b = 0;
+      if (kEmit) cond = false;
       if (kRecord) {
         hash = 2166136261u;
         occl_cnt = 0;
@@ -548,6 +567,52 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // next bounce (shader.cu:251-252), end the sample after the last bounce.
   auto end_nee = [&](F3 nee) -> bool {
     color = color + nee * att;
+    if (kEmit && rough == 1.0f && scp->emit.T > 0 && b < pp->bounces) {
+      // Expected emission of this vertex's bounce ray.  The reference adds
+      // Le * att when the bounce (a hemisphere draw at roughness 1, density
+      // q = 1 / (12 max|w_i|^3) on N's side) hits an emitter, and ends the
+      // path.  Here one light sample estimates p = P(the bounce's closest
+      // hit emits) and a = E[Le; hit] without bias: p^ = V q |cos_y| A / r^2,
+      // a^ = Le p^; the path adds a^ * att, continues with att * (1 - p^),
+      // and its bounce is drawn again whenever it hits an emitter (a draw on
+      // the condition that it does not).  The expectation is the reference's.
+      const float u0 = rnd(seed), u1 = rnd(seed), u2 = rnd(seed);
+      const int32_t T = scp->emit.T;
+      const float ta = u0 * pp->emit_area;
+      int32_t lo = 0, hi = T - 1;  // first triangle whose prefix sum exceeds ta
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (ta < pp->emit_cdf[mid])
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      const float4* tp = scp->emit.tris + 3 * (size_t)lo;
+      const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+      const F3 v0 = xyz(p0), v1 = xyz(p1), v2 = xyz(p2);
+      const float su = sqrtf(u1);
+      const F3 yl = v0 * (1.0f - su) + v1 * (su * (1.0f - u2)) + v2 * (su * u2);
+      const F3 wv = yl - O;
+      const float r2 = dot(wv, wv);
+      const F3 w = wv * (1.0f / sqrtf(r2));
+      if (r2 > 0.0f && dot(N, w) > 0.0f && ray_valid(w)) {
+        const F3 e1 = v1 - v0, e2 = v2 - v0;
+        const F3 nt = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+        const float cy = fabsf(dot(nt, w)) / sqrtf(dot(nt, nt));
+        const float mw = fmaxf(fabsf(w.x), fmaxf(fabsf(w.y), fabsf(w.z)));
+        const float q = 1.0f / (12.0f * mw * mw * mw);
+        lw = q * cy * pp->emit_area / r2;
+        ls_k = lo;
+        D = w;
+        qstage = 4;  // the emitter query after the NEE loop: is lo the closest emitter?
+        return true;
+      }
+      D = Bsdf::sample(Din, N, seed, rough);  // p^ = 0
+      cond = true;
+      rejects = 0;
+      qstage = 0;
+      return true;
+    }
     D = Bsdf::sample(Din, N, seed, rough);
     qstage = 0;
     if (b == pp->bounces) {
@@ -648,6 +713,25 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       if (!alive) break;
     }
     if (!alive) continue;  // the pixel is written at the top of the loop
+    if (kEmit && qstage == 4) {
+      // kMode 3's light sample: is the sampled triangle the closest emitter
+      // along the line?  Then its any-hit test (qstage 3), else its bounce.
+      float t = 0.0f;
+      const int32_t k = closest<kCount, kPathBlock>(scp->emit, make_ray(O, D), kTmin, kTmax, 0xFFFFFFFFu,
+                                                    false, t, my_stk, tc);
+      if (k == ls_k) {
+        const float4 e0 = scp->emit.tris[3 * (size_t)k], e1 = scp->emit.tris[3 * (size_t)k + 1];
+        tE = t;
+        primE = __float_as_uint(e0.w);
+        matE = (int32_t)__float_as_uint(e1.w);
+        qstage = 3;
+      } else {  // p^ = 0
+        D = Bsdf::sample(Din, N, seed, rough);
+        cond = true;
+        rejects = 0;
+        qstage = 0;
+      }
+    }
 
     // -- one full traversal per iteration: the closest hit of a radiance ray
     // (qstage 0) or the any-hit query of an occlusion ray (qstage 2).
@@ -656,7 +740,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     float t = 0.0f;
     if (valid && scp->all.T > 0) {
       const RayT r = make_ray(O, D);
-      const bool any = qstage == 2;
+      const bool any = qstage >= 2;
       k = closest<kCount, kPathBlock>(scp->all, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
                                       any, t, my_stk, tc);
     }
@@ -684,12 +768,23 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       if (k < 0) {
         color = color + mk(0.0f, 0.0f, 0.0f);
         finish = true;
+        if (kEmit) cond = false;
       } else {
         const DevMat m = scp->mats[mat];
-        if (m.emission_emit.w != 0.0f) {
+        if (kEmit && cond && m.emission_emit.w != 0.0f) {
+          // a conditioned bounce hit an emitter: draw it again (not a bounce)
+          --b;
+          if (++rejects == kMaxRedraws) {
+            finish = true;
+          } else {
+            D = Bsdf::sample(Din, N, seed, rough);
+            continue;
+          }
+        } else if (m.emission_emit.w != 0.0f) {
           color = color + xyz(m.emission_emit) * att;
           finish = true;
         } else {
+          if (kEmit) cond = false;
           if (kCount) c_rhit++;
           O = O + t * D;
           Din = D;
@@ -699,7 +794,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
             N = bary_normal(O, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
           }
           bool cached = false;
-          if (kMode == 1 && b == pp->cache_at) {
+          if (kQuery && b == pp->cache_at) {
             const uint32_t slot = wave_reserve(pp->q_count);
             if (slot < pp->q_cap) {
               // the hit itself; k_cache_features turns it into the features
@@ -750,6 +845,15 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
         finish_sample();
         alive = begin_sample();
       }
+    } else if (kEmit && qstage == 3) {
+      // the light sample of a prefix bounce: visible unless blocked before (tE, primE)
+      const float ph = k >= 0 ? 0.0f : lw;
+      color = color + (xyz(scp->mats[matE].emission_emit) * ph) * att;
+      att = att * (1.0f - ph);
+      D = Bsdf::sample(Din, N, seed, rough);
+      cond = true;
+      rejects = 0;
+      qstage = 0;
     } else {
       // occlusion, second query: blocked before (tE, primE)?
       const bool blocked = k >= 0;
@@ -822,7 +926,8 @@ __global__ __launch_bounds__(kPathBlock, (kRecord || kCount) ? 4 : LISA_PATH_MIN
 // Path loop in a cache mode (1: cached render of the full image in tiles,
 // 2: training paths on a pixel list).
 template <int kMode>
-__global__ __launch_bounds__(kPathBlock, LISA_MODE_WAVES) void k_path_mode(SceneView sc, PathArgs p) {
+__global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : LISA_MODE_WAVES) void k_path_mode(
+    SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   OnePixel src{sc, p, false};
   path_lane<false, false, OnePixel, kMode>(src, stk + threadIdx.x);
@@ -1487,6 +1592,8 @@ struct lisa_ctx {
   DevBuf<float> c_qfeat, c_qatt, c_qout, c_trec, c_tfeat, c_ttgt;
   DevBuf<int32_t> c_qitem, c_tpix;
   DevBuf<uint32_t> c_qcount;
+  DevBuf<float> c_ecdf;     // area prefix sums of the emitting triangles (LISA_CACHE_EMISSION)
+  float emit_area = 0.0f;
 };
 
 struct lisa_batch {
@@ -1710,6 +1817,19 @@ lisa_status launch_persistent(lisa_ctx* ctx, PathArgs a, int slot) {
 }
 
 // Scene box of the cache features: union of the root's two child boxes.
+// Area prefix sums of the emitting triangles in emitter-tree order (one
+// thread: the emitter set is small, and the sums are sequential).
+__global__ void k_emit_cdf(const float4* __restrict__ tris, int32_t T, float* cdf) {
+  float acc = 0.0f;
+  for (int32_t k = 0; k < T; ++k) {
+    const F3 v0 = xyz(tris[3 * (size_t)k]), v1 = xyz(tris[3 * (size_t)k + 1]), v2 = xyz(tris[3 * (size_t)k + 2]);
+    const F3 e1 = v1 - v0, e2 = v2 - v0;
+    const F3 c = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+    acc = acc + 0.5f * sqrtf(dot(c, c));
+    cdf[k] = acc;
+  }
+}
+
 // Union of the child boxes of a root node (its 16 words).
 void root_box(const float* rb, float lo[3], float hi[3]) {
   uint32_t ref[2];
@@ -1747,6 +1867,13 @@ lisa_status read_scene_box(lisa_ctx* ctx) {
   float lo[3], hi[3];
   HIP_TRY(ctx, hipMemcpy(rb, ctx->emit.nodes, 64, hipMemcpyDeviceToHost));
   root_box(rb, lo, hi);
+  // light sampling of LISA_CACHE_EMISSION: area prefix sums of the emitters
+  HIP_TRY(ctx, ctx->c_ecdf.ensure((size_t)ctx->emit.T));
+  k_emit_cdf<<<1, 1, 0, ctx->stream>>>(ctx->emit.tris, ctx->emit.T, ctx->c_ecdf.p);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(&ctx->emit_area, ctx->c_ecdf.p + (ctx->emit.T - 1), 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (int a = 0; a < 3; ++a) {
     ctx->ecull_c[a] = 0.5f * (lo[a] + hi[a]);
     ctx->ecull_h[a] = 0.5f * (hi[a] - lo[a]) + m2;
@@ -1763,6 +1890,9 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
               1.0f / std::max(ctx->bmax[1] - ctx->bmin[1], 1e-20f),
               1.0f / std::max(ctx->bmax[2] - ctx->bmin[2], 1e-20f));
   a.fp64 = 0;
+  a.emit_expect = 0;
+  a.emit_cdf = ctx->c_ecdf.p;
+  a.emit_area = ctx->emit_area;
   return a;
 }
 
@@ -1799,7 +1929,7 @@ lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, si
   return LISA_OK;
 }
 
-// Prefix paths (path loop kMode 1): the full image in tiles (d_pix null) or
+// Prefix paths (path loop kMode 1, or 3 with a.emit_expect): the full image in tiles (d_pix null) or
 // n listed device pixels; radiance gathered before the cache into `frame`
 // (per pixel mean), queries into ctx->c_q* (capacity n * spp); *nq = queries.
 lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, uint32_t prefix,
@@ -1824,7 +1954,7 @@ lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_
   a.q_count = ctx->c_qcount.p;
   a.q_cap = (uint32_t)qcap;
   if (persistent) {
-    lisa_status rc = launch_persistent<1>(ctx, a, 2);
+    lisa_status rc = a.emit_expect ? launch_persistent<3>(ctx, a, 2) : launch_persistent<1>(ctx, a, 2);
     if (rc != LISA_OK) return rc;
   } else {
     unsigned blocks;
@@ -1834,7 +1964,10 @@ lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_
       a.tiles_x = (a.width + kTileW - 1) / kTileW;
       blocks = a.tiles_x * ((a.height + kTileH - 1) / kTileH);
     }
-    k_path_mode<1><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
+    if (a.emit_expect)
+      k_path_mode<3><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
+    else
+      k_path_mode<1><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
     HIP_TRY(ctx, hipGetLastError());
   }
   HIP_TRY(ctx, hipMemcpyAsync(nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
@@ -1916,6 +2049,7 @@ void lisa_destroy(lisa_ctx* c) {
   c->c_qitem.release();
   c->c_tpix.release();
   c->c_qcount.release();
+  c->c_ecdf.release();
   free_scene(c);
   (void)hipFree(c->d_accum);
   (void)hipFree(c->d_counters);
@@ -2480,7 +2614,9 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     out_stats.fell_back = 1;
   } else {
     uint32_t nq = 0;
-    rc = trace_queries(ctx, base, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, &nq,
+    PathArgs qa = base;
+    qa.emit_expect = (cp->flags & LISA_CACHE_EMISSION) ? 1 : 0;
+    rc = trace_queries(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, &nq,
                        (cp->flags & LISA_CACHE_PERSISTENT) != 0);
     if (rc != LISA_OK) return rc;
     nq = std::min<uint32_t>(nq, (uint32_t)(npx * p->spp));
@@ -2536,6 +2672,7 @@ lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
   if (rc != LISA_OK) return rc;
   HIP_TRY(ctx, ctx->c_frame.ensure((size_t)n));
   PathArgs a = cache_args(ctx, p);
+  a.emit_expect = (p->flags & LISA_FLAG_CACHE_EMISSION) ? 1 : 0;
   uint32_t nq = 0;
   rc = trace_queries(ctx, a, ctx->c_tpix.p, (size_t)n, prefix_bounces, ctx->c_frame.p, &nq);
   if (rc != LISA_OK) return rc;

@@ -205,7 +205,25 @@ struct Scene {
   Bvh all;   // every triangle
   Bvh emit;  // the triangles whose material emits (light queries of NEE)
   EmitCull ecull;
+  // area prefix sums of the emitting triangles in emitter-tree order, and the
+  // total (the light sample of the cache's prefix-emission estimator)
+  std::vector<float> ecdf;
+  float earea = 0.0f;
 };
+
+// The emitter area prefix sums, accumulated in tree order in FP32.
+void emit_cdf(Scene& s) {
+  s.ecdf.assign((size_t)s.emit.T, 0.0f);
+  float acc = 0.0f;
+  for (int32_t k = 0; k < s.emit.T; ++k) {
+    const float* t = &s.emit.tv[9 * (size_t)k];
+    const V3 v0 = ld3(t), v1 = ld3(t + 3), v2 = ld3(t + 6);
+    const V3 c = cross(v1 - v0, v2 - v0);
+    acc = acc + 0.5f * std::sqrt(dot(c, c));
+    s.ecdf[(size_t)k] = acc;
+  }
+  s.earea = acc;
+}
 
 inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
 
@@ -517,6 +535,7 @@ int32_t closest_brute(const Scene& s, const Ray& r, float tmin, float tmax, floa
 constexpr float kTmin = 1e-4f;  // shader.cu:114, :202
 constexpr float kTmax = 1e16f;  // shader.cu:115, :202
 constexpr int kNeeTries = 30;   // shader.cu:199
+constexpr int kMaxRedraws = 64;  // conditioned bounces drawn again at most this often
 
 struct RayState {  // shader.cu:30-42
   int32_t light_mat = -1;
@@ -745,6 +764,15 @@ struct Tracer {
   bool queried = false;
   float qf[12] = {};
   V3 qatt{0, 0, 0};
+  // prefix-emission estimator (queries only): at a roughness-1 Lambertian
+  // vertex with a bounce left, one light sample takes the emission of the
+  // bounce in expectation and the bounce is drawn on the condition that it
+  // hits no emitter (cond; din and rough_v are the vertex's for redraws)
+  bool emit_expect = false;
+  bool cond = false;
+  int32_t rejects = 0, max_b = 0;
+  V3 din{0, 0, 0};
+  float rough_v = 0.0f;
 
   const orc_material& mat_of(int32_t prim) const { return s.mats[s.mat[prim]]; }
 
@@ -782,24 +810,80 @@ struct Tracer {
     return mk(0.0f, 0.0f, 0.0f);
   }
 
+  // The light sample of the prefix-emission estimator at the current vertex.
+  // The bounce (a hemisphere draw at roughness 1: density q = 1 / (12
+  // max|w_i|^3) on N's side) would hit an emitter with probability p and add
+  // Le * att, ending the path.  With y drawn by area over the emitters,
+  // p^ = V q |cos_y| A / r^2 estimates p and Le p^ the emission without bias;
+  // the path adds Le p^ att and goes on with att (1 - p^) behind a bounce
+  // drawn on the condition that it hits no emitter, whose expectation is the
+  // reference's.  V: y's triangle is the closest emitter along the line and
+  // nothing lies before it.
+  void emit_sample(RayState& st) {
+    const float u0 = rnd(seed), u1 = rnd(seed), u2 = rnd(seed);
+    const float ta = u0 * s.earea;
+    int32_t lo = 0, hi = s.emit.T - 1;  // first prefix sum above ta
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (ta < s.ecdf[(size_t)mid])
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    const float* tv = &s.emit.tv[9 * (size_t)lo];
+    const V3 v0 = ld3(tv), v1 = ld3(tv + 3), v2 = ld3(tv + 6);
+    const float su = std::sqrt(u1);
+    const V3 y = v0 * (1.0f - su) + v1 * (su * (1.0f - u2)) + v2 * (su * u2);
+    const V3 wv = y - st.origin;
+    const float r2 = dot(wv, wv);
+    const V3 w = wv * (1.0f / std::sqrt(r2));
+    const Ray r = make_ray(st.origin, w);
+    if (!(r2 > 0.0f && dot(st.normal, w) > 0.0f && r.valid)) return;
+    const V3 nt = cross(v1 - v0, v2 - v0);
+    const float cy = std::fabs(dot(nt, w)) / std::sqrt(dot(nt, nt));
+    const float mw = std::fmax(std::fabs(w.x), std::fmax(std::fabs(w.y), std::fabs(w.z)));
+    const float q = 1.0f / (12.0f * mw * mw * mw);
+    float te = 0.0f, tb = 0.0f;
+    const int32_t e = closest<false>(s.emit, r, kTmin, kTmax, 0xFFFFFFFFu, te, cnt);
+    if (e != (int32_t)s.emit.prim[(size_t)lo]) return;
+    const float lw = q * cy * s.earea / r2;
+    const float ph = closest<true>(s.all, r, kTmin, te, (uint32_t)e, tb, cnt) >= 0 ? 0.0f : lw;
+    st.color = st.color + (ld3(s.mats[s.mat[e]].emission) * ph) * st.att;
+    st.att = st.att * (1.0f - ph);
+  }
+
   // One radiance trace: shader.cu:77-98 + closest-hit 211-255 / miss 189-194.
-  void radiance(RayState& st, V3 O, V3 D) {
+  // Returns false for a conditioned bounce that hit an emitter and was drawn
+  // again (not a bounce; after kMaxRedraws of them the path ends).
+  bool radiance(RayState& st, V3 O, V3 D) {
     cnt.c[0]++;
     float t = 0.0f;
     const int32_t p = trace_ray(O, D, t);
+    if (cond) {
+      cond = false;
+      if (p >= 0 && mat_of(p).emit) {
+        if (++rejects == kMaxRedraws) {
+          st.done = true;
+        } else {
+          st.direction = bsdf_bounce(din, st.normal, seed, rough_v);
+          cond = true;
+        }
+        return false;
+      }
+    }
     rec.ray(p, false);
     if (rec.rad_hits && rec.bounces < rec.rb) rec.rad_hits[rec.bounces] = p;
     rec.bounces++;
     if (p < 0) {
       st.color = st.color + mk(0.0f, 0.0f, 0.0f);
       st.done = true;
-      return;
+      return true;
     }
     const orc_material& m = mat_of(p);
     if (m.emit) {
       st.color = st.color + ld3(m.emission) * st.att;
       st.done = true;
-      return;
+      return true;
     }
     cnt.c[4]++;
     st.origin = O + t * D;
@@ -814,7 +898,7 @@ struct Tracer {
       qatt = st.att;
       queried = true;
       st.done = true;
-      return;
+      return true;
     }
     if (verts && (vert_depth == 0 || rec.bounces == vert_depth)) {
       CacheVertex cv;
@@ -847,15 +931,25 @@ struct Tracer {
       st.att = st.att * ld3(m.diffuse);
       const V3 nee = shoot_ray_to_light(st);
       st.color = st.color + nee * st.att;
+      if (emit_expect && m.roughness == 1.0f && s.emit.T > 0 && rec.bounces < max_b) {
+        emit_sample(st);
+        cond = true;
+        rejects = 0;
+        din = st.direction;
+        rough_v = m.roughness;
+      }
       st.direction = bsdf_bounce(st.direction, st.normal, seed, m.roughness);
     }
+    return true;
   }
 
   // shader.cu:102-124.
   V3 trace(V3 O, V3 D, int bounces) {
     RayState st;
-    for (int i = 0; i < bounces; ++i) {
-      radiance(st, O, D);
+    max_b = bounces;
+    cond = false;
+    for (int i = 0; i < bounces;) {
+      if (radiance(st, O, D)) ++i;
       D = st.direction;
       O = st.origin;
       if (st.done) break;
@@ -1007,6 +1101,7 @@ orc_scene* orc_scene_create(const float* verts, const float* normals, const int3
   build_bvh(s.all, s.v, gid_all);
   build_bvh(s.emit, s.v, gid_emit);
   s.ecull = emit_cull(s);
+  emit_cdf(s);
   return o;
 }
 void orc_scene_destroy(orc_scene* s) { delete s; }
@@ -1018,6 +1113,7 @@ void orc_scene_update_vertices(orc_scene* o, const float* verts, const float* no
   update_bvh(s.all, s.v);
   update_bvh(s.emit, s.v);
   s.ecull = emit_cull(s);
+  emit_cdf(s);
 }
 int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
 
@@ -1101,7 +1197,8 @@ void orc_cache_bounds(const orc_scene* o, float* bmin, float* binv) {
 
 int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t prefix_bounces,
                       const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
-                      float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads) {
+                      float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads,
+                      int emit_expect) {
   const Scene& s = o->s;
   if (p->width == 0 || p->height == 0 || p->spp == 0 || n <= 0) return -1;
   const int64_t npx = (int64_t)p->width * p->height;
@@ -1131,6 +1228,7 @@ int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t p
         Tracer tr{s, cnt, seed, rec};
         tr.cache_at = (int32_t)prefix_bounces + 1;
         tr.box = &box;
+        tr.emit_expect = emit_expect != 0;
         const V3 c = tr.trace(eye, D, (int)prefix_bounces + 1);
         accum = accum + c;
         if (tr.queried) {

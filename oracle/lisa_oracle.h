@@ -125,10 +125,16 @@ void orc_cache_bounds(const orc_scene* s, float* bmin, float* binv);
  * rays at most: the surface hit of radiance ray prefix_bounces + 1 ends the
  * path in a query (12 features, throughput).  accum_out (4n): per-pixel mean
  * of the radiance gathered before the queries.  Queries in (pixel, sample)
- * order; q_item = index into pixel_ids; *n_queries may exceed q_cap. */
+ * order; q_item = index into pixel_ids; *n_queries may exceed q_cap.
+ * emit_expect != 0: at each roughness-1 Lambertian vertex with a bounce left,
+ * one light sample (by area over the emitters) adds the bounce's emission in
+ * expectation, the throughput is scaled by (1 - the sample's hit estimate)
+ * and the bounce is drawn on the condition that it hits no emitter (at most
+ * 64 redraws, then the path ends).  Same expectation as without it. */
 int orc_cache_queries(const orc_scene* s, const orc_render_params* p, uint32_t prefix_bounces,
                       const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
-                      float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads);
+                      float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads,
+                      int emit_expect);
 /* Training rows of plain paths (p->bounces) on the listed pixels: one row per
  * surface hit (depth 0) or per surface hit of radiance ray number `depth`,
  * 12 features and the target (final - gathered before the hit) / throughput

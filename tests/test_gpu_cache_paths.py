@@ -45,15 +45,19 @@ def _close(a, b, rtol=1e-5):
     return np.all(np.abs(a - b) <= rtol * np.abs(b) + 1e-30)
 
 
-@pytest.mark.parametrize("spp,prefix", [(1, 2), (4, 2), (2, 1), (2, 0)])
-def test_queries_match_oracle(capi, oracle_mod, ctx, spp, prefix):
+@pytest.mark.parametrize("spp,prefix,emit", [(1, 2, False), (4, 2, False), (2, 1, False), (2, 0, False),
+                                             (4, 2, True), (2, 1, True), (8, 3, True)])
+def test_queries_match_oracle(capi, oracle_mod, ctx, spp, prefix, emit):
+    """emit: the prefix-emission estimator (LISA_FLAG_CACHE_EMISSION), whose
+    light samples, conditioned bounces and throughputs are bit-exact too."""
     W = H = 64
     sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
     eye = sc["camera"]["eye"]
     px = np.sort(np.random.default_rng(spp).choice(W * H, 1500, replace=False)).astype(np.int32)
-    p = capi.render_params(W, H, spp, prefix + 1, eye, U, V, Wv, subframe=3)
+    p = capi.render_params(W, H, spp, prefix + 1, eye, U, V, Wv, subframe=3,
+                           flags=capi.FLAG_CACHE_EMISSION if emit else 0)
     g = capi.cache_trace_queries(ctx, p, prefix, px)
-    r = o.cache_queries(W, H, spp, prefix, eye, U, V, Wv, px, subframe=3, threads=8)
+    r = o.cache_queries(W, H, spp, prefix, eye, U, V, Wv, px, subframe=3, threads=8, emit_expect=emit)
     print("queries", g["n"], r["n"], flush=True)
     assert g["n"] == r["n"] > 0
     gi, ri = _order(g["item"], g["feat"]), _order(r["item"], r["feat"])
@@ -172,7 +176,8 @@ def test_trained_cache_gains_over_truncated_paths(capi, oracle_mod, ctx):
             c.close()
 
 
-def test_persistent_cached_frames_same_image(capi, oracle_mod, ctx):
+@pytest.mark.parametrize("emission", [False, True])
+def test_persistent_cached_frames_same_image(capi, oracle_mod, ctx, emission):
     """LISA_CACHE_PERSISTENT runs a frame's paths in one persistent launch.
     With a network that answers 0 (and no training) the frame is the prefix
     radiance alone, which both launch forms compute identically."""
@@ -185,9 +190,10 @@ def test_persistent_cached_frames_same_image(capi, oracle_mod, ctx):
         other.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
         for c in (a, b):
             c.set_weights(np.zeros(capi.CACHE_PARAMS, np.float32))
-        tiled = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0)
+        em = capi.CACHE_EMISSION if emission else 0
+        tiled = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0, flags=em)
         pers = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0,
-                                       flags=capi.CACHE_PERSISTENT)
+                                       flags=capi.CACHE_PERSISTENT | em)
         for f in range(3):
             p = capi.render_params(W, H, 2, 7, eye, U, V, Wv, subframe=f)
             s1 = capi.render_cached(ctx, a, p, tiled)

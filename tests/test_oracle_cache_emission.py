@@ -1,0 +1,94 @@
+"""CPU: the prefix-emission estimator of the cached renderer's prefix paths
+(orc_cache_queries with emit_expect, the oracle of LISA_CACHE_EMISSION).
+
+At a roughness-1 Lambertian vertex with a bounce left, the plain tracer's
+bounce (shader.cu:251-252 -> closest hit 211-223) adds Le * att when it hits
+an emitter and ends the path.  The estimator takes that term in expectation
+from one light sample and draws the bounce on the condition that it hits no
+emitter, with the throughput scaled by (1 - the sample's hit estimate).  Its
+expectation is the plain one, so both quantities that the cache's image is
+built from keep their means: the radiance gathered before the queries
+(accum) and the throughput that reaches the queries (what a network that
+answers 1 everywhere would add).  The check is a paired z-test over pixels
+(same seeds, independent pixels) on two scenes: the island's sun seen from
+the ground and the Cornell box's ceiling light.  The bound 4 leaves room for
+chance; leaving out the emission term gives z = -10 and -19, leaving out the
+(1 - p) throughput factor z = 7.4 on the Cornell box, so both mistakes fail.
+"""
+import numpy as np
+import pytest
+
+from lisa_amd import scenes
+from tests.scene_util import cornell
+
+Z_BOUND = 4.0
+
+
+def _pixel_sums(o, om, sc, W, H, spp, prefix, subframe, emit):
+    cam = sc["camera"]
+    U, V, Wv = om.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+    px = np.arange(W * H, dtype=np.int32)
+    r = o.cache_queries(W, H, spp, prefix, cam["eye"], U, V, Wv, px, subframe=subframe, threads=8,
+                        emit_expect=emit)
+    acc = r["accum"][:, :3].astype(np.float64).sum(1)
+    q = np.zeros(W * H)
+    np.add.at(q, r["item"], r["att"].astype(np.float64).sum(1))
+    return acc, q / spp, r
+
+
+def _scene(om, sc):
+    return om.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+
+
+CASES = {
+    "island": (lambda: scenes.island(n=128, width=64, height=36), 64, 36, 128, 2),
+    "cornell": (lambda: cornell(32, 16, 48, 48, 4, 7), 48, 48, 256, 1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_prefix_emission_keeps_the_expectation(oracle_mod, case):
+    make, W, H, spp, prefix = CASES[case]
+    sc = make()
+    o = _scene(oracle_mod, sc)
+    a0, q0, r0 = _pixel_sums(o, oracle_mod, sc, W, H, spp, prefix, 3, False)
+    a1, q1, r1 = _pixel_sums(o, oracle_mod, sc, W, H, spp, prefix, 3, True)
+    assert r0["n"] > 0 and r1["n"] > 0
+    for name, x, y in (("accum", a0, a1), ("query throughput", q0, q1)):
+        d = y - x
+        z = d.mean() / (d.std() / np.sqrt(d.size))
+        print(f"{case} {name}: plain {x.mean():.5f} estimator {y.mean():.5f} z {z:+.2f}", flush=True)
+        assert abs(z) < Z_BOUND, (case, name, z)
+
+
+def test_prefix_emission_lowers_the_island_variance(oracle_mod):
+    """On the island most of the prefix's noise is the sun hit by a diffuse
+    bounce; taken in expectation it halves the per-pixel Monte Carlo
+    variance of the prefix radiance (measured: 0.050 -> 0.024 at 16 spp)."""
+    W, H, spp = 96, 54, 16
+    sc = scenes.island(n=256, width=W, height=H)
+    o = _scene(oracle_mod, sc)
+    var = {}
+    for emit in (False, True):
+        x = _pixel_sums(o, oracle_mod, sc, W, H, spp, 2, 5, emit)[0]
+        y = _pixel_sums(o, oracle_mod, sc, W, H, spp, 2, 6, emit)[0]
+        var[emit] = float(np.var(x - y) / 2)
+    print("Monte Carlo variance of the prefix radiance:", var, flush=True)
+    assert var[True] < 0.7 * var[False]
+
+
+def test_no_emitter_is_the_plain_prefix(oracle_mod):
+    """Without emitters there is nothing to sample: the estimator's paths
+    are the plain prefix paths, bit for bit."""
+    sc = cornell(16, 8, 32, 32, 2, 7)
+    mats = [dict(m) for m in sc["materials"]]
+    for m in mats:
+        m["emit"] = False
+        m["emission"] = (0.0, 0.0, 0.0)
+    o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], mats)
+    sc2 = dict(sc)
+    a = _pixel_sums(o, oracle_mod, sc2, 32, 32, 2, 2, 1, False)[2]
+    b = _pixel_sums(o, oracle_mod, sc2, 32, 32, 2, 2, 1, True)[2]
+    assert a["n"] == b["n"] > 0
+    for k in ("accum", "feat", "att", "item"):
+        assert np.array_equal(a[k], b[k]), k
