@@ -60,7 +60,6 @@ typedef struct lisa_render_params {
 #define LISA_MODE_PLAIN 0        /* one work-item per pixel megakernel (the drop-in) */
 #define LISA_FLAG_FP64_ACCUM 1u  /* per-pixel FP64 accumulation (reference renders >= 16k spp) */
 #define LISA_FLAG_COUNT 2u       /* also count rays / node visits / triangle tests */
-#define LISA_FLAG_CACHE_EMISSION 8u  /* lisa_cache_trace_queries only: the LISA_CACHE_EMISSION prefix */
 #define LISA_FLAG_PERSISTENT 4u  /* one persistent launch whose lanes pull pixels from a queue
                                     (same image; not with COUNT or FP64_ACCUM) */
 
@@ -208,14 +207,19 @@ typedef struct lisa_cache_frame_params {
   uint32_t train_max;      /* at most this many training rows (suffixes) per frame; 0: no cap */
   uint32_t train_steps;    /* Adam steps per frame over the frame's rows */
   float fallback_loss;     /* > 0: plain frame when the last training loss exceeds it */
-  uint32_t flags;          /* LISA_CACHE_PERSISTENT: the frame's paths in a persistent launch */
+  uint32_t flags;          /* LISA_CACHE_* */
+  uint32_t nee_loops;      /* with LISA_CACHE_EMISSION: NEE loops averaged per prefix vertex
+                              (0 or 1: the reference's one; at most 16) */
   uint64_t seed;           /* draws the training pixels */
 } lisa_cache_frame_params;
-#define LISA_CACHE_PERSISTENT 1u
+#define LISA_CACHE_PERSISTENT 1u /* the frame's paths in a persistent launch */
 /* The prefix paths add the expected emission of each Lambertian (roughness 1)
  * bounce from one light sample and draw the bounce on the condition that it
  * hits no emitter, weighted by 1 - (the sampled chance that it would): the
- * same expectation as the plain tracer's bounces, with less variance. */
+ * same expectation as the plain tracer's bounces, with less variance.  With
+ * nee_loops = M > 1 the NEE of a prefix vertex is the mean of M independent
+ * runs of the reference's NEE loop from the vertex's hit flag and light; the
+ * last run's flag and light go on down the path (same expectation). */
 #define LISA_CACHE_EMISSION 2u
 
 typedef struct lisa_cache_frame_stats {
@@ -236,10 +240,11 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
  * (atan2(z, x) + pi)/2pi), albedo rgb, roughness (0 for a dielectric),
  * dielectric flag.
  *
- * lisa_cache_trace_queries: p->spp samples per listed pixel (seeded by
- * p->subframe; LISA_FLAG_CACHE_EMISSION in p->flags selects the prefix form
- * of LISA_CACHE_EMISSION), at most prefix_bounces + 1 radiance rays; the surface hit of
- * radiance ray prefix_bounces + 1 ends the path in a query (features,
+ * lisa_cache_trace_queries: the prefix paths of a frame with cp's
+ * prefix_bounces, flags (LISA_CACHE_EMISSION) and nee_loops: p->spp samples
+ * per listed pixel (seeded by p->subframe), at most prefix_bounces + 1
+ * radiance rays; the surface hit of radiance ray prefix_bounces + 1 ends the
+ * path in a query (features,
  * throughput, index into `pixels`).  accum_out (4n) = per-pixel mean of the
  * radiance gathered before the queries.  Queries come in no fixed order;
  * *n_queries may exceed q_cap (the rest is dropped).
@@ -249,9 +254,10 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
  * ray number `depth`: features and the target (path radiance - radiance
  * gathered before the hit) / throughput on arrival.  Rows of one path are
  * contiguous, paths in no fixed order; *n_rows may exceed cap. */
-lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p, uint32_t prefix_bounces,
-                                     const int32_t* pixels, int32_t n, float* accum_out, float* q_feat,
-                                     float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries);
+lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
+                                     const lisa_cache_frame_params* cp, const int32_t* pixels, int32_t n,
+                                     float* accum_out, float* q_feat, float* q_att, int32_t* q_item,
+                                     int32_t q_cap, int32_t* n_queries);
 lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, const int32_t* pixels,
                                   int32_t n, uint32_t depth, float* feat, float* tgt, int32_t cap,
                                   int32_t* n_rows);

@@ -29,7 +29,6 @@ MODE_PLAIN = 0
 FLAG_FP64_ACCUM = 1
 FLAG_COUNT = 2
 FLAG_PERSISTENT = 4
-FLAG_CACHE_EMISSION = 8  # lisa_cache_trace_queries: the CACHE_EMISSION prefix
 NCOUNTERS = 8
 COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
                  "radiance_hits", "samples", "invalid_dir_rays", "pixels")
@@ -136,8 +135,8 @@ def lib():
         "lisa_cache_last_ms": (C.c_float, [vp]),
         "lisa_render_cached": (st, [vp, vp, C.POINTER(RenderParams), C.POINTER(CacheFrameParams),
                                     C.POINTER(CacheFrameStats)]),
-        "lisa_cache_trace_queries": (st, [vp, C.POINTER(RenderParams), C.c_uint32, i32p, C.c_int32,
-                                          f32p, f32p, f32p, i32p, C.c_int32, i32p]),
+        "lisa_cache_trace_queries": (st, [vp, C.POINTER(RenderParams), C.POINTER(CacheFrameParams), i32p,
+                                          C.c_int32, f32p, f32p, f32p, i32p, C.c_int32, i32p]),
         "lisa_cache_trace_rows": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, C.c_uint32,
                                        f32p, f32p, C.c_int32, i32p]),
         "lisa_camera_uvw": (st, [f32p, f32p, C.c_float, C.c_float, f32p, f32p, f32p]),
@@ -436,7 +435,7 @@ class CacheFrameParams(C.Structure):
     _fields_ = [("prefix_bounces", C.c_uint32), ("train_paths", C.c_uint32),
                 ("train_bounces", C.c_uint32), ("train_max", C.c_uint32),
                 ("train_steps", C.c_uint32), ("fallback_loss", C.c_float),
-                ("flags", C.c_uint32), ("seed", C.c_uint64)]
+                ("flags", C.c_uint32), ("nee_loops", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class CacheFrameStats(C.Structure):
@@ -447,9 +446,9 @@ class CacheFrameStats(C.Structure):
 
 
 def cache_frame_params(prefix_bounces=2, train_paths=8192, train_bounces=12, train_max=1 << 16,
-                       train_steps=1, fallback_loss=0.0, seed=1, flags=0):
+                       train_steps=1, fallback_loss=0.0, seed=1, flags=0, nee_loops=1):
     return CacheFrameParams(prefix_bounces, train_paths, train_bounces, train_max, train_steps,
-                            fallback_loss, flags, seed)
+                            fallback_loss, flags, nee_loops, seed)
 
 
 def render_cached(ctx, cache, params, cparams):
@@ -462,8 +461,9 @@ def render_cached(ctx, cache, params, cparams):
     return {k: getattr(s, k) for k, _ in CacheFrameStats._fields_}
 
 
-def cache_trace_queries(ctx, params, prefix_bounces, pixels):
-    """Prefix paths on a pixel list (lisa_cache_trace_queries): accum, features, throughput, item."""
+def cache_trace_queries(ctx, params, cparams, pixels):
+    """Prefix paths on a pixel list (lisa_cache_trace_queries) with the prefix settings of
+    cparams (prefix_bounces, CACHE_EMISSION, nee_loops): accum, features, throughput, item."""
     px = np.ascontiguousarray(pixels, dtype=np.int32)
     n = px.shape[0]
     cap = n * params.spp
@@ -472,7 +472,7 @@ def cache_trace_queries(ctx, params, prefix_bounces, pixels):
     at = np.zeros((cap, 3), np.float32)
     it = np.zeros(cap, np.int32)
     nq = C.c_int32()
-    ctx._ck(lib().lisa_cache_trace_queries(ctx.h, C.byref(params), prefix_bounces, _p(px, C.c_int32), n,
+    ctx._ck(lib().lisa_cache_trace_queries(ctx.h, C.byref(params), C.byref(cparams), _p(px, C.c_int32), n,
                                            _p(acc, C.c_float), _p(f, C.c_float), _p(at, C.c_float),
                                            _p(it, C.c_int32), cap, C.byref(nq)))
     k = min(nq.value, cap)

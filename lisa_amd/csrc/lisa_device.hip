@@ -64,7 +64,7 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #define LISA_MODE_WAVES 5  // the cache-mode path kernels
 #endif
 #ifndef LISA_EMIT_WAVES
-#define LISA_EMIT_WAVES 5  // the query kernel with the prefix-emission light samples
+#define LISA_EMIT_WAVES 4  // the query kernel with the prefix-emission light samples
 #endif
 #ifndef LISA_TILE_WAVES
 #define LISA_TILE_WAVES 5  // the batched wave-tile form
@@ -394,6 +394,7 @@ struct PathArgs {
   uint32_t t_cap;
   uint32_t t_depth;   // kMode 2: > 0 records only the hits of radiance ray number t_depth
   int32_t emit_expect;      // queries: path_lane's kMode 3 instead of kMode 1
+  int32_t nee_loops;        // kMode 3: NEE loops averaged per vertex (>= 1)
   const float* emit_cdf;    // area prefix sums of the emitting triangles (emitter tree order)
   float emit_area;          // their total area
   F3 bmin, binv;      // position normalisation for the features
@@ -500,6 +501,9 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   bool cond = false;
   int32_t rejects = 0, ls_k = 0;  // ls_k: the light sample's emitter triangle
   float lw = 0.0f;
+  // kMode 3 with nee_loops > 1: the vertex's entry light and flag (light * 2
+  // + flag) and the NEE runs done
+  int32_t ent = 0, nl = 0;
   // accum_color += trace(...) (shader.cu:154) and the per-sample records.
   auto finish_sample = [&]() {
     if (kMode == 2 && t_used > 0) {
@@ -566,53 +570,18 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // End of one shoot_ray_to_light: add its light, sample the BSDF for the
   // next bounce (shader.cu:251-252), end the sample after the last bounce.
   auto end_nee = [&](F3 nee) -> bool {
-    color = color + nee * att;
-    if (kEmit && rough == 1.0f && scp->emit.T > 0 && b < pp->bounces) {
-      // Expected emission of this vertex's bounce ray.  The reference adds
-      // Le * att when the bounce (a hemisphere draw at roughness 1, density
-      // q = 1 / (12 max|w_i|^3) on N's side) hits an emitter, and ends the
-      // path.  Here one light sample estimates p = P(the bounce's closest
-      // hit emits) and a = E[Le; hit] without bias: p^ = V q |cos_y| A / r^2,
-      // a^ = Le p^; the path adds a^ * att, continues with att * (1 - p^),
-      // and its bounce is drawn again whenever it hits an emitter (a draw on
-      // the condition that it does not).  The expectation is the reference's.
-      const float u0 = rnd(seed), u1 = rnd(seed), u2 = rnd(seed);
-      const int32_t T = scp->emit.T;
-      const float ta = u0 * pp->emit_area;
-      int32_t lo = 0, hi = T - 1;  // first triangle whose prefix sum exceeds ta
-      while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (ta < pp->emit_cdf[mid])
-          hi = mid;
-        else
-          lo = mid + 1;
-      }
-      const float4* tp = scp->emit.tris + 3 * (size_t)lo;
-      const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-      const F3 v0 = xyz(p0), v1 = xyz(p1), v2 = xyz(p2);
-      const float su = sqrtf(u1);
-      const F3 yl = v0 * (1.0f - su) + v1 * (su * (1.0f - u2)) + v2 * (su * u2);
-      const F3 wv = yl - O;
-      const float r2 = dot(wv, wv);
-      const F3 w = wv * (1.0f / sqrtf(r2));
-      if (r2 > 0.0f && dot(N, w) > 0.0f && ray_valid(w)) {
-        const F3 e1 = v1 - v0, e2 = v2 - v0;
-        const F3 nt = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
-        const float cy = fabsf(dot(nt, w)) / sqrtf(dot(nt, nt));
-        const float mw = fmaxf(fabsf(w.x), fmaxf(fabsf(w.y), fabsf(w.z)));
-        const float q = 1.0f / (12.0f * mw * mw * mw);
-        lw = q * cy * pp->emit_area / r2;
-        ls_k = lo;
-        D = w;
-        qstage = 4;  // the emitter query after the NEE loop: is lo the closest emitter?
-        return true;
-      }
-      D = Bsdf::sample(Din, N, seed, rough);  // p^ = 0
-      cond = true;
-      rejects = 0;
-      qstage = 0;
+    if (kEmit) {
+      // kMode 3: the light of one NEE run (the mean of nee_loops runs, each
+      // added as it ends); the rest of the vertex follows the NEE loop below
+      // (qstage 5), in one place for every way a run ends
+      if (pp->nee_loops > 1)
+        color = color + (nee * (1.0f / (float)pp->nee_loops)) * att;
+      else
+        color = color + nee * att;
+      qstage = 5;
       return true;
     }
+    color = color + nee * att;
     D = Bsdf::sample(Din, N, seed, rough);
     qstage = 0;
     if (b == pp->bounces) {
@@ -713,6 +682,72 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       if (!alive) break;
     }
     if (!alive) continue;  // the pixel is written at the top of the loop
+    if (kEmit && qstage == 5) {
+      // kMode 3, the end of a vertex's NEE run: another run from the
+      // vertex's entry flag and light (the last run's flag and light stay),
+      // then the light sample or the bounce
+      if (++nl < pp->nee_loops) {
+        hitf = (ent & 1) != 0;
+        light_mat = ent >> 1;
+        tries = 0;
+        D = hemisphere_raw(seed);
+        qstage = 1;
+        continue;
+      }
+      if (rough == 1.0f && scp->emit.T > 0 && b < pp->bounces) {
+        // Expected emission of this vertex's bounce ray.  The reference adds
+        // Le * att when the bounce (a hemisphere draw at roughness 1, density
+        // q = 1 / (12 max|w_i|^3) on N's side) hits an emitter, and ends the
+        // path.  Here one light sample estimates p = P(the bounce's closest
+        // hit emits) and a = E[Le; hit] without bias: p^ = V q |cos_y| A / r^2,
+        // a^ = Le p^; the path adds a^ * att, continues with att * (1 - p^),
+        // and its bounce is drawn again whenever it hits an emitter (a draw on
+        // the condition that it does not).  The expectation is the reference's.
+        const float u0 = rnd(seed), u1 = rnd(seed), u2 = rnd(seed);
+        const int32_t T = scp->emit.T;
+        const float ta = u0 * pp->emit_area;
+        int32_t lo = 0, hi = T - 1;  // first triangle whose prefix sum exceeds ta
+        while (lo < hi) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (ta < pp->emit_cdf[mid])
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        const float4* tp = scp->emit.tris + 3 * (size_t)lo;
+        const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        const F3 v0 = xyz(p0), v1 = xyz(p1), v2 = xyz(p2);
+        const float su = sqrtf(u1);
+        const F3 yl = v0 * (1.0f - su) + v1 * (su * (1.0f - u2)) + v2 * (su * u2);
+        const F3 wv = yl - O;
+        const float r2 = dot(wv, wv);
+        const F3 w = wv * (1.0f / sqrtf(r2));
+        if (r2 > 0.0f && dot(N, w) > 0.0f && ray_valid(w)) {
+          const F3 e1 = v1 - v0, e2 = v2 - v0;
+          const F3 nt = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+          const float cy = fabsf(dot(nt, w)) / sqrtf(dot(nt, nt));
+          const float mw = fmaxf(fabsf(w.x), fmaxf(fabsf(w.y), fabsf(w.z)));
+          const float q = 1.0f / (12.0f * mw * mw * mw);
+          lw = q * cy * pp->emit_area / r2;
+          ls_k = lo;
+          D = w;
+          qstage = 4;  // the emitter query below: is lo the closest emitter?
+        } else {
+          D = Bsdf::sample(Din, N, seed, rough);  // p^ = 0
+          cond = true;
+          rejects = 0;
+          qstage = 0;
+        }
+      } else {
+        D = Bsdf::sample(Din, N, seed, rough);
+        qstage = 0;
+        if (b == pp->bounces) {
+          finish_sample();
+          alive = begin_sample();
+          if (!alive) continue;
+        }
+      }
+    }
     if (kEmit && qstage == 4) {
       // kMode 3's light sample: is the sampled triangle the closest emitter
       // along the line?  Then its any-hit test (qstage 3), else its bounce.
@@ -837,6 +872,10 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
             rough = m.rough_n.x;
             qstage = 1;
             tries = 0;
+            if (kEmit) {  // the NEE loops of this vertex start from this flag and light
+              ent = (light_mat << 1) | (hitf ? 1 : 0);
+              nl = 0;
+            }
             D = hemisphere_raw(seed);
           }
         }
@@ -1891,6 +1930,7 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
               1.0f / std::max(ctx->bmax[2] - ctx->bmin[2], 1e-20f));
   a.fp64 = 0;
   a.emit_expect = 0;
+  a.nee_loops = 1;
   a.emit_cdf = ctx->c_ecdf.p;
   a.emit_area = ctx->emit_area;
   return a;
@@ -2525,6 +2565,18 @@ lisa_status lisa_batch_autotune(lisa_batch* b, int32_t tune_spp, lisa_batch_conf
   return LISA_OK;
 }
 
+// The prefix-path settings of a frame's cache parameters.
+lisa_status prefix_args(lisa_ctx* ctx, const lisa_cache_frame_params* cp, PathArgs& a) {
+  if (cp->prefix_bounces > 63) return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 0..63");
+  if (cp->nee_loops > 16) return fail(ctx, LISA_ERR_INVALID_ARG, "nee_loops must be in 0..16");
+  const bool emit = (cp->flags & LISA_CACHE_EMISSION) != 0;
+  if (cp->nee_loops > 1 && !emit)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "nee_loops > 1 needs LISA_CACHE_EMISSION");
+  a.emit_expect = emit ? 1 : 0;
+  a.nee_loops = cp->nee_loops > 1 ? (int32_t)cp->nee_loops : 1;
+  return LISA_OK;
+}
+
 lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_render_params* p,
                                const lisa_cache_frame_params* cp, lisa_cache_frame_stats* stats) {
   if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
@@ -2532,7 +2584,9 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
   if (p->flags & LISA_FLAG_COUNT) return fail(ctx, LISA_ERR_UNSUPPORTED, "no counters in cached renders");
-  if (cp->prefix_bounces > 63) return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 0..63");
+  PathArgs pre;
+  rc = prefix_args(ctx, cp, pre);
+  if (rc != LISA_OK) return rc;
   if (cp->train_paths > 0 && (cp->train_bounces == 0 || cp->train_bounces > 64))
     return fail(ctx, LISA_ERR_INVALID_ARG, "train_bounces must be in 1..64");
   const size_t npx = (size_t)p->width * p->height;
@@ -2615,7 +2669,8 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   } else {
     uint32_t nq = 0;
     PathArgs qa = base;
-    qa.emit_expect = (cp->flags & LISA_CACHE_EMISSION) ? 1 : 0;
+    qa.emit_expect = pre.emit_expect;
+    qa.nee_loops = pre.nee_loops;
     rc = trace_queries(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, &nq,
                        (cp->flags & LISA_CACHE_PERSISTENT) != 0);
     if (rc != LISA_OK) return rc;
@@ -2657,13 +2712,17 @@ lisa_status upload_pixels(lisa_ctx* ctx, const lisa_render_params* p, const int3
 }
 }  // namespace
 
-lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p, uint32_t prefix_bounces,
-                                     const int32_t* pixels, int32_t n, float* accum_out, float* q_feat,
-                                     float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries) {
+lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
+                                     const lisa_cache_frame_params* cp, const int32_t* pixels, int32_t n,
+                                     float* accum_out, float* q_feat, float* q_att, int32_t* q_item,
+                                     int32_t q_cap, int32_t* n_queries) {
   if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!cp) return fail(ctx, LISA_ERR_INVALID_ARG, "cache params NULL");
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
-  if (prefix_bounces > 63) return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 0..63");
+  PathArgs pre;
+  rc = prefix_args(ctx, cp, pre);
+  if (rc != LISA_OK) return rc;
   if (!accum_out || !n_queries || q_cap < 0 || (q_cap > 0 && (!q_feat || !q_att || !q_item)))
     return fail(ctx, LISA_ERR_INVALID_ARG, "output arrays");
   if ((size_t)n * p->spp >= (1ull << 31)) return fail(ctx, LISA_ERR_INVALID_ARG, "n * spp >= 2^31");
@@ -2672,9 +2731,10 @@ lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
   if (rc != LISA_OK) return rc;
   HIP_TRY(ctx, ctx->c_frame.ensure((size_t)n));
   PathArgs a = cache_args(ctx, p);
-  a.emit_expect = (p->flags & LISA_FLAG_CACHE_EMISSION) ? 1 : 0;
+  a.emit_expect = pre.emit_expect;
+  a.nee_loops = pre.nee_loops;
   uint32_t nq = 0;
-  rc = trace_queries(ctx, a, ctx->c_tpix.p, (size_t)n, prefix_bounces, ctx->c_frame.p, &nq);
+  rc = trace_queries(ctx, a, ctx->c_tpix.p, (size_t)n, cp->prefix_bounces, ctx->c_frame.p, &nq);
   if (rc != LISA_OK) return rc;
   HIP_TRY(ctx, hipMemcpy(accum_out, ctx->c_frame.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
   const size_t k = std::min<size_t>(nq, (size_t)q_cap);

@@ -769,6 +769,7 @@ struct Tracer {
   // bounce in expectation and the bounce is drawn on the condition that it
   // hits no emitter (cond; din and rough_v are the vertex's for redraws)
   bool emit_expect = false;
+  int32_t nee_loops = 1;  // with emit_expect: NEE runs averaged per vertex
   bool cond = false;
   int32_t rejects = 0, max_b = 0;
   V3 din{0, 0, 0};
@@ -929,8 +930,22 @@ struct Tracer {
       }
     } else {
       st.att = st.att * ld3(m.diffuse);
-      const V3 nee = shoot_ray_to_light(st);
-      st.color = st.color + nee * st.att;
+      if (emit_expect && nee_loops > 1) {
+        // the mean of nee_loops NEE runs from the vertex's entry flag and
+        // light, each added as it ends; the last run's flag and light stay
+        const bool e_hit = st.hit;
+        const int32_t e_light = st.light_mat;
+        const float inv = 1.0f / (float)nee_loops;
+        for (int32_t k = 0; k < nee_loops; ++k) {
+          st.hit = e_hit;
+          st.light_mat = e_light;
+          const V3 nee = shoot_ray_to_light(st);
+          st.color = st.color + (nee * inv) * st.att;
+        }
+      } else {
+        const V3 nee = shoot_ray_to_light(st);
+        st.color = st.color + nee * st.att;
+      }
       if (emit_expect && m.roughness == 1.0f && s.emit.T > 0 && rec.bounces < max_b) {
         emit_sample(st);
         cond = true;
@@ -1198,9 +1213,10 @@ void orc_cache_bounds(const orc_scene* o, float* bmin, float* binv) {
 int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t prefix_bounces,
                       const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
                       float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads,
-                      int emit_expect) {
+                      int emit_expect, int nee_loops) {
   const Scene& s = o->s;
   if (p->width == 0 || p->height == 0 || p->spp == 0 || n <= 0) return -1;
+  if (nee_loops > 16 || (nee_loops > 1 && !emit_expect)) return -3;
   const int64_t npx = (int64_t)p->width * p->height;
   for (int32_t i = 0; i < n; ++i)
     if (pixel_ids[i] < 0 || pixel_ids[i] >= npx) return -2;
@@ -1229,6 +1245,7 @@ int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t p
         tr.cache_at = (int32_t)prefix_bounces + 1;
         tr.box = &box;
         tr.emit_expect = emit_expect != 0;
+        tr.nee_loops = nee_loops > 1 ? nee_loops : 1;
         const V3 c = tr.trace(eye, D, (int)prefix_bounces + 1);
         accum = accum + c;
         if (tr.queried) {
