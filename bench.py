@@ -410,7 +410,9 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
            "config": {"workload": f"{sc['name']} learned cache frames", "width": W, "height": H,
                       "bounces": B, "cache_prefix_bounces": a.prefix,
                       "cache_spp_per_frame": a.cache_spp, "train_paths": a.train_paths,
-                      "train_steps": a.train_steps, "parallelism": "single GPU"},
+                      "train_steps": a.train_steps, "cache_emission": a.cache_emission,
+                      "nee_loops": a.nee_loops if a.cache_emission else 1,
+                      "parallelism": "single GPU"},
            "frame": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in med.items()},
            "mlp_roofline": mlp_roofline(med, a.train_steps), "roofline": None,
            "cpu_baseline": None}
