@@ -461,21 +461,24 @@ def render_cached(ctx, cache, params, cparams):
     return {k: getattr(s, k) for k, _ in CacheFrameStats._fields_}
 
 
-def cache_trace_queries(ctx, params, cparams, pixels):
+def cache_trace_queries(ctx, params, cparams, pixels, cap=None):
     """Prefix paths on a pixel list (lisa_cache_trace_queries) with the prefix settings of
-    cparams (prefix_bounces, CACHE_EMISSION, nee_loops): accum, features, throughput, item."""
+    cparams (prefix_bounces, CACHE_EMISSION, nee_loops): accum, features, throughput, item
+    (at most `cap` queries copied back, default all; 0: the per-pixel accum only)."""
     px = np.ascontiguousarray(pixels, dtype=np.int32)
     n = px.shape[0]
-    cap = n * params.spp
+    cap = n * params.spp if cap is None else cap
     acc = np.zeros((n, 4), np.float32)
-    f = np.zeros((cap, CACHE_INPUTS), np.float32)
-    at = np.zeros((cap, 3), np.float32)
-    it = np.zeros(cap, np.int32)
+    f = np.zeros((cap, CACHE_INPUTS), np.float32) if cap > 0 else None
+    at = np.zeros((cap, 3), np.float32) if cap > 0 else None
+    it = np.zeros(cap, np.int32) if cap > 0 else None
     nq = C.c_int32()
     ctx._ck(lib().lisa_cache_trace_queries(ctx.h, C.byref(params), C.byref(cparams), _p(px, C.c_int32), n,
                                            _p(acc, C.c_float), _p(f, C.c_float), _p(at, C.c_float),
                                            _p(it, C.c_int32), cap, C.byref(nq)))
     k = min(nq.value, cap)
+    if cap == 0:
+        return {"accum": acc, "feat": None, "att": None, "item": None, "n": nq.value}
     return {"accum": acc, "feat": f[:k], "att": at[:k], "item": it[:k], "n": nq.value}
 
 

@@ -1022,8 +1022,8 @@ struct PixelQueue {
 // `total` work indices (pixels of the list, or 64 per 8x8 tile) handed out
 // by `counter`, which must be zero at launch.
 template <int kMode>
-__global__ __launch_bounds__(kPathBlock) void k_path_pq(SceneView sc, PathArgs p, uint32_t* counter,
-                                                        uint32_t total, uint32_t chunk) {
+__global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : 1) void k_path_pq(
+    SceneView sc, PathArgs p, uint32_t* counter, uint32_t total, uint32_t chunk) {
   __shared__ uint32_t stk[kStackLds * kPathBlock];
   PixelQueue src{sc, p, counter, total, chunk, 0u, 0u};
   path_lane<false, false, PixelQueue, kMode>(src, stk + threadIdx.x);

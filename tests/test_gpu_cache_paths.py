@@ -209,3 +209,43 @@ def test_persistent_cached_frames_same_image(capi, oracle_mod, ctx, emission):
         other.close()
         a.close()
         b.close()
+
+
+def test_variance_reduced_prefix_on_the_gpu(capi, oracle_mod, ctx):
+    """The prefix radiance of a cached frame (the image with a cache that
+    answers 0) from the GPU's two prefix forms on a small island: the
+    variance-reduced prefix (LISA_CACHE_EMISSION, 3 NEE runs) converges to
+    the plain prefix's mean (paired over pixels at 4096 spp, |z| < 4) and at
+    64 spp lands much closer to it (RMSE below 0.75x the plain prefix's)."""
+    from lisa_amd import scenes
+    W, H = 192, 108
+    sc = scenes.island(n=256, width=W, height=H)
+    other = capi.Context(0)
+    try:
+        other.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        cam = sc["camera"]
+        U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+        px = scenes.seeded_pixels(W, H, 2048, seed=3)
+        plain = capi.cache_frame_params(prefix_bounces=2)
+        vr = capi.cache_frame_params(prefix_bounces=2, flags=capi.CACHE_EMISSION, nee_loops=3)
+
+        def prefix(cp, spp, sub, calls=1):
+            acc = np.zeros((px.size, 3))
+            for c in range(calls):
+                p = capi.render_params(W, H, spp, 3, cam["eye"], U, V, Wv, subframe=sub + c)
+                acc += capi.cache_trace_queries(other, p, cp, px, cap=0)["accum"][:, :3] / calls
+            return acc.sum(1)
+
+        ref = prefix(plain, 1024, 1000, 4)
+        hi = prefix(vr, 1024, 2000, 4)
+        d = hi - ref
+        z = d.mean() / (d.std() / np.sqrt(d.size))
+        e_plain = float(np.sqrt(np.mean((prefix(plain, 64, 7) - ref) ** 2)))
+        e_vr = float(np.sqrt(np.mean((prefix(vr, 64, 7) - ref) ** 2)))
+        print(f"prefix mean plain {ref.mean():.5f} reduced {hi.mean():.5f} z {z:+.2f}; "
+              f"RMSE at 64 spp: plain {e_plain:.5f} reduced {e_vr:.5f}", flush=True)
+        assert abs(z) < 4.0
+        assert e_vr < 0.75 * e_plain
+    finally:
+        other.close()
+
