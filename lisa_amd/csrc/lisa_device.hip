@@ -2584,7 +2584,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
   if (p->flags & LISA_FLAG_COUNT) return fail(ctx, LISA_ERR_UNSUPPORTED, "no counters in cached renders");
-  PathArgs pre;
+  PathArgs pre{};  // only emit_expect and nee_loops are read
   rc = prefix_args(ctx, cp, pre);
   if (rc != LISA_OK) return rc;
   if (cp->train_paths > 0 && (cp->train_bounces == 0 || cp->train_bounces > 64))
@@ -2720,7 +2720,7 @@ lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
   if (!cp) return fail(ctx, LISA_ERR_INVALID_ARG, "cache params NULL");
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
-  PathArgs pre;
+  PathArgs pre{};  // only emit_expect and nee_loops are read
   rc = prefix_args(ctx, cp, pre);
   if (rc != LISA_OK) return rc;
   if (!accum_out || !n_queries || q_cap < 0 || (q_cap > 0 && (!q_feat || !q_att || !q_item)))
