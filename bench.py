@@ -13,7 +13,10 @@ counts from the counting kernel, which is bit-identical in work to the
 timed one) per second of wall time over the K timed steps.  `roofline` uses
 the average kernel duration from HIP events recorded on the context's own
 stream; `cpu_baseline` times the CPU oracle on a seeded pixel subset of the
-same workload on this host's cores.
+same workload on this host's cores.  The default run also times the other
+half of the metric, the learned cache against the plain tracer to RMSE <=
+1e-2 on the island (`bench_rmse`), and reports it under `cache_vs_plain`
+(`--no-cache-leg` skips it; `--workload island` prints it on its own).
 """
 from __future__ import annotations
 
@@ -77,6 +80,9 @@ def parse():
                     help="with --cache-emission: NEE runs averaged per prefix vertex")
     ap.add_argument("--compare-emission", action="store_true",
                     help="island / cornell_cache: also time the cache with --cache-emission flipped")
+    ap.add_argument("--cache-leg", action=argparse.BooleanOptionalAction, default=True,
+                    help="default workload: also time the island's cache against plain "
+                         "(time to RMSE <= 1e-2) and report it under cache_vs_plain")
     ap.add_argument("--cache-frames", type=int, default=0,
                     help="island / cornell_cache: only run this many cached frames (no "
                          "reference, no plain sweep) and report their timings, for profiling")
@@ -196,8 +202,21 @@ def main():
            "samples_per_s": round(W * H * spp * a.steps / wall, 1),
            "rays_per_step": rays, "bvh_build_ms": round(bvh_ms, 3),
            "roofline": roofline, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if a.workload == "cornell65k" and a.cache_leg:
+        # the metric's second half: time to RMSE <= 1e-2 on the island, cache vs plain
+        try:
+            r = bench_rmse(a, workload="island", emit=False)
+            out["cache_vs_plain"] = {
+                k: r[k] for k in ("metric", "plain_time_fit_s", "plain_progressive_time_s",
+                                  "plain_time_s", "speedup_vs_plain_fit",
+                                  "speedup_vs_plain_progressive", "speedup_vs_plain",
+                                  "cache_frames", "cache_rmse", "config", "mlp_roofline")}
+            out["cache_vs_plain"]["cache_time_s"] = r["value"]
+        except Exception as e:  # the plain line is still reported
+            out["cache_vs_plain"] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] island leg failed: {e!r}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
 
 
 RMSE_TARGET = 1e-2
@@ -206,8 +225,10 @@ MLP_WEIGHTS = 21696  # 144*64 + 3*64*64 + 64*3 (no biases)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def bench_rmse(a):
+def bench_rmse(a, workload=None, emit=True):
     """Time to RMSE <= 1e-2 against a 16k-spp plain render: learned cache vs plain.
+    Returns the result line (printed unless emit is False); `workload`
+    overrides `--workload`.
 
     Reference: 16 x 1024 spp with FP64 accumulation on 65536 seeded pixels
     (the plain loop on a pixel list; subframes 1000..1015 seed it apart from
@@ -229,10 +250,11 @@ def bench_rmse(a):
     import numpy as np
     from lisa_amd import capi, scenes
 
-    factory = WORKLOADS[a.workload][0]
+    workload = workload or a.workload
+    factory = WORKLOADS[workload][0]
     sc = getattr(scenes, factory)()
     W, H = sc["width"], sc["height"]
-    B = 12 if a.workload == "island" else sc["bounces"]
+    B = 12 if workload == "island" else sc["bounces"]
     cam = sc["camera"]
     U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
     ctx = capi.Context(0)
@@ -351,8 +373,10 @@ def bench_rmse(a):
            "reference": {"spp": 16384, "pixels": REF_PIXELS, "fp64": True,
                          "seconds": round(ref_s, 2)},
            "last_frame": last, "mlp_roofline": mlp, "roofline": None, "cpu_baseline": None}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
     ctx.close()
+    return out
 
 
 def mlp_roofline(st, train_steps):
