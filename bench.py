@@ -83,6 +83,10 @@ def parse():
     ap.add_argument("--cache-leg", action=argparse.BooleanOptionalAction, default=True,
                     help="default workload: also time the island's cache against plain "
                          "(time to RMSE <= 1e-2) and report it under cache_vs_plain")
+    ap.add_argument("--cache-overlap", action="store_true",
+                    help="--cache-frames: train beside the frame's paths as the time-to-RMSE "
+                         "runs do (default there: train first, so the training kernels are "
+                         "timed alone)")
     ap.add_argument("--cache-frames", type=int, default=0,
                     help="island / cornell_cache: only run this many cached frames (no "
                          "reference, no plain sweep) and report their timings, for profiling")
@@ -339,7 +343,7 @@ def bench_rmse(a, workload=None, emit=True):
                  "time_s": None if o_t is None else round(o_t, 4), "frames": o_frames,
                  "rmse": round(o_e, 6)}
     cache_t, tot, frames, last, e_cache = run_cache(cache_flags(a, capi))
-    mlp = mlp_roofline(last, a.train_steps)
+    mlp = mlp_roofline(last, a.train_steps, overlapped=True)
     fit_t = None
     above = [r for r in sweep if r["rmse"] > RMSE_TARGET]
     if plain_t is not None and above:
@@ -379,10 +383,12 @@ def bench_rmse(a, workload=None, emit=True):
     return out
 
 
-def mlp_roofline(st, train_steps):
+def mlp_roofline(st, train_steps, overlapped=False):
     """MFMA roofline of the cache MLP from one frame's HIP-event kernel times.
     Algorithmic FLOPs: 2 per real weight per query forward; 3x that per
-    training row (forward, backward data pass, weight gradients)."""
+    training row (forward, backward data pass, weight gradients).  Training
+    that ran beside the frame's paths has no kernel time of its own (its
+    events include the wait for the paths), so it is left out then."""
     if st is None or st["queries"] <= 0 or st["infer_kernel_ms"] <= 0:
         return None
     f_q = 2 * MLP_WEIGHTS
@@ -391,7 +397,7 @@ def mlp_roofline(st, train_steps):
            "infer_queries": st["queries"], "infer_kernel_ms": round(st["infer_kernel_ms"], 4),
            "infer_achieved": round(inf_tf, 2),
            "infer_frac": round(inf_tf / MFMA_BF16_PEAK_TFLOPS, 5)}
-    if st["train_kernel_ms"] > 0:
+    if st["train_kernel_ms"] > 0 and not overlapped:
         tr_tf = (st["train_samples"] * 3 * f_q * train_steps / (st["train_kernel_ms"] * 1e-3)
                  / 1e12)
         out.update({"train_rows": st["train_samples"], "train_steps": train_steps,
@@ -401,9 +407,10 @@ def mlp_roofline(st, train_steps):
     return out
 
 
-def cache_flags(a, capi):
+def cache_flags(a, capi, serial=False):
     return ((capi.CACHE_PERSISTENT if a.cache_persistent else 0)
-            | (capi.CACHE_EMISSION if a.cache_emission else 0))
+            | (capi.CACHE_EMISSION if a.cache_emission else 0)
+            | (capi.CACHE_SERIAL if serial else 0))
 
 
 def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
@@ -411,9 +418,10 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
     wall time and the MLP's MFMA roofline (kernel times from HIP events)."""
     cam = sc["camera"]
     cache = capi.Cache(0, seed=1, lr=1e-3)
+    serial = not a.cache_overlap  # by default the training kernels are timed alone here
     cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                  train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
-                                 seed=3, flags=cache_flags(a, capi),
+                                 seed=3, flags=cache_flags(a, capi, serial),
                                  nee_loops=a.nee_loops if a.cache_emission else 1)
     stats = []
     for f in range(a.warmup + a.cache_frames):
@@ -435,10 +443,11 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
                       "bounces": B, "cache_prefix_bounces": a.prefix,
                       "cache_spp_per_frame": a.cache_spp, "train_paths": a.train_paths,
                       "train_steps": a.train_steps, "cache_emission": a.cache_emission,
+                      "training_beside_paths": not serial,
                       "nee_loops": a.nee_loops if a.cache_emission else 1,
                       "parallelism": "single GPU"},
            "frame": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in med.items()},
-           "mlp_roofline": mlp_roofline(med, a.train_steps), "roofline": None,
+           "mlp_roofline": mlp_roofline(med, a.train_steps, overlapped=not serial), "roofline": None,
            "cpu_baseline": None}
     print(json.dumps(out), flush=True)
     cache.close()

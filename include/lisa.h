@@ -221,6 +221,12 @@ typedef struct lisa_cache_frame_params {
  * runs of the reference's NEE loop from the vertex's hit flag and light; the
  * last run's flag and light go on down the path (same expectation). */
 #define LISA_CACHE_EMISSION 2u
+/* Train before the frame's paths instead of beside them.  Without a
+ * fallback_loss the frame's prefix paths (which do not read the network) run
+ * while the training steps run on the cache's stream; then train_ms and
+ * train_kernel_ms include the wait for the paths.  This flag keeps the two
+ * apart (for timing the training kernels alone). */
+#define LISA_CACHE_SERIAL 8u
 
 typedef struct lisa_cache_frame_stats {
   float trace_train_ms, train_ms, render_ms, infer_ms, total_ms;  /* host wall time */

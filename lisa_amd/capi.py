@@ -429,6 +429,7 @@ CACHE_INPUTS = 12
 
 CACHE_PERSISTENT = 1
 CACHE_EMISSION = 2  # prefix bounces' emission in expectation (same mean, less variance)
+CACHE_SERIAL = 8  # train before the frame's paths (training kernels timed alone)
 
 
 class CacheFrameParams(C.Structure):

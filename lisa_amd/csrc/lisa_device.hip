@@ -1971,9 +1971,11 @@ lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, si
 
 // Prefix paths (path loop kMode 1, or 3 with a.emit_expect): the full image in tiles (d_pix null) or
 // n listed device pixels; radiance gathered before the cache into `frame`
-// (per pixel mean), queries into ctx->c_q* (capacity n * spp); *nq = queries.
-lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, uint32_t prefix,
-                          float4* frame, uint32_t* nq, bool persistent = false) {
+// (per pixel mean), queries into ctx->c_q* (capacity n * spp).  The launch
+// returns without waiting; trace_queries_finish waits for it, reads the
+// query count (*nq) and turns the query records into features.
+lisa_status trace_queries_launch(lisa_ctx* ctx, PathArgs& a, const int32_t* d_pix, size_t n, uint32_t prefix,
+                                 float4* frame, bool persistent) {
   hipStream_t st = ctx->stream;
   const size_t qcap = n * a.spp;
   HIP_TRY(ctx, ctx->c_qfeat.ensure(qcap * 12));
@@ -2010,15 +2012,27 @@ lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_
       k_path_mode<1><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
     HIP_TRY(ctx, hipGetLastError());
   }
+  return LISA_OK;
+}
+
+lisa_status trace_queries_finish(lisa_ctx* ctx, const PathArgs& a, uint32_t* nq) {
+  hipStream_t st = ctx->stream;
   HIP_TRY(ctx, hipMemcpyAsync(nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
-  const uint32_t nf = std::min<uint32_t>(*nq, (uint32_t)qcap);
+  const uint32_t nf = std::min<uint32_t>(*nq, a.q_cap);
   if (nf > 0) {
     k_cache_features<<<(nf + 255) / 256, 256, 0, st>>>(ctx->c_qfeat.p, nf, a.bmin, a.binv, ctx->d_mats);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipStreamSynchronize(st));  // the network reads them on the cache's stream
   }
   return LISA_OK;
+}
+
+lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, uint32_t prefix,
+                          float4* frame, uint32_t* nq, bool persistent = false) {
+  lisa_status rc = trace_queries_launch(ctx, a, d_pix, n, prefix, frame, persistent);
+  if (rc != LISA_OK) return rc;
+  return trace_queries_finish(ctx, a, nq);
 }
 
 extern "C" {
@@ -2608,6 +2622,16 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   }
   PathArgs base = cache_args(ctx, p);
   float loss = 0.0f;
+  HIP_TRY(ctx, ctx->c_frame.ensure(npx));
+  // The frame's prefix paths do not read the network, so without a fallback
+  // decision (which waits for this frame's loss) they run on the context's
+  // stream while the training steps run on the cache's.
+  const bool overlap = !(cp->fallback_loss > 0.0f) && !(cp->flags & LISA_CACHE_SERIAL);
+  PathArgs qa = base;
+  qa.emit_expect = pre.emit_expect;
+  qa.nee_loops = pre.nee_loops;
+  const bool persistent = (cp->flags & LISA_CACHE_PERSISTENT) != 0;
+  bool launched = false;
   // 1. online training on suffixes of full-length paths traced by the plain loop
   if (cp->train_paths > 0 && cp->train_steps > 0) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -2634,6 +2658,11 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     if (rc != LISA_OK) return rc;
     rows = std::min<uint32_t>(rows, (uint32_t)cap);
     const auto t1 = std::chrono::steady_clock::now();
+    if (overlap) {
+      rc = trace_queries_launch(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, persistent);
+      if (rc != LISA_OK) return rc;
+      launched = true;
+    }
     if (rows > 0) {
       lisa_status r2 = lisa::cache_train_device(cache, ctx->c_tfeat.p, 12, ctx->c_ttgt.p, (int64_t)rows,
                                                 &loss, (int)cp->train_steps);
@@ -2647,8 +2676,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   }
   out_stats.loss = loss;
   // 2. this frame: cached paths, or the plain tracer as the fallback
-  HIP_TRY(ctx, ctx->c_frame.ensure(npx));
-  const bool fallback = cp->fallback_loss > 0.0f && loss > cp->fallback_loss;
+  const bool fallback = !launched && cp->fallback_loss > 0.0f && loss > cp->fallback_loss;
   const auto t3 = std::chrono::steady_clock::now();
   if (fallback) {
     PathArgs a = base;
@@ -2668,11 +2696,11 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     out_stats.fell_back = 1;
   } else {
     uint32_t nq = 0;
-    PathArgs qa = base;
-    qa.emit_expect = pre.emit_expect;
-    qa.nee_loops = pre.nee_loops;
-    rc = trace_queries(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, &nq,
-                       (cp->flags & LISA_CACHE_PERSISTENT) != 0);
+    if (!launched) {
+      rc = trace_queries_launch(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, persistent);
+      if (rc != LISA_OK) return rc;
+    }
+    rc = trace_queries_finish(ctx, qa, &nq);
     if (rc != LISA_OK) return rc;
     nq = std::min<uint32_t>(nq, (uint32_t)(npx * p->spp));
     const auto t4 = std::chrono::steady_clock::now();
