@@ -694,7 +694,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
         qstage = 1;
         continue;
       }
-      if (rough == 1.0f && scp->emit.T > 0 && b < pp->bounces) {
+      if (rough == 1.0f && scp->emit.T > 0 && pp->emit_area > 0.0f && b < pp->bounces) {
         // Expected emission of this vertex's bounce ray.  The reference adds
         // Le * att when the bounce (a hemisphere draw at roughness 1, density
         // q = 1 / (12 max|w_i|^3) on N's side) hits an emitter, and ends the
@@ -1899,6 +1899,7 @@ lisa_status read_scene_box(lisa_ctx* ctx) {
   float rb[16];
   HIP_TRY(ctx, hipMemcpy(rb, ctx->all.nodes, 64, hipMemcpyDeviceToHost));
   root_box(rb, ctx->bmin, ctx->bmax);
+  ctx->emit_area = 0.0f;  // no light samples unless the emitters below have area
   if (ctx->emit.T <= 0) return LISA_OK;
   float S = 0.0f;
   for (int a = 0; a < 3; ++a) S = std::max(S, std::max(std::fabs(ctx->bmin[a]), std::fabs(ctx->bmax[a])));

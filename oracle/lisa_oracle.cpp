@@ -946,7 +946,7 @@ struct Tracer {
         const V3 nee = shoot_ray_to_light(st);
         st.color = st.color + nee * st.att;
       }
-      if (emit_expect && m.roughness == 1.0f && s.emit.T > 0 && rec.bounces < max_b) {
+      if (emit_expect && m.roughness == 1.0f && s.emit.T > 0 && s.earea > 0.0f && rec.bounces < max_b) {
         emit_sample(st);
         cond = true;
         rejects = 0;

@@ -249,3 +249,23 @@ def test_variance_reduced_prefix_on_the_gpu(capi, oracle_mod, ctx):
     finally:
         other.close()
 
+
+
+def test_emission_prefix_after_the_emitters_are_gone(capi, oracle_mod, ctx):
+    """A scene without emitters set on a context that had some: the
+    estimator has nothing to sample, and its prefix paths (one NEE run)
+    are the plain prefix's bit for bit."""
+    W = H = 32
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)  # with its ceiling light
+    mats = [dict(m, emit=False, emission=(0.0, 0.0, 0.0)) for m in sc["materials"]]
+    ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], mats)
+    px = np.arange(W * H, dtype=np.int32)
+    p = capi.render_params(W, H, 2, 3, sc["camera"]["eye"], U, V, Wv, subframe=1)
+    a = capi.cache_trace_queries(ctx, p, capi.cache_frame_params(prefix_bounces=2), px)
+    b = capi.cache_trace_queries(ctx, p, capi.cache_frame_params(prefix_bounces=2,
+                                                                 flags=capi.CACHE_EMISSION), px)
+    assert a["n"] == b["n"] > 0
+    assert np.array_equal(a["accum"].view(np.uint32), b["accum"].view(np.uint32))
+    ia, ib = _order(a["item"], a["feat"]), _order(b["item"], b["feat"])
+    assert np.array_equal(a["feat"][ia], b["feat"][ib])
+    assert np.array_equal(a["att"][ia], b["att"][ib])
