@@ -62,7 +62,7 @@ def parse():
                          "traversal-bound 5.2M-triangle scene; batch64: the batched "
                          "wavefront form; island / cornell_cache: time to RMSE <= 1e-2, "
                          "learned cache vs plain")
-    ap.add_argument("--cache-spp", type=int, default=6, help="spp per cached frame")
+    ap.add_argument("--cache-spp", type=int, default=4, help="spp per cached frame")
     ap.add_argument("--prefix", type=int, default=2, help="radiance rays before the cache")
     ap.add_argument("--train-paths", type=int, default=262144,
                     help="training paths per frame (their rows at the query depth: ~2^16 on "
@@ -76,8 +76,10 @@ def parse():
                     help="cached frames' paths in one persistent launch (LISA_CACHE_PERSISTENT)")
     ap.add_argument("--cache-emission", action=argparse.BooleanOptionalAction, default=True,
                     help="prefix bounces' emission in expectation (LISA_CACHE_EMISSION)")
-    ap.add_argument("--nee-loops", type=int, default=3,
-                    help="with --cache-emission: NEE runs averaged per prefix vertex")
+    ap.add_argument("--nee-loops", type=int, default=8,
+                    help="with --cache-emission: NEE runs averaged at the camera ray's hit")
+    ap.add_argument("--nee-loops-deep", type=int, default=1,
+                    help="with --cache-emission: NEE runs averaged at later prefix vertices")
     ap.add_argument("--compare-emission", action="store_true",
                     help="island / cornell_cache: also time the cache with --cache-emission flipped")
     ap.add_argument("--cache-leg", action=argparse.BooleanOptionalAction, default=True,
@@ -317,7 +319,8 @@ def bench_rmse(a, workload=None, emit=True):
         cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                      train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
                                      seed=3, flags=flags,
-                                     nee_loops=a.nee_loops if flags & capi.CACHE_EMISSION else 1)
+                                     nee_loops=a.nee_loops if flags & capi.CACHE_EMISSION else 1,
+                                     nee_loops_deep=a.nee_loops_deep if flags & capi.CACHE_EMISSION else 1)
         cache_t, tot, frames, last = None, 0.0, 0, None
         for f in range(1 << 20):
             p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
@@ -362,6 +365,7 @@ def bench_rmse(a, workload=None, emit=True):
                       "persistent_frames": a.cache_persistent,
                       "cache_emission": a.cache_emission,
                       "nee_loops": a.nee_loops if a.cache_emission else 1,
+                      "nee_loops_deep": a.nee_loops_deep if a.cache_emission else 1,
                       "parallelism": "single GPU"},
            "plain_time_s": None if plain_t is None else round(plain_t, 4),
            "plain_progressive_time_s": None if prog_t is None else round(prog_t, 4),
@@ -422,7 +426,8 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
     cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                  train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
                                  seed=3, flags=cache_flags(a, capi, serial),
-                                 nee_loops=a.nee_loops if a.cache_emission else 1)
+                                 nee_loops=a.nee_loops if a.cache_emission else 1,
+                                 nee_loops_deep=a.nee_loops_deep if a.cache_emission else 1)
     stats = []
     for f in range(a.warmup + a.cache_frames):
         p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
@@ -445,6 +450,7 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
                       "train_steps": a.train_steps, "cache_emission": a.cache_emission,
                       "training_beside_paths": not serial,
                       "nee_loops": a.nee_loops if a.cache_emission else 1,
+                      "nee_loops_deep": a.nee_loops_deep if a.cache_emission else 1,
                       "parallelism": "single GPU"},
            "frame": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in med.items()},
            "mlp_roofline": mlp_roofline(med, a.train_steps, overlapped=not serial), "roofline": None,

@@ -208,8 +208,9 @@ typedef struct lisa_cache_frame_params {
   uint32_t train_steps;    /* Adam steps per frame over the frame's rows */
   float fallback_loss;     /* > 0: plain frame when the last training loss exceeds it */
   uint32_t flags;          /* LISA_CACHE_* */
-  uint32_t nee_loops;      /* with LISA_CACHE_EMISSION: NEE loops averaged per prefix vertex
-                              (0 or 1: the reference's one; at most 16) */
+  uint32_t nee_loops;      /* with LISA_CACHE_EMISSION: NEE runs averaged at the camera ray's
+                              hit (0 or 1: the reference's one; at most 16) */
+  uint32_t nee_loops_deep; /* the same at the later prefix vertices */
   uint64_t seed;           /* draws the training pixels */
 } lisa_cache_frame_params;
 #define LISA_CACHE_PERSISTENT 1u /* the frame's paths in a persistent launch */
@@ -217,9 +218,10 @@ typedef struct lisa_cache_frame_params {
  * bounce from one light sample and draw the bounce on the condition that it
  * hits no emitter, weighted by 1 - (the sampled chance that it would): the
  * same expectation as the plain tracer's bounces, with less variance.  With
- * nee_loops = M > 1 the NEE of a prefix vertex is the mean of M independent
- * runs of the reference's NEE loop from the vertex's hit flag and light; the
- * last run's flag and light go on down the path (same expectation). */
+ * nee_loops = M > 1 the NEE at the camera ray's hit is the mean of M
+ * independent runs of the reference's NEE loop from the vertex's hit flag
+ * and light (nee_loops_deep at later prefix vertices); the last run's flag
+ * and light go on down the path (same expectation). */
 #define LISA_CACHE_EMISSION 2u
 /* Train before the frame's paths instead of beside them.  Without a
  * fallback_loss the frame's prefix paths (which do not read the network) run

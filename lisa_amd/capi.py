@@ -436,7 +436,8 @@ class CacheFrameParams(C.Structure):
     _fields_ = [("prefix_bounces", C.c_uint32), ("train_paths", C.c_uint32),
                 ("train_bounces", C.c_uint32), ("train_max", C.c_uint32),
                 ("train_steps", C.c_uint32), ("fallback_loss", C.c_float),
-                ("flags", C.c_uint32), ("nee_loops", C.c_uint32), ("seed", C.c_uint64)]
+                ("flags", C.c_uint32), ("nee_loops", C.c_uint32), ("nee_loops_deep", C.c_uint32),
+                ("seed", C.c_uint64)]
 
 
 class CacheFrameStats(C.Structure):
@@ -447,9 +448,12 @@ class CacheFrameStats(C.Structure):
 
 
 def cache_frame_params(prefix_bounces=2, train_paths=8192, train_bounces=12, train_max=1 << 16,
-                       train_steps=1, fallback_loss=0.0, seed=1, flags=0, nee_loops=1):
+                       train_steps=1, fallback_loss=0.0, seed=1, flags=0, nee_loops=1,
+                       nee_loops_deep=None):
+    """nee_loops_deep defaults to nee_loops (the same NEE runs at every prefix vertex)."""
     return CacheFrameParams(prefix_bounces, train_paths, train_bounces, train_max, train_steps,
-                            fallback_loss, flags, nee_loops, seed)
+                            fallback_loss, flags, nee_loops,
+                            nee_loops if nee_loops_deep is None else nee_loops_deep, seed)
 
 
 def render_cached(ctx, cache, params, cparams):

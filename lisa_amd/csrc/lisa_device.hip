@@ -394,7 +394,8 @@ struct PathArgs {
   uint32_t t_cap;
   uint32_t t_depth;   // kMode 2: > 0 records only the hits of radiance ray number t_depth
   int32_t emit_expect;      // queries: path_lane's kMode 3 instead of kMode 1
-  int32_t nee_loops;        // kMode 3: NEE loops averaged per vertex (>= 1)
+  int32_t nee_loops;        // kMode 3: NEE runs averaged at the camera ray's hit (>= 1)
+  int32_t nee_loops_deep;   // kMode 3: ... at the later vertices (>= 1)
   const float* emit_cdf;    // area prefix sums of the emitting triangles (emitter tree order)
   float emit_area;          // their total area
   F3 bmin, binv;      // position normalisation for the features
@@ -571,11 +572,13 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // next bounce (shader.cu:251-252), end the sample after the last bounce.
   auto end_nee = [&](F3 nee) -> bool {
     if (kEmit) {
-      // kMode 3: the light of one NEE run (the mean of nee_loops runs, each
-      // added as it ends); the rest of the vertex follows the NEE loop below
+      // kMode 3: the light of one NEE run (the mean of nee_loops runs at the
+      // camera ray's hit, nee_loops_deep at later vertices, each added as it
+      // ends); the rest of the vertex follows the NEE loop below
       // (qstage 5), in one place for every way a run ends
-      if (pp->nee_loops > 1)
-        color = color + (nee * (1.0f / (float)pp->nee_loops)) * att;
+      const int32_t runs = b > 1 ? pp->nee_loops_deep : pp->nee_loops;
+      if (runs > 1)
+        color = color + (nee * (1.0f / (float)runs)) * att;
       else
         color = color + nee * att;
       qstage = 5;
@@ -686,7 +689,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       // kMode 3, the end of a vertex's NEE run: another run from the
       // vertex's entry flag and light (the last run's flag and light stay),
       // then the light sample or the bounce
-      if (++nl < pp->nee_loops) {
+      if (++nl < (b > 1 ? pp->nee_loops_deep : pp->nee_loops)) {
         hitf = (ent & 1) != 0;
         light_mat = ent >> 1;
         tries = 0;
@@ -1932,6 +1935,7 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
   a.fp64 = 0;
   a.emit_expect = 0;
   a.nee_loops = 1;
+  a.nee_loops_deep = 1;
   a.emit_cdf = ctx->c_ecdf.p;
   a.emit_area = ctx->emit_area;
   return a;
@@ -2583,12 +2587,14 @@ lisa_status lisa_batch_autotune(lisa_batch* b, int32_t tune_spp, lisa_batch_conf
 // The prefix-path settings of a frame's cache parameters.
 lisa_status prefix_args(lisa_ctx* ctx, const lisa_cache_frame_params* cp, PathArgs& a) {
   if (cp->prefix_bounces > 63) return fail(ctx, LISA_ERR_INVALID_ARG, "prefix_bounces must be in 0..63");
-  if (cp->nee_loops > 16) return fail(ctx, LISA_ERR_INVALID_ARG, "nee_loops must be in 0..16");
+  if (cp->nee_loops > 16 || cp->nee_loops_deep > 16)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "nee_loops and nee_loops_deep must be in 0..16");
   const bool emit = (cp->flags & LISA_CACHE_EMISSION) != 0;
-  if (cp->nee_loops > 1 && !emit)
+  if ((cp->nee_loops > 1 || cp->nee_loops_deep > 1) && !emit)
     return fail(ctx, LISA_ERR_INVALID_ARG, "nee_loops > 1 needs LISA_CACHE_EMISSION");
   a.emit_expect = emit ? 1 : 0;
   a.nee_loops = cp->nee_loops > 1 ? (int32_t)cp->nee_loops : 1;
+  a.nee_loops_deep = cp->nee_loops_deep > 1 ? (int32_t)cp->nee_loops_deep : 1;
   return LISA_OK;
 }
 
@@ -2599,7 +2605,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
   if (p->flags & LISA_FLAG_COUNT) return fail(ctx, LISA_ERR_UNSUPPORTED, "no counters in cached renders");
-  PathArgs pre{};  // only emit_expect and nee_loops are read
+  PathArgs pre{};  // only emit_expect, nee_loops and nee_loops_deep are read
   rc = prefix_args(ctx, cp, pre);
   if (rc != LISA_OK) return rc;
   if (cp->train_paths > 0 && (cp->train_bounces == 0 || cp->train_bounces > 64))
@@ -2631,6 +2637,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   PathArgs qa = base;
   qa.emit_expect = pre.emit_expect;
   qa.nee_loops = pre.nee_loops;
+  qa.nee_loops_deep = pre.nee_loops_deep;
   const bool persistent = (cp->flags & LISA_CACHE_PERSISTENT) != 0;
   bool launched = false;
   // 1. online training on suffixes of full-length paths traced by the plain loop
@@ -2749,7 +2756,7 @@ lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
   if (!cp) return fail(ctx, LISA_ERR_INVALID_ARG, "cache params NULL");
   lisa_status rc = check_params(ctx, p);
   if (rc != LISA_OK) return rc;
-  PathArgs pre{};  // only emit_expect and nee_loops are read
+  PathArgs pre{};  // only emit_expect, nee_loops and nee_loops_deep are read
   rc = prefix_args(ctx, cp, pre);
   if (rc != LISA_OK) return rc;
   if (!accum_out || !n_queries || q_cap < 0 || (q_cap > 0 && (!q_feat || !q_att || !q_item)))
@@ -2762,6 +2769,7 @@ lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
   PathArgs a = cache_args(ctx, p);
   a.emit_expect = pre.emit_expect;
   a.nee_loops = pre.nee_loops;
+  a.nee_loops_deep = pre.nee_loops_deep;
   uint32_t nq = 0;
   rc = trace_queries(ctx, a, ctx->c_tpix.p, (size_t)n, cp->prefix_bounces, ctx->c_frame.p, &nq);
   if (rc != LISA_OK) return rc;

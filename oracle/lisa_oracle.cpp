@@ -769,7 +769,8 @@ struct Tracer {
   // bounce in expectation and the bounce is drawn on the condition that it
   // hits no emitter (cond; din and rough_v are the vertex's for redraws)
   bool emit_expect = false;
-  int32_t nee_loops = 1;  // with emit_expect: NEE runs averaged per vertex
+  int32_t nee_loops = 1;       // with emit_expect: NEE runs averaged at the camera ray's hit
+  int32_t nee_loops_deep = 1;  // ... and at later vertices
   bool cond = false;
   int32_t rejects = 0, max_b = 0;
   V3 din{0, 0, 0};
@@ -930,13 +931,15 @@ struct Tracer {
       }
     } else {
       st.att = st.att * ld3(m.diffuse);
-      if (emit_expect && nee_loops > 1) {
-        // the mean of nee_loops NEE runs from the vertex's entry flag and
-        // light, each added as it ends; the last run's flag and light stay
+      const int32_t runs = rec.bounces > 1 ? nee_loops_deep : nee_loops;
+      if (emit_expect && runs > 1) {
+        // the mean of the vertex's NEE runs (nee_loops at the camera ray's
+        // hit, nee_loops_deep later) from its entry flag and light, each
+        // added as it ends; the last run's flag and light stay
         const bool e_hit = st.hit;
         const int32_t e_light = st.light_mat;
-        const float inv = 1.0f / (float)nee_loops;
-        for (int32_t k = 0; k < nee_loops; ++k) {
+        const float inv = 1.0f / (float)runs;
+        for (int32_t k = 0; k < runs; ++k) {
           st.hit = e_hit;
           st.light_mat = e_light;
           const V3 nee = shoot_ray_to_light(st);
@@ -1213,10 +1216,11 @@ void orc_cache_bounds(const orc_scene* o, float* bmin, float* binv) {
 int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t prefix_bounces,
                       const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
                       float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads,
-                      int emit_expect, int nee_loops) {
+                      int emit_expect, int nee_loops, int nee_loops_deep) {
   const Scene& s = o->s;
   if (p->width == 0 || p->height == 0 || p->spp == 0 || n <= 0) return -1;
-  if (nee_loops > 16 || (nee_loops > 1 && !emit_expect)) return -3;
+  if (nee_loops > 16 || nee_loops_deep > 16 || ((nee_loops > 1 || nee_loops_deep > 1) && !emit_expect))
+    return -3;
   const int64_t npx = (int64_t)p->width * p->height;
   for (int32_t i = 0; i < n; ++i)
     if (pixel_ids[i] < 0 || pixel_ids[i] >= npx) return -2;
@@ -1246,6 +1250,7 @@ int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t p
         tr.box = &box;
         tr.emit_expect = emit_expect != 0;
         tr.nee_loops = nee_loops > 1 ? nee_loops : 1;
+        tr.nee_loops_deep = nee_loops_deep > 1 ? nee_loops_deep : 1;
         const V3 c = tr.trace(eye, D, (int)prefix_bounces + 1);
         accum = accum + c;
         if (tr.queried) {

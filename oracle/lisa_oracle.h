@@ -131,13 +131,14 @@ void orc_cache_bounds(const orc_scene* s, float* bmin, float* binv);
  * expectation, the throughput is scaled by (1 - the sample's hit estimate)
  * and the bounce is drawn on the condition that it hits no emitter (at most
  * 64 redraws, then the path ends).  Same expectation as without it.
- * nee_loops > 1 (needs emit_expect, at most 16): a vertex's NEE is the mean of
- * that many runs from its entry hit flag and light, each added as it ends;
- * the last run's flag and light go on. */
+ * nee_loops > 1 (needs emit_expect, at most 16): the NEE at the camera ray's
+ * hit is the mean of that many runs from its entry hit flag and light, each
+ * added as it ends; the last run's flag and light go on (nee_loops_deep: the
+ * same at later vertices). */
 int orc_cache_queries(const orc_scene* s, const orc_render_params* p, uint32_t prefix_bounces,
                       const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
                       float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads,
-                      int emit_expect, int nee_loops);
+                      int emit_expect, int nee_loops, int nee_loops_deep);
 /* Training rows of plain paths (p->bounces) on the listed pixels: one row per
  * surface hit (depth 0) or per surface hit of radiance ray number `depth`,
  * 12 features and the target (final - gathered before the hit) / throughput

@@ -97,7 +97,7 @@ def lib():
         L.orc_cache_queries.restype = C.c_int
         L.orc_cache_queries.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_uint32, i32p,
                                         C.c_int32, f32p, f32p, f32p, i32p, C.c_int32, i32p, C.c_int,
-                                        C.c_int, C.c_int]
+                                        C.c_int, C.c_int, C.c_int]
         L.orc_cache_train_rows.restype = C.c_int
         L.orc_cache_train_rows.argtypes = [C.c_void_p, C.POINTER(RenderParams), i32p, C.c_int32,
                                            C.c_uint32, f32p, f32p, i32p, C.c_int32, i32p, C.c_int]
@@ -326,10 +326,13 @@ class Scene:
         return p
 
     def cache_queries(self, width, height, spp, prefix_bounces, eye, U, V, W, pixels, subframe=0,
-                      threads=1, emit_expect=False, nee_loops=1):
+                      threads=1, emit_expect=False, nee_loops=1, nee_loops_deep=None):
         """Prefix paths ending in cache queries (orc_cache_queries); emit_expect
         takes the prefix bounces' emission in expectation, nee_loops averages
-        that many NEE runs per prefix vertex."""
+        that many NEE runs at the camera ray's hit and nee_loops_deep (default:
+        nee_loops) at later vertices."""
+        if nee_loops_deep is None:
+            nee_loops_deep = nee_loops
         p = self._params(width, height, spp, prefix_bounces + 1, eye, U, V, W, subframe)
         px = np.ascontiguousarray(pixels, dtype=np.int32)
         n = px.shape[0]
@@ -342,7 +345,7 @@ class Scene:
         rc = lib().orc_cache_queries(self.h, C.byref(p), prefix_bounces, _p(px, C.c_int32), n,
                                      _p(acc, C.c_float), _p(f, C.c_float), _p(at, C.c_float),
                                      _p(it, C.c_int32), cap, C.byref(nq), threads,
-                                     1 if emit_expect else 0, nee_loops)
+                                     1 if emit_expect else 0, nee_loops, nee_loops_deep)
         if rc != 0:
             raise RuntimeError(f"orc_cache_queries failed: {rc}")
         k = nq.value

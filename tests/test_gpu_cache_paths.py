@@ -45,23 +45,24 @@ def _close(a, b, rtol=1e-5):
     return np.all(np.abs(a - b) <= rtol * np.abs(b) + 1e-30)
 
 
-@pytest.mark.parametrize("spp,prefix,emit,loops", [(1, 2, False, 1), (4, 2, False, 1), (2, 1, False, 1),
-                                                   (2, 0, False, 1), (4, 2, True, 1), (2, 1, True, 1),
-                                                   (8, 3, True, 1), (4, 2, True, 3), (2, 3, True, 16)])
-def test_queries_match_oracle(capi, oracle_mod, ctx, spp, prefix, emit, loops):
+@pytest.mark.parametrize("spp,prefix,emit,loops,deep", [
+    (1, 2, False, 1, 1), (4, 2, False, 1, 1), (2, 1, False, 1, 1), (2, 0, False, 1, 1),
+    (4, 2, True, 1, 1), (2, 1, True, 1, 1), (8, 3, True, 1, 1), (4, 2, True, 3, 3),
+    (2, 3, True, 16, 16), (4, 2, True, 8, 1), (2, 3, True, 1, 5)])
+def test_queries_match_oracle(capi, oracle_mod, ctx, spp, prefix, emit, loops, deep):
     """emit: the prefix-emission estimator (LISA_CACHE_EMISSION), whose light
-    samples, conditioned bounces and throughputs are bit-exact too; loops:
-    NEE runs averaged per prefix vertex."""
+    samples, conditioned bounces and throughputs are bit-exact too; loops /
+    deep: NEE runs averaged at the camera ray's hit / at later vertices."""
     W = H = 64
     sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
     eye = sc["camera"]["eye"]
     px = np.sort(np.random.default_rng(spp).choice(W * H, 1500, replace=False)).astype(np.int32)
     p = capi.render_params(W, H, spp, prefix + 1, eye, U, V, Wv, subframe=3)
     cp = capi.cache_frame_params(prefix_bounces=prefix, flags=capi.CACHE_EMISSION if emit else 0,
-                                 nee_loops=loops)
+                                 nee_loops=loops, nee_loops_deep=deep)
     g = capi.cache_trace_queries(ctx, p, cp, px)
     r = o.cache_queries(W, H, spp, prefix, eye, U, V, Wv, px, subframe=3, threads=8, emit_expect=emit,
-                        nee_loops=loops)
+                        nee_loops=loops, nee_loops_deep=deep)
     print("queries", g["n"], r["n"], flush=True)
     assert g["n"] == r["n"] > 0
     gi, ri = _order(g["item"], g["feat"]), _order(r["item"], r["feat"])

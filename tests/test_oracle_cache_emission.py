@@ -24,12 +24,12 @@ from tests.scene_util import cornell
 Z_BOUND = 4.0
 
 
-def _pixel_sums(o, om, sc, W, H, spp, prefix, subframe, emit, loops=1):
+def _pixel_sums(o, om, sc, W, H, spp, prefix, subframe, emit, loops=1, deep=None):
     cam = sc["camera"]
     U, V, Wv = om.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
     px = np.arange(W * H, dtype=np.int32)
     r = o.cache_queries(W, H, spp, prefix, cam["eye"], U, V, Wv, px, subframe=subframe, threads=8,
-                        emit_expect=emit, nee_loops=loops)
+                        emit_expect=emit, nee_loops=loops, nee_loops_deep=deep)
     acc = r["accum"][:, :3].astype(np.float64).sum(1)
     q = np.zeros(W * H)
     np.add.at(q, r["item"], r["att"].astype(np.float64).sum(1))
@@ -46,25 +46,25 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("loops", [1, 3])
+@pytest.mark.parametrize("loops,deep", [(1, 1), (3, 3), (8, 1)])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_prefix_emission_keeps_the_expectation(oracle_mod, case, loops):
-    """loops: NEE runs averaged per prefix vertex (each from the vertex's
-    entry hit flag and light; the last run's flag goes on), which keeps the
-    expectation of the reference's NEE at that vertex and of what follows.
-    Starting the later runs from the flag the first run left instead gives
-    z = +25 on the Cornell box."""
+def test_prefix_emission_keeps_the_expectation(oracle_mod, case, loops, deep):
+    """loops / deep: NEE runs averaged at the camera ray's hit / at later
+    prefix vertices (each from the vertex's entry hit flag and light; the
+    last run's flag goes on), which keeps the expectation of the reference's
+    NEE at that vertex and of what follows.  Starting the later runs from
+    the flag the first run left instead gives z = +25 on the Cornell box."""
     make, W, H, spp, prefix = CASES[case]
     sc = make()
     o = _scene(oracle_mod, sc)
     a0, q0, r0 = _pixel_sums(o, oracle_mod, sc, W, H, spp, prefix, 3, False)
-    a1, q1, r1 = _pixel_sums(o, oracle_mod, sc, W, H, spp, prefix, 3, True, loops)
+    a1, q1, r1 = _pixel_sums(o, oracle_mod, sc, W, H, spp, prefix, 3, True, loops, deep)
     assert r0["n"] > 0 and r1["n"] > 0
     for name, x, y in (("accum", a0, a1), ("query throughput", q0, q1)):
         d = y - x
         z = d.mean() / (d.std() / np.sqrt(d.size))
-        print(f"{case} loops {loops} {name}: plain {x.mean():.5f} estimator {y.mean():.5f} z {z:+.2f}",
-              flush=True)
+        print(f"{case} loops {loops}/{deep} {name}: plain {x.mean():.5f} estimator {y.mean():.5f} "
+              f"z {z:+.2f}", flush=True)
         assert abs(z) < Z_BOUND, (case, name, z)
 
 
