@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--train-paths", type=int, default=262144,
                     help="training paths per frame (their rows at the query depth: ~2^16 on "
                          "the island)")
-    ap.add_argument("--train-steps", type=int, default=6)
+    ap.add_argument("--train-steps", type=int, default=8)
     ap.add_argument("--rmse-limit-s", type=float, default=30.0,
                     help="give up a method after this much render time")
     ap.add_argument("--persistent", action="store_true",
