@@ -1,10 +1,14 @@
-// `lisa -s scene.rto [-d] [--dump-float file.pfm]`: the MI355X build of
-// LiSA's command line (src/LiSA/src/main.cc:6-28).
+// `lisa -s scene.rto [-d] [--cache] [--dump-float file.pfm]`: the MI355X
+// build of LiSA's command line (src/LiSA/src/main.cc:6-28).
 //
 //   -s path   scene file (required), parsed like SceneParser
 //   -d        progressive mode: subframes of min(spp, 16) samples, each
 //             lerped into the accum buffer (render.cc:75-131, shader.cu:160-164,
 //             optix_wrapper.cc:430) -- without the GLFW window
+//   --cache   progressive frames of the learned radiance cache instead
+//             (lisa_render_cached, trained online: paths end at the cache
+//             after 2 radiance rays; 4 samples per frame until the scene's
+//             sample count; a biased estimate, see lisa.h)
 //   --dump-float path   also write the linear float accum buffer as PFM
 // Render mode puts every sample in one launch (render.cc:133-148), saves the
 // PPM named by output_image and prints the wall time in minutes.
@@ -88,7 +92,30 @@ int main(int argc, char** argv) {
   std::vector<float> accum(4 * npx);
   std::vector<uint8_t> rgba8(4 * npx);
   const auto t0 = std::chrono::system_clock::now();
-  if (has_option(argc, argv, "-d")) {
+  if (has_option(argc, argv, "--cache")) {
+    lisa_cache* cache = nullptr;
+    if (lisa_cache_create(&cache, 0, 1, 1e-3f) != LISA_OK) return die("lisa_cache_create", nullptr);
+    lisa_cache_frame_params cp;
+    std::memset(&cp, 0, sizeof(cp));
+    cp.prefix_bounces = 2;
+    cp.train_paths = 262144;
+    cp.train_bounces = std::max<uint32_t>(1u, std::min<uint32_t>(info.num_bounces, 64u));
+    cp.train_max = 1u << 16;
+    cp.train_steps = 8;
+    cp.flags = LISA_CACHE_EMISSION;
+    cp.nee_loops = 8;
+    cp.nee_loops_deep = 1;
+    cp.seed = 1;
+    p.spp = std::max<uint32_t>(1u, std::min<uint32_t>(info.num_samples, 4u));
+    for (p.subframe = 0;; ++p.subframe) {
+      lisa_cache_frame_stats st;
+      if (lisa_render_cached(ctx, cache, &p, &cp, &st) != LISA_OK) return die("lisa_render_cached", ctx);
+      std::printf("nb sample   : %8u\n", (p.subframe + 1) * p.spp);
+      if ((p.subframe + 1) * p.spp >= info.num_samples) break;
+    }
+    p.subframe = 0;
+    lisa_cache_destroy(cache);
+  } else if (has_option(argc, argv, "-d")) {
     p.spp = std::min<uint32_t>(info.num_samples, 16u);
     if (p.spp == 0) p.spp = 1;
     for (p.subframe = 0;; ++p.subframe) {

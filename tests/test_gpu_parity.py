@@ -264,6 +264,36 @@ def test_cli_renders_rto(capi, oracle_mod, tmp_path):
     assert out.read_bytes() == ref.read_bytes()
 
 
+def test_cli_cache_mode(capi, tmp_path):
+    """`lisa -s scene.rto --cache`: progressive frames of the learned cache
+    (4 samples each up to the scene's count) through the same CLI; the PPM
+    and the float dump hold a finite, non-negative image of the right size
+    whose mean is within 30 % of the plain render's."""
+    import subprocess
+    from lisa_amd import scenes
+    sc = scenes.cornell(32, 16, 48, 32, 64, 5)
+    out = tmp_path / "out.ppm"
+    pfm = tmp_path / "out.pfm"
+    rto = scenes.write_rto(sc, str(tmp_path / "scene"), output_image=str(out))
+    res = subprocess.run([capi.CLI_PATH, "-s", rto, "--cache", "--dump-float", str(pfm)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.count("nb sample") == 16
+    data = out.read_bytes()
+    assert data.startswith(b"P6\n48 32\n255\n") and len(data) == len(b"P6\n48 32\n255\n") + 48 * 32 * 3
+    raw = pfm.read_bytes()
+    img = np.frombuffer(raw[raw.index(b"-1.0\n") + 5:], dtype="<f4").reshape(32, 48, 3)
+    assert np.all(np.isfinite(img)) and np.all(img >= 0)
+    plain = tmp_path / "plain.pfm"
+    res = subprocess.run([capi.CLI_PATH, "-s", rto, "--dump-float", str(plain)], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    raw = plain.read_bytes()
+    ref = np.frombuffer(raw[raw.index(b"-1.0\n") + 5:], dtype="<f4")
+    print("cache mean", img.mean(), "plain mean", ref.mean(), flush=True)
+    assert abs(img.mean() - ref.mean()) <= 0.3 * ref.mean()
+
+
 def test_refit_update_bit_exact(capi, oracle_mod, ctx):
     """lisa_update_vertices (refit with the build's topology) matches the
     oracle's refit bit for bit, and the path loop stays in parity after it."""
