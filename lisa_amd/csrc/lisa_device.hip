@@ -729,8 +729,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           const F3 e1 = v1 - v0, e2 = v2 - v0;
           const F3 nt = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
           const float cy = fabsf(dot(nt, w)) / sqrtf(dot(nt, nt));
-          const float mw = fmaxf(fabsf(w.x), fmaxf(fabsf(w.y), fabsf(w.z)));
-          const float q = 1.0f / (12.0f * mw * mw * mw);
+          const float q = Bsdf::pdf(Din, N, w, 1.0f);  // the bounce's density at roughness 1
           lw = q * cy * pp->emit_area / r2;
           ls_k = lo;
           D = w;

@@ -113,6 +113,39 @@ struct Lambertian {
     const F3 h = hemisphere(N, s);
     return lerp(r, h, roughness);
   }
+  // Density per steradian with which sample(d, N, ., roughness) returns the
+  // unit direction L (0 for roughness 0, a mirror).  The hemisphere draw h
+  // is a cube-uniform point normalised and folded to N's side, density
+  // q(h) = 1 / (12 max|h_i|^3); sample's direction is that of
+  // v = (1 - t) r + t h (t = roughness, r = reflect(d, N)), i.e. of a point
+  // on a unit sphere about c = (1 - t) r / t seen from the origin, so each
+  // preimage h of L adds q(h) (s / t)^2 / |h . L| with v = s L.
+  __device__ static __forceinline__ float pdf(F3 d, F3 N, F3 L, float roughness) {
+    if (!(roughness > 0.0f && roughness <= 1.0f)) return 0.0f;
+    if (roughness == 1.0f) {  // h = L
+      if (!(dot(N, L) > 0.0f)) return 0.0f;
+      const float mw = fmaxf(fabsf(L.x), fmaxf(fabsf(L.y), fabsf(L.z)));
+      return 1.0f / (12.0f * mw * mw * mw);
+    }
+    const float t = roughness, u = 1.0f - roughness;
+    const F3 r = reflect(d, N);
+    const float a = u * dot(L, r);
+    const float disc = a * a - u * u + t * t;
+    if (disc < 0.0f) return 0.0f;
+    const float sq = sqrtf(disc);
+    float p = 0.0f;
+    for (int k = 0; k < 2; ++k) {
+      const float s = k == 0 ? a + sq : a - sq;
+      if (!(s > 0.0f)) continue;
+      const F3 h = (s * L - u * r) * (1.0f / t);
+      const float hl = fabsf(dot(h, L));
+      if (!(dot(h, N) > 0.0f) || !(hl > 0.0f)) continue;
+      const float mh = fmaxf(fabsf(h.x), fmaxf(fabsf(h.y), fabsf(h.z)));
+      p += (1.0f / (12.0f * mh * mh * mh)) * ((s / t) * (s / t)) / hl;
+      if (sq == 0.0f) break;
+    }
+    return p;
+  }
   // eval * pdf weight: NdotL * (float)((double)NdotL / M_PI).
   __device__ static __forceinline__ float eval(F3 N, F3 L) {
     const float NdotL = clampf(dot(N, L), 0.0f, 1.0f);

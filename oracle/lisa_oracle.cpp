@@ -124,6 +124,40 @@ inline V3 bsdf_bounce(V3 d, V3 N, uint32_t& s, float roughness) {
   const V3 h = hemisphere(N, s);
   return lerp(r, h, roughness);
 }
+// Density per steradian with which bsdf_bounce(d, N, ., roughness) returns
+// the unit direction L (0 for roughness 0, a mirror).  The hemisphere draw h
+// is a cube-uniform point normalised and folded to N's side, density
+// q(h) = 1 / (12 max|h_i|^3); the bounce's direction is that of
+// v = (1 - t) r + t h (t = roughness, r = reflect(d, N)), i.e. of a point on
+// a unit sphere about c = (1 - t) r / t seen from the origin, so each
+// preimage h of L adds q(h) (s / t)^2 / |h . L| with v = s L.  (This
+// implementation's; lambertian.cu has no pdf.)
+inline float bsdf_pdf(V3 d, V3 N, V3 L, float roughness) {
+  if (!(roughness > 0.0f && roughness <= 1.0f)) return 0.0f;
+  if (roughness == 1.0f) {  // h = L
+    if (!(dot(N, L) > 0.0f)) return 0.0f;
+    const float mw = std::fmax(std::fabs(L.x), std::fmax(std::fabs(L.y), std::fabs(L.z)));
+    return 1.0f / (12.0f * mw * mw * mw);
+  }
+  const float t = roughness, u = 1.0f - roughness;
+  const V3 r = reflect(d, N);
+  const float a = u * dot(L, r);
+  const float disc = a * a - u * u + t * t;
+  if (disc < 0.0f) return 0.0f;
+  const float sq = std::sqrt(disc);
+  float p = 0.0f;
+  for (int k = 0; k < 2; ++k) {
+    const float s = k == 0 ? a + sq : a - sq;
+    if (!(s > 0.0f)) continue;
+    const V3 h = (s * L - u * r) * (1.0f / t);
+    const float hl = std::fabs(dot(h, L));
+    if (!(dot(h, N) > 0.0f) || !(hl > 0.0f)) continue;
+    const float mh = std::fmax(std::fabs(h.x), std::fmax(std::fabs(h.y), std::fabs(h.z)));
+    p += (1.0f / (12.0f * mh * mh * mh)) * ((s / t) * (s / t)) / hl;
+    if (sq == 0.0f) break;
+  }
+  return p;
+}
 inline float bsdf_eval(V3 N, V3 L) {
   const float NdotL = clampf(dot(N, L), 0.0f, 1.0f);
   const float p = (float)((double)NdotL / 3.14159265358979323846);
@@ -843,8 +877,7 @@ struct Tracer {
     if (!(r2 > 0.0f && dot(st.normal, w) > 0.0f && r.valid)) return;
     const V3 nt = cross(v1 - v0, v2 - v0);
     const float cy = std::fabs(dot(nt, w)) / std::sqrt(dot(nt, nt));
-    const float mw = std::fmax(std::fabs(w.x), std::fmax(std::fabs(w.y), std::fabs(w.z)));
-    const float q = 1.0f / (12.0f * mw * mw * mw);
+    const float q = bsdf_pdf(st.direction, st.normal, w, 1.0f);  // the bounce at roughness 1
     float te = 0.0f, tb = 0.0f;
     const int32_t e = closest<false>(s.emit, r, kTmin, kTmax, 0xFFFFFFFFu, te, cnt);
     if (e != (int32_t)s.emit.prim[(size_t)lo]) return;
@@ -1039,6 +1072,13 @@ void orc_reflect(const float* i, const float* n, float* out3) { st3(out3, reflec
 float orc_brdf(const float* N, const float* L) { return bsdf_eval(ld3(N), ld3(L)); }
 void orc_bounce(const float* d, const float* N, uint32_t* state, float roughness, float* out3) {
   st3(out3, bsdf_bounce(ld3(d), ld3(N), *state, roughness));
+}
+void orc_bounce_many(const float* d, const float* N, uint32_t* state, float roughness, int32_t n,
+                     float* out3) {
+  for (int32_t i = 0; i < n; ++i) st3(out3 + 3 * (size_t)i, bsdf_bounce(ld3(d), ld3(N), *state, roughness));
+}
+void orc_bsdf_pdf(const float* d, const float* N, const float* L, int32_t n, float roughness, float* out) {
+  for (int32_t i = 0; i < n; ++i) out[i] = bsdf_pdf(ld3(d), ld3(N), ld3(L + 3 * (size_t)i), roughness);
 }
 void orc_bary_normal(const float* P, const float* n1, const float* n2, const float* n3,
                      const float* v1, const float* v2, const float* v3, float* out3) {

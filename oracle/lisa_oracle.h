@@ -69,6 +69,10 @@ void orc_refract(float cosI, const float* d, const float* N, float eta, float* o
 void orc_reflect(const float* i, const float* n, float* out3);
 float orc_brdf(const float* N, const float* L);
 void orc_bounce(const float* d, const float* N, uint32_t* state, float roughness, float* out3);
+/* n bounces from one RNG stream; the sampler's density at n unit directions. */
+void orc_bounce_many(const float* d, const float* N, uint32_t* state, float roughness, int32_t n,
+                     float* out3);
+void orc_bsdf_pdf(const float* d, const float* N, const float* L, int32_t n, float roughness, float* out);
 void orc_bary_normal(const float* P, const float* n1, const float* n2, const float* n3,
                      const float* v1, const float* v2, const float* v3, float* out3);
 void orc_camera_uvw(const float* eye, const float* lookat, float fov, float aspect,

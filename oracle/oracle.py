@@ -64,6 +64,8 @@ def lib():
         L.orc_brdf.restype = C.c_float
         L.orc_brdf.argtypes = [f32p, f32p]
         L.orc_bounce.argtypes = [f32p, f32p, u32p, C.c_float, f32p]
+        L.orc_bounce_many.argtypes = [f32p, f32p, u32p, C.c_float, C.c_int32, f32p]
+        L.orc_bsdf_pdf.argtypes = [f32p, f32p, f32p, C.c_int32, C.c_float, f32p]
         L.orc_bary_normal.argtypes = [f32p] * 8
         L.orc_camera_uvw.argtypes = [f32p, f32p, C.c_float, C.c_float, f32p, f32p, f32p]
         L.orc_srgb_quantize.argtypes = [f32p, C.c_int, u8p]
@@ -174,6 +176,26 @@ def bounce(d, N, seed, roughness):
     d, N = f32(d), f32(N)
     lib().orc_bounce(_p(d, C.c_float), _p(N, C.c_float), C.byref(s), roughness, _p(out, C.c_float))
     return out, s.value
+
+
+def bounce_many(d, N, seed, roughness, n):
+    """n bounce directions (not normalised) from one RNG stream."""
+    s = C.c_uint32(seed)
+    out = np.zeros((n, 3), np.float32)
+    d, N = f32(d), f32(N)
+    lib().orc_bounce_many(_p(d, C.c_float), _p(N, C.c_float), C.byref(s), roughness, n,
+                          _p(out, C.c_float))
+    return out
+
+
+def bsdf_pdf(d, N, L, roughness):
+    """Density per steradian of bounce(d, N, ., roughness) at the unit directions L (n x 3)."""
+    L = f32(L).reshape(-1, 3)
+    out = np.zeros(L.shape[0], np.float32)
+    d, N = f32(d), f32(N)
+    lib().orc_bsdf_pdf(_p(d, C.c_float), _p(N, C.c_float), _p(L, C.c_float), L.shape[0], roughness,
+                       _p(out, C.c_float))
+    return out
 
 
 def bary_normal(P, n1, n2, n3, v1, v2, v3):
