@@ -453,7 +453,9 @@ struct Job {
 // path throughput, the pixel); misses and emitters are handled exactly.
 // kMode 3: kMode 1 with the emission that the prefix's diffuse bounces would
 // hit taken in expectation (a light sample at each Lambertian vertex, the
-// bounce drawn on the condition that it hits no emitter).
+// bounce drawn on the condition that it hits no emitter) and each vertex's
+// NEE the mean of nee_loops (first hit) or nee_loops_deep runs; the same
+// expectation as kMode 1.
 // kMode 2: every surface hit of the (full-length) path (or, with t_depth > 0,
 // the hit of radiance ray number t_depth only) records its features,
 // throughput and the radiance gathered so far; at the end of the sample the
