@@ -249,7 +249,8 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
  * dielectric flag.
  *
  * lisa_cache_trace_queries: the prefix paths of a frame with cp's
- * prefix_bounces, flags (LISA_CACHE_EMISSION) and nee_loops: p->spp samples
+ * prefix_bounces, flags (LISA_CACHE_EMISSION), nee_loops and nee_loops_deep
+ * (the other fields are not read): p->spp samples
  * per listed pixel (seeded by p->subframe), at most prefix_bounces + 1
  * radiance rays; the surface hit of radiance ray prefix_bounces + 1 ends the
  * path in a query (features,

@@ -468,7 +468,8 @@ def render_cached(ctx, cache, params, cparams):
 
 def cache_trace_queries(ctx, params, cparams, pixels, cap=None):
     """Prefix paths on a pixel list (lisa_cache_trace_queries) with the prefix settings of
-    cparams (prefix_bounces, CACHE_EMISSION, nee_loops): accum, features, throughput, item
+    cparams (prefix_bounces, CACHE_EMISSION, nee_loops, nee_loops_deep): accum, features,
+    throughput, item
     (at most `cap` queries copied back, default all; 0: the per-pixel accum only)."""
     px = np.ascontiguousarray(pixels, dtype=np.int32)
     n = px.shape[0]
