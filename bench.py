@@ -114,7 +114,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "the multi-GPU design (pixel-tile sharding) is described in DESIGN.md",
+              "more GPUs would shard the image by pixel tiles (no exchange during a render)",
               file=sys.stderr)
         sys.exit(2)
     from lisa_amd import capi, scenes
