@@ -80,6 +80,10 @@ def parse():
                     help="with --cache-emission: NEE runs averaged at the camera ray's hit")
     ap.add_argument("--nee-loops-deep", type=int, default=1,
                     help="with --cache-emission: NEE runs averaged at later prefix vertices")
+    ap.add_argument("--judge-pixels", type=int, default=0,
+                    help="cached frames judged by the plain tracer on this many pixels "
+                         "(plain frames once the bias estimate passes --judge-rmse)")
+    ap.add_argument("--judge-rmse", type=float, default=0.01)
     ap.add_argument("--compare-emission", action="store_true",
                     help="island / cornell_cache: also time the cache with --cache-emission flipped")
     ap.add_argument("--cache-leg", action=argparse.BooleanOptionalAction, default=True,
@@ -320,7 +324,8 @@ def bench_rmse(a, workload=None, emit=True):
                                      train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
                                      seed=3, flags=flags,
                                      nee_loops=a.nee_loops if flags & capi.CACHE_EMISSION else 1,
-                                     nee_loops_deep=a.nee_loops_deep if flags & capi.CACHE_EMISSION else 1)
+                                     nee_loops_deep=a.nee_loops_deep if flags & capi.CACHE_EMISSION else 1,
+                                     judge_pixels=a.judge_pixels, judge_rmse=a.judge_rmse)
         cache_t, tot, frames, last = None, 0.0, 0, None
         for f in range(1 << 20):
             p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
@@ -366,6 +371,7 @@ def bench_rmse(a, workload=None, emit=True):
                       "cache_emission": a.cache_emission,
                       "nee_loops": a.nee_loops if a.cache_emission else 1,
                       "nee_loops_deep": a.nee_loops_deep if a.cache_emission else 1,
+                      "judge_pixels": a.judge_pixels, "judge_rmse": a.judge_rmse,
                       "parallelism": "single GPU"},
            "plain_time_s": None if plain_t is None else round(plain_t, 4),
            "plain_progressive_time_s": None if prog_t is None else round(prog_t, 4),
@@ -451,6 +457,7 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
                       "training_beside_paths": not serial,
                       "nee_loops": a.nee_loops if a.cache_emission else 1,
                       "nee_loops_deep": a.nee_loops_deep if a.cache_emission else 1,
+                      "judge_pixels": a.judge_pixels, "judge_rmse": a.judge_rmse,
                       "parallelism": "single GPU"},
            "frame": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in med.items()},
            "mlp_roofline": mlp_roofline(med, a.train_steps, overlapped=not serial), "roofline": None,

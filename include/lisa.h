@@ -211,6 +211,13 @@ typedef struct lisa_cache_frame_params {
   uint32_t nee_loops;      /* with LISA_CACHE_EMISSION: NEE runs averaged at the camera ray's
                               hit (0 or 1: the reference's one; at most 16) */
   uint32_t nee_loops_deep; /* the same at the later prefix vertices */
+  uint32_t judge_pixels;   /* > 0: the plain tracer judges the cached frames on this many seeded
+                              pixels (at most 2^20): its own samples there (an RNG stream apart)
+                              against the cached frames' mean; once the bias estimate (squared
+                              difference less both means' variances) exceeds judge_rmse, after
+                              >= 16 plain samples per pixel and >= 4 frames, the sequence
+                              continues with plain frames, their mean restarted */
+  float judge_rmse;
   uint64_t seed;           /* draws the training pixels */
 } lisa_cache_frame_params;
 #define LISA_CACHE_PERSISTENT 1u /* the frame's paths in a persistent launch */
@@ -236,6 +243,8 @@ typedef struct lisa_cache_frame_stats {
   int32_t queries, train_samples, fell_back;
   float train_kernel_ms; /* HIP-event time of the training steps (forward, backward, Adam) */
   float infer_kernel_ms; /* HIP-event time of the inference launch */
+  float judge_rmse_est;  /* the judge's bias estimate after this frame (-1: not judged) */
+  int32_t judge_plain;   /* 1: this frame was a plain frame because the judge took over */
 } lisa_cache_frame_stats;
 
 lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_render_params* p,

@@ -437,23 +437,25 @@ class CacheFrameParams(C.Structure):
                 ("train_bounces", C.c_uint32), ("train_max", C.c_uint32),
                 ("train_steps", C.c_uint32), ("fallback_loss", C.c_float),
                 ("flags", C.c_uint32), ("nee_loops", C.c_uint32), ("nee_loops_deep", C.c_uint32),
-                ("seed", C.c_uint64)]
+                ("judge_pixels", C.c_uint32), ("judge_rmse", C.c_float), ("seed", C.c_uint64)]
 
 
 class CacheFrameStats(C.Structure):
     _fields_ = [("trace_train_ms", C.c_float), ("train_ms", C.c_float), ("render_ms", C.c_float),
                 ("infer_ms", C.c_float), ("total_ms", C.c_float), ("loss", C.c_float),
                 ("queries", C.c_int32), ("train_samples", C.c_int32), ("fell_back", C.c_int32),
-                ("train_kernel_ms", C.c_float), ("infer_kernel_ms", C.c_float)]
+                ("train_kernel_ms", C.c_float), ("infer_kernel_ms", C.c_float),
+                ("judge_rmse_est", C.c_float), ("judge_plain", C.c_int32)]
 
 
 def cache_frame_params(prefix_bounces=2, train_paths=8192, train_bounces=12, train_max=1 << 16,
                        train_steps=1, fallback_loss=0.0, seed=1, flags=0, nee_loops=1,
-                       nee_loops_deep=None):
+                       nee_loops_deep=None, judge_pixels=0, judge_rmse=0.0):
     """nee_loops_deep defaults to nee_loops (the same NEE runs at every prefix vertex)."""
     return CacheFrameParams(prefix_bounces, train_paths, train_bounces, train_max, train_steps,
                             fallback_loss, flags, nee_loops,
-                            nee_loops if nee_loops_deep is None else nee_loops_deep, seed)
+                            nee_loops if nee_loops_deep is None else nee_loops_deep, judge_pixels,
+                            judge_rmse, seed)
 
 
 def render_cached(ctx, cache, params, cparams):

@@ -1058,6 +1058,13 @@ __global__ __launch_bounds__(256) void k_cache_add(const float* __restrict__ pre
 }
 
 // Running mean over frames: accum = frame (first) or lerp(accum, frame, a).
+// out[i] = a[px[i]] for n listed pixels.
+__global__ __launch_bounds__(256) void k_pick(const float4* __restrict__ a, const int32_t* __restrict__ px,
+                                              int32_t n, float4* __restrict__ out) {
+  const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+  if (i < n) out[i] = a[px[i]];
+}
+
 __global__ __launch_bounds__(256) void k_blend(float4* accum, const float4* __restrict__ frame,
                                                int64_t n, float a, int first) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1637,6 +1644,16 @@ struct lisa_ctx {
   DevBuf<uint32_t> c_qcount;
   DevBuf<float> c_ecdf;     // area prefix sums of the emitting triangles (LISA_CACHE_EMISSION)
   float emit_area = 0.0f;
+  // the plain tracer judging cached frames (judge_pixels > 0): its pixels,
+  // per pixel-channel sums and squares of its samples and of the cached
+  // frames' values there, samples per pixel and frames judged, and the frame
+  // from which plain frames replace cached ones (-1: none)
+  std::vector<int32_t> j_pix;
+  std::vector<double> j_sum, j_sq, j_csum, j_csq;
+  uint32_t j_n = 0, j_f = 0;
+  int64_t j_from = -1;
+  DevBuf<int32_t> c_jpix;
+  DevBuf<float4> c_jout;
 };
 
 struct lisa_batch {
@@ -2110,6 +2127,8 @@ void lisa_destroy(lisa_ctx* c) {
   c->c_tpix.release();
   c->c_qcount.release();
   c->c_ecdf.release();
+  c->c_jpix.release();
+  c->c_jout.release();
   free_scene(c);
   (void)hipFree(c->d_accum);
   (void)hipFree(c->d_counters);
@@ -2628,6 +2647,35 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     ctx->accum_px = npx;
     ctx->accum_bytes = npx * sizeof(float4);
   }
+  if (cp->judge_pixels > npx || cp->judge_pixels > (1u << 20))
+    return fail(ctx, LISA_ERR_INVALID_ARG, "judge_pixels must be at most the pixel count and 2^20");
+  if (p->subframe == 0) {  // a new sequence: the judge starts over
+    ctx->j_from = -1;
+    ctx->j_n = 0;
+    ctx->j_f = 0;
+    const size_t nj = cp->judge_pixels;
+    ctx->j_pix.assign(nj, 0);
+    ctx->j_sum.assign(3 * nj, 0.0);
+    ctx->j_sq.assign(3 * nj, 0.0);
+    ctx->j_csum.assign(3 * nj, 0.0);
+    ctx->j_csq.assign(3 * nj, 0.0);
+    uint64_t sm = cp->seed ^ 0x6A09E667F3BCC909ull;
+    for (size_t i = 0; i < nj; ++i) {  // splitmix64
+      sm += 0x9E3779B97F4A7C15ull;
+      uint64_t z = sm;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      ctx->j_pix[i] = (int32_t)(z % npx);
+    }
+    if (nj > 0) {
+      HIP_TRY(ctx, ctx->c_jpix.ensure(nj));
+      HIP_TRY(ctx, ctx->c_jout.ensure(nj));
+      HIP_TRY(ctx, hipMemcpy(ctx->c_jpix.p, ctx->j_pix.data(), 4 * nj, hipMemcpyHostToDevice));
+    }
+  }
+  // plain frames from j_from on, once the judge found the cache inaccurate
+  const bool taken_over = ctx->j_from >= 0 && (int64_t)p->subframe >= ctx->j_from;
   PathArgs base = cache_args(ctx, p);
   float loss = 0.0f;
   HIP_TRY(ctx, ctx->c_frame.ensure(npx));
@@ -2642,7 +2690,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   const bool persistent = (cp->flags & LISA_CACHE_PERSISTENT) != 0;
   bool launched = false;
   // 1. online training on suffixes of full-length paths traced by the plain loop
-  if (cp->train_paths > 0 && cp->train_steps > 0) {
+  if (cp->train_paths > 0 && cp->train_steps > 0 && !taken_over) {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t nt = cp->train_paths;
     const size_t cap = cp->train_max > 0 ? cp->train_max : nt * cp->train_bounces;
@@ -2685,7 +2733,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   }
   out_stats.loss = loss;
   // 2. this frame: cached paths, or the plain tracer as the fallback
-  const bool fallback = !launched && cp->fallback_loss > 0.0f && loss > cp->fallback_loss;
+  const bool fallback = taken_over || (!launched && cp->fallback_loss > 0.0f && loss > cp->fallback_loss);
   const auto t3 = std::chrono::steady_clock::now();
   if (fallback) {
     PathArgs a = base;
@@ -2727,10 +2775,65 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   }
   out_stats.render_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t3).count();
   // 3. running mean over frames
+  // (after a takeover the mean restarts with the first plain frame)
+  const uint32_t mean_from = taken_over ? (uint32_t)ctx->j_from : 0u;
   k_blend<<<grid_for((int64_t)npx, 256), 256, 0, st>>>(ctx->d_accum, ctx->c_frame.p, (int64_t)npx,
-                                                      1.0f / (float)(p->subframe + 1), p->subframe == 0);
+                                                      1.0f / (float)(p->subframe - mean_from + 1),
+                                                      p->subframe == mean_from);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipStreamSynchronize(st));
+  out_stats.judge_plain = taken_over ? 1 : 0;
+  out_stats.judge_rmse_est = -1.0f;
+  if (cp->judge_pixels > 0 && !taken_over) {
+    // 4. the plain tracer judges: its samples on the judge pixels (an RNG
+    // stream apart) give their mean m, the cached frames' values there their
+    // mean c; bias^2 = mean over pixels and channels of (c - m)^2 - var(m) -
+    // var(c).  Past the bound (with >= 16 plain samples per pixel and >= 4
+    // frames), plain frames from the next frame on.
+    const int32_t nj = (int32_t)cp->judge_pixels;
+    lisa_render_params jp = *p;
+    jp.subframe = p->subframe ^ 0x20000000u;
+    jp.flags = 0;
+    std::vector<float> jacc(4 * (size_t)nj), jrgb(3 * (size_t)nj * p->spp);
+    rc = lisa_trace_subset(ctx, &jp, ctx->j_pix.data(), nj, jacc.data(), jrgb.data(), nullptr, nullptr,
+                           nullptr, nullptr, 0);
+    if (rc != LISA_OK) return rc;
+    for (int32_t i = 0; i < nj; ++i)
+      for (uint32_t s = 0; s < p->spp; ++s)
+        for (int c = 0; c < 3; ++c) {
+          const double x = jrgb[3 * ((size_t)i * p->spp + s) + c];
+          ctx->j_sum[3 * (size_t)i + c] += x;
+          ctx->j_sq[3 * (size_t)i + c] += x * x;
+        }
+    ctx->j_n += p->spp;
+    k_pick<<<(nj + 255) / 256, 256, 0, st>>>(ctx->c_frame.p, ctx->c_jpix.p, nj, ctx->c_jout.p);
+    HIP_TRY(ctx, hipGetLastError());
+    std::vector<float> fv(4 * (size_t)nj);
+    HIP_TRY(ctx, hipMemcpy(fv.data(), ctx->c_jout.p, 16 * (size_t)nj, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < nj; ++i)
+      for (int c = 0; c < 3; ++c) {
+        const double x = fv[4 * (size_t)i + c];
+        ctx->j_csum[3 * (size_t)i + c] += x;
+        ctx->j_csq[3 * (size_t)i + c] += x * x;
+      }
+    ctx->j_f += 1;
+    const double n = (double)ctx->j_n, nf = (double)ctx->j_f;
+    auto var_of_mean = [](double s, double q, double k) {
+      if (k < 2.0) return 0.0;
+      const double m = s / k;
+      return (q / k - m * m) / (k - 1.0);  // sample variance / k
+    };
+    double e2 = 0.0;
+    for (size_t k = 0; k < 3 * (size_t)nj; ++k) {
+      const double d = ctx->j_csum[k] / nf - ctx->j_sum[k] / n;
+      e2 += d * d - var_of_mean(ctx->j_sum[k], ctx->j_sq[k], n) -
+            var_of_mean(ctx->j_csum[k], ctx->j_csq[k], nf);
+    }
+    e2 /= 3.0 * nj;
+    out_stats.judge_rmse_est = (float)std::sqrt(std::max(e2, 0.0));
+    if (ctx->j_n >= 16 && ctx->j_f >= 4 && out_stats.judge_rmse_est > cp->judge_rmse)
+      ctx->j_from = (int64_t)p->subframe + 1;
+  }
   out_stats.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (stats) *stats = out_stats;
   return LISA_OK;

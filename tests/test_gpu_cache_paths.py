@@ -270,3 +270,46 @@ def test_emission_prefix_after_the_emitters_are_gone(capi, oracle_mod, ctx):
     ia, ib = _order(a["item"], a["feat"]), _order(b["item"], b["feat"])
     assert np.array_equal(a["feat"][ia], b["feat"][ib])
     assert np.array_equal(a["att"][ia], b["att"][ib])
+
+
+def test_plain_tracer_judges_the_cache(capi, oracle_mod, ctx):
+    """judge_pixels: the plain tracer's own samples on seeded pixels judge
+    the cached running mean.  A network that answers 0 (the frame is the
+    prefix radiance alone, biased dark) is found out once 16 plain samples
+    per pixel are in, and the sequence goes on with plain frames whose mean
+    restarts, so the image ends near the plain reference.  With a bound no
+    error reaches, the judge never takes over."""
+    W = H = 64
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    plain = capi.Context(0)
+    try:
+        plain.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        ref = plain.trace_subset(capi.render_params(W, H, 4096, 7, eye, U, V, Wv, subframe=77),
+                                 np.arange(W * H, dtype=np.int32), records=False)["accum"][:, :3]
+    finally:
+        plain.close()
+    for bound, expect in ((0.01, True), (1e9, False)):
+        cache = capi.Cache(0, seed=1)
+        try:
+            cache.set_weights(np.zeros(capi.CACHE_PARAMS, np.float32))
+            cp = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0,
+                                         judge_pixels=1024, judge_rmse=bound)
+            took = None
+            for f in range(24):
+                st = capi.render_cached(ctx, cache, capi.render_params(W, H, 4, 7, eye, U, V, Wv, subframe=f),
+                                        cp)
+                if st["judge_plain"] and took is None:
+                    took = f
+                if not st["judge_plain"]:
+                    assert st["judge_rmse_est"] >= 0
+            img = ctx.copy_accum(W * H)[:, :3]
+            e = float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
+            print(f"bound {bound}: plain frames from {took}, RMSE {e:.4f}", flush=True)
+            if expect:
+                assert took is not None and 4 <= took <= 8
+                assert e < 0.05
+            else:
+                assert took is None and e > 0.05
+        finally:
+            cache.close()
