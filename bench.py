@@ -83,7 +83,7 @@ def parse():
     ap.add_argument("--judge-pixels", type=int, default=0,
                     help="cached frames judged by the plain tracer on this many pixels "
                          "(plain frames once the bias estimate passes --judge-rmse)")
-    ap.add_argument("--judge-rmse", type=float, default=0.01)
+    ap.add_argument("--judge-rmse", type=float, default=0.02)
     ap.add_argument("--compare-emission", action="store_true",
                     help="island / cornell_cache: also time the cache with --cache-emission flipped")
     ap.add_argument("--cache-leg", action=argparse.BooleanOptionalAction, default=True,

@@ -213,10 +213,11 @@ typedef struct lisa_cache_frame_params {
   uint32_t nee_loops_deep; /* the same at the later prefix vertices */
   uint32_t judge_pixels;   /* > 0: the plain tracer judges the cached frames on this many seeded
                               pixels (at most 2^20): its own samples there (an RNG stream apart)
-                              against the cached frames' mean; once the bias estimate (squared
-                              difference less both means' variances) exceeds judge_rmse, after
-                              >= 16 plain samples per pixel and >= 4 frames, the sequence
-                              continues with plain frames, their mean restarted */
+                              against the mean of the cached frames of the current block of 8;
+                              once the bias estimate (squared difference less both means'
+                              variances) exceeds judge_rmse, from frame 16 on with >= 4 frames
+                              in the block, the sequence continues with plain frames, their
+                              mean restarted */
   float judge_rmse;
   uint64_t seed;           /* draws the training pixels */
 } lisa_cache_frame_params;

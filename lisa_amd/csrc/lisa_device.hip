@@ -2786,10 +2786,18 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   out_stats.judge_rmse_est = -1.0f;
   if (cp->judge_pixels > 0 && !taken_over) {
     // 4. the plain tracer judges: its samples on the judge pixels (an RNG
-    // stream apart) give their mean m, the cached frames' values there their
-    // mean c; bias^2 = mean over pixels and channels of (c - m)^2 - var(m) -
-    // var(c).  Past the bound (with >= 16 plain samples per pixel and >= 4
-    // frames), plain frames from the next frame on.
+    // stream apart) give their mean m, the cached frames' values there in
+    // the current block of kJudgeBlock frames their mean c (so the frames of
+    // a network still training drop out); bias^2 = mean over pixels and
+    // channels of (c - m)^2 - var(m) - var(c).  From the third block on,
+    // with >= 4 frames in the block, past the bound: plain frames from the
+    // next frame on.
+    constexpr uint32_t kJudgeBlock = 8;
+    if (p->subframe % kJudgeBlock == 0) {
+      std::fill(ctx->j_csum.begin(), ctx->j_csum.end(), 0.0);
+      std::fill(ctx->j_csq.begin(), ctx->j_csq.end(), 0.0);
+      ctx->j_f = 0;
+    }
     const int32_t nj = (int32_t)cp->judge_pixels;
     lisa_render_params jp = *p;
     jp.subframe = p->subframe ^ 0x20000000u;
@@ -2831,7 +2839,8 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     }
     e2 /= 3.0 * nj;
     out_stats.judge_rmse_est = (float)std::sqrt(std::max(e2, 0.0));
-    if (ctx->j_n >= 16 && ctx->j_f >= 4 && out_stats.judge_rmse_est > cp->judge_rmse)
+    if (p->subframe >= 2 * kJudgeBlock && ctx->j_f >= 4 && ctx->j_n >= 16 &&
+        out_stats.judge_rmse_est > cp->judge_rmse)
       ctx->j_from = (int64_t)p->subframe + 1;
   }
   out_stats.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();

@@ -275,8 +275,9 @@ def test_emission_prefix_after_the_emitters_are_gone(capi, oracle_mod, ctx):
 def test_plain_tracer_judges_the_cache(capi, oracle_mod, ctx):
     """judge_pixels: the plain tracer's own samples on seeded pixels judge
     the cached running mean.  A network that answers 0 (the frame is the
-    prefix radiance alone, biased dark) is found out once 16 plain samples
-    per pixel are in, and the sequence goes on with plain frames whose mean
+    prefix radiance alone, biased dark) is found out at the first frame it
+    may judge (frame 19: from the third block of 8 frames, with 4 frames of
+    the block in), and the sequence goes on with plain frames whose mean
     restarts, so the image ends near the plain reference.  With a bound no
     error reaches, the judge never takes over."""
     W = H = 64
@@ -296,7 +297,7 @@ def test_plain_tracer_judges_the_cache(capi, oracle_mod, ctx):
             cp = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0,
                                          judge_pixels=1024, judge_rmse=bound)
             took = None
-            for f in range(24):
+            for f in range(40):
                 st = capi.render_cached(ctx, cache, capi.render_params(W, H, 4, 7, eye, U, V, Wv, subframe=f),
                                         cp)
                 if st["judge_plain"] and took is None:
@@ -307,7 +308,7 @@ def test_plain_tracer_judges_the_cache(capi, oracle_mod, ctx):
             e = float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
             print(f"bound {bound}: plain frames from {took}, RMSE {e:.4f}", flush=True)
             if expect:
-                assert took is not None and 4 <= took <= 8
+                assert took == 20
                 assert e < 0.05
             else:
                 assert took is None and e > 0.05
