@@ -1653,7 +1653,8 @@ struct lisa_ctx {
   uint32_t j_n = 0, j_f = 0;
   int64_t j_from = -1;
   DevBuf<int32_t> c_jpix;
-  DevBuf<float4> c_jout;
+  DevBuf<float4> c_jout, c_jacc;
+  DevBuf<float> c_jrgb;
 };
 
 struct lisa_batch {
@@ -2129,6 +2130,8 @@ void lisa_destroy(lisa_ctx* c) {
   c->c_ecdf.release();
   c->c_jpix.release();
   c->c_jout.release();
+  c->c_jacc.release();
+  c->c_jrgb.release();
   free_scene(c);
   (void)hipFree(c->d_accum);
   (void)hipFree(c->d_counters);
@@ -2802,10 +2805,20 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     lisa_render_params jp = *p;
     jp.subframe = p->subframe ^ 0x20000000u;
     jp.flags = 0;
-    std::vector<float> jacc(4 * (size_t)nj), jrgb(3 * (size_t)nj * p->spp);
-    rc = lisa_trace_subset(ctx, &jp, ctx->j_pix.data(), nj, jacc.data(), jrgb.data(), nullptr, nullptr,
-                           nullptr, nullptr, 0);
-    if (rc != LISA_OK) return rc;
+    const size_t ns = (size_t)nj * p->spp;
+    HIP_TRY(ctx, ctx->c_jacc.ensure((size_t)nj));
+    HIP_TRY(ctx, ctx->c_jrgb.ensure(3 * ns));
+    PathArgs ja = make_args(&jp);  // the plain loop on the judge pixels, per-sample radiance kept
+    ja.pixels = ctx->c_jpix.p;
+    ja.n_items = nj;
+    ja.accum = ctx->c_jacc.p;
+    ja.accum_prev = nullptr;
+    ja.s_rgb = ctx->c_jrgb.p;
+    k_path<true, false><<<grid_for(nj, kPathBlock), kPathBlock, 0, st>>>(view(ctx), ja);
+    HIP_TRY(ctx, hipGetLastError());
+    std::vector<float> jrgb(3 * ns);
+    HIP_TRY(ctx, hipMemcpyAsync(jrgb.data(), ctx->c_jrgb.p, 4 * 3 * ns, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
     for (int32_t i = 0; i < nj; ++i)
       for (uint32_t s = 0; s < p->spp; ++s)
         for (int c = 0; c < 3; ++c) {
