@@ -72,6 +72,8 @@ def parse():
                     help="give up a method after this much render time")
     ap.add_argument("--persistent", action="store_true",
                     help="time the persistent launch (LISA_FLAG_PERSISTENT) of the plain path")
+    ap.add_argument("--wavefront", action="store_true",
+                    help="time the ray-level wavefront form (LISA_FLAG_WAVEFRONT) of the plain path")
     ap.add_argument("--cache-persistent", action="store_true",
                     help="cached frames' paths in one persistent launch (LISA_CACHE_PERSISTENT)")
     ap.add_argument("--cache-emission", action=argparse.BooleanOptionalAction, default=True,
@@ -89,6 +91,12 @@ def parse():
     ap.add_argument("--cache-leg", action=argparse.BooleanOptionalAction, default=True,
                     help="default workload: also time the island's cache against plain "
                          "(time to RMSE <= 1e-2) and report it under cache_vs_plain")
+    ap.add_argument("--hbm-leg", action=argparse.BooleanOptionalAction, default=True,
+                    help="default workload: also time the 5.2M-triangle icosphere (two steps) "
+                         "and report its roofline under roofline_hbm_case")
+    ap.add_argument("--vr-spp", type=int, default=16,
+                    help="spp per frame of the variance-reduced plain tracer in the time-to-RMSE "
+                         "comparison (0: skip it)")
     ap.add_argument("--cache-overlap", action="store_true",
                     help="--cache-frames: train beside the frame's paths as the time-to-RMSE "
                          "runs do (default there: train first, so the training kernels are "
@@ -114,23 +122,18 @@ def traffic_from_profile(workload, spp):
         return None
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "more GPUs would shard the image by pixel tiles (no exchange during a render)",
-              file=sys.stderr)
-        sys.exit(2)
+def bench_plain(a, workload, steps, warmup, cpu=True):
+    """One plain-megakernel workload: exact ray counts from the counting
+    instantiation, `steps` timed renders after `warmup`, the kernel's
+    roofline (algorithmic bytes over the HIP-event kernel time; DRAM traffic
+    from the committed rocprof counters) and, with `cpu`, the CPU oracle on a
+    seeded pixel subset.  Returns (result dict, ctx triangle count)."""
     from lisa_amd import capi, scenes
 
-    if a.workload == "batch64":
-        return bench_batch(a)
-    if a.workload in ("island", "cornell_cache"):
-        return bench_rmse(a)
-    factory, def_spp, def_cpu_px = WORKLOADS[a.workload]
+    factory, def_spp, def_cpu_px = WORKLOADS[workload]
     sc = getattr(scenes, factory)()
     W, H, B = sc["width"], sc["height"], sc["bounces"]
-    spp = a.spp or def_spp
+    spp = (a.spp if workload == a.workload else None) or def_spp
     cpu_pixels = a.cpu_pixels or def_cpu_px
     cam = sc["camera"]
     U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
@@ -148,34 +151,42 @@ def main():
     rays = counts["rays_radiance"] + counts["rays_occlusion"]
     print(f"[bench] counted step: {counts}", file=sys.stderr, flush=True)
 
-    p.flags = capi.FLAG_PERSISTENT if a.persistent else 0
-    for _ in range(a.warmup):
+    p.flags = (capi.FLAG_PERSISTENT if a.persistent else 0) | (capi.FLAG_WAVEFRONT if a.wavefront else 0)
+    for _ in range(warmup):
         ctx.render(p, want_accum=False)
     ctx.synchronize()
     kernel_ms = []
     t0 = time.perf_counter()
-    for i in range(a.steps):
+    for i in range(steps):
         ctx.render(p, want_accum=False)
         kernel_ms.append(ctx.stats()["render_ms"])
         print(f"[bench] step {i}: kernel {kernel_ms[-1]:.1f} ms", file=sys.stderr, flush=True)
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / a.steps
-    value = rays * a.steps / wall / 1e6
     avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
 
     alg_bytes = (S_NODE * counts["node_visits"] + S_TRI * counts["tri_tests"]
                  + S_SHADE * counts["radiance_hits"] + S_PIXEL * W * H)
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile(workload, spp)
+    resident = workload in ("cornell", "cornell65k")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profile(a.workload, spp),
-                "kernel": "k_path_pq<0>" if a.persistent else "k_path<false,false,false>",
+                "traffic": traffic,
+                "dram_frac": (None if traffic is None else
+                              round(traffic / (avg_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+                "limiter": ("instruction issue under divergence; the scene is L2 / Infinity "
+                            "Cache resident, so `frac` counts bytes served mostly from cache"
+                            if resident else
+                            "instruction issue under divergence, with the scene's nodes and "
+                            "triangles streamed from HBM"),
+                "kernel": ("k_path_pq<0>" if a.persistent else
+                           "k_wf_trace + k_wf_shade rounds" if a.wavefront else "k_path<false,false,false>"),
                 "alg_bytes_per_launch": alg_bytes,
                 "avg_kernel_ms": round(avg_kernel_ms, 3)}
 
-    cpu = None
-    if not a.no_cpu:
+    cpu_res = None
+    if cpu:
         from oracle import oracle
         threads = len(os.sched_getaffinity(0))
         o = oracle.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
@@ -191,28 +202,58 @@ def main():
         dt1 = time.perf_counter() - t
         crays1 = int(r1["counters"][0] + r1["counters"][1])
         whole = " (the whole image)" if len(px) == W * H else ""
-        cpu = {"value": round(crays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{len(px)} seeded pixels of the same {W}x{H} render{whole} at "
-                         f"{spp} spp x {B} bounces ({crays} rays, {dt:.1f} s, OpenMP over pixels)",
-               "samples_per_s": round(len(px) * spp / dt, 1),
-               "single_thread": {"value": round(crays1 / dt1 / 1e6, 3), "unit": "Mrays/s",
-                                 "cores": 1, "sample": f"every {max(threads, 1)}th pixel of that "
-                                 f"sample ({len(px1)} pixels, {crays1} rays, {dt1:.1f} s)"}}
-        print(f"[bench] cpu oracle: {cpu}", file=sys.stderr, flush=True)
+        cpu_res = {"value": round(crays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                   "kind": "port",
+                   "sample": f"{len(px)} seeded pixels of the same {W}x{H} render{whole} at "
+                             f"{spp} spp x {B} bounces ({crays} rays, {dt:.1f} s, OpenMP over "
+                             f"pixels)",
+                   "samples_per_s": round(len(px) * spp / dt, 1),
+                   "single_thread": {"value": round(crays1 / dt1 / 1e6, 3), "unit": "Mrays/s",
+                                     "cores": 1,
+                                     "sample": f"every {max(threads, 1)}th pixel of that sample "
+                                               f"({len(px1)} pixels, {crays1} rays, {dt1:.1f} s)"}}
+        print(f"[bench] cpu oracle: {cpu_res}", file=sys.stderr, flush=True)
 
-    out = {"metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
-           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 2),
+    out = {"metric": METRIC, "value": round(rays * steps / wall / 1e6, 2), "unit": "Mrays/s",
+           "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(wall * 1e3 / steps, 2),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
-           "config": {"workload": f"{sc['name']} plain megakernel"
+           "config": {"workload": f"{sc['name']} plain "
+                                  + ("wavefront form" if a.wavefront else "megakernel")
                                   + (" (persistent launch)" if a.persistent else ""),
                       "width": W, "height": H,
                       "spp": spp, "bounces": B, "triangles": ctx.T, "parallelism": "single GPU"},
-           "samples_per_s": round(W * H * spp * a.steps / wall, 1),
+           "samples_per_s": round(W * H * spp * steps / wall, 1),
            "rays_per_step": rays, "bvh_build_ms": round(bvh_ms, 3),
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu_res}
     ctx.close()
+    return out
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "more GPUs would shard the image by pixel tiles (no exchange during a render)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    if a.workload == "batch64":
+        return bench_batch(a)
+    if a.workload in ("island", "cornell_cache"):
+        return bench_rmse(a)
+    out = bench_plain(a, a.workload, a.steps, a.warmup, cpu=not a.no_cpu)
+    if a.workload == "cornell65k" and a.hbm_leg:
+        # the HBM-facing case: the 5.2M-triangle icosphere, whose nodes and
+        # triangles stream from HBM (the headline scene is cache-resident)
+        try:
+            r = bench_plain(a, "icosphere", 2, 1, cpu=False)
+            out["roofline_hbm_case"] = dict(r["roofline"], workload=r["config"]["workload"],
+                                            mrays_per_s=r["value"], ms_per_step=r["ms_per_step"],
+                                            triangles=r["config"]["triangles"])
+        except Exception as e:  # the headline line is still reported
+            out["roofline_hbm_case"] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] icosphere leg failed: {e!r}", file=sys.stderr, flush=True)
     if a.workload == "cornell65k" and a.cache_leg:
         # the metric's second half: time to RMSE <= 1e-2 on the island, cache vs plain
         try:
@@ -221,7 +262,9 @@ def main():
                 k: r[k] for k in ("metric", "plain_time_fit_s", "plain_progressive_time_s",
                                   "plain_time_s", "speedup_vs_plain_fit",
                                   "speedup_vs_plain_progressive", "speedup_vs_plain",
-                                  "cache_frames", "cache_rmse", "config", "mlp_roofline")}
+                                  "vr_plain_time_s", "vr_plain_frames", "speedup_vs_vr_plain",
+                                  "cache_frames", "cache_rmse", "rmse_sample", "config",
+                                  "mlp_roofline")}
             out["cache_vs_plain"]["cache_time_s"] = r["value"]
         except Exception as e:  # the plain line is still reported
             out["cache_vs_plain"] = {"error": f"{type(e).__name__}: {e}"}
@@ -318,7 +361,8 @@ def bench_rmse(a, workload=None, emit=True):
             break
     print(f"[bench] plain progressive: {prog_t}", file=sys.stderr, flush=True)
     # cache
-    def run_cache(flags):
+    def run_cache(flags, frame_spp=None):
+        frame_spp = frame_spp or a.cache_spp
         cache = capi.Cache(0, seed=1, lr=1e-3)
         cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                      train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
@@ -328,7 +372,7 @@ def bench_rmse(a, workload=None, emit=True):
                                      judge_pixels=a.judge_pixels, judge_rmse=a.judge_rmse)
         cache_t, tot, frames, last = None, 0.0, 0, None
         for f in range(1 << 20):
-            p = capi.render_params(W, H, a.cache_spp, B, cam["eye"], U, V, Wv, subframe=f)
+            p = capi.render_params(W, H, frame_spp, B, cam["eye"], U, V, Wv, subframe=f)
             t = time.perf_counter()
             last = capi.render_cached(ctx, cache, p, cp)
             tot += time.perf_counter() - t
@@ -351,6 +395,12 @@ def bench_rmse(a, workload=None, emit=True):
                  "time_s": None if o_t is None else round(o_t, 4), "frames": o_frames,
                  "rmse": round(o_e, 6)}
     cache_t, tot, frames, last, e_cache = run_cache(cache_flags(a, capi))
+    # the same estimators on whole paths without the network (the
+    # variance-reduced plain tracer): what the network itself gains
+    vr_t, vr_frames = None, 0
+    if a.vr_spp > 0 and a.cache_emission:
+        vr_t, _, vr_frames, _, _ = run_cache(capi.CACHE_FULL_PATHS | capi.CACHE_EMISSION,
+                                             frame_spp=a.vr_spp)
     mlp = mlp_roofline(last, a.train_steps, overlapped=True)
     fit_t = None
     above = [r for r in sweep if r["rmse"] > RMSE_TARGET]
@@ -382,6 +432,13 @@ def bench_rmse(a, workload=None, emit=True):
                                 else round(plain_t / cache_t, 3)),
            "speedup_vs_plain_progressive": (None if (prog_t is None or cache_t is None)
                                             else round(prog_t / cache_t, 3)),
+           "vr_plain_time_s": None if vr_t is None else round(vr_t, 4),
+           "vr_plain_frames": vr_frames,
+           "vr_plain_spp_per_frame": a.vr_spp,
+           "speedup_vs_vr_plain": (None if (vr_t is None or cache_t is None)
+                                   else round(vr_t / cache_t, 3)),
+           "rmse_sample": (f"{REF_PIXELS} seeded pixels of {W * H}: the mean squared error over "
+                           "them estimates the full image's without bias"),
            "cache_frames": frames, "cache_rmse": round(e_cache, 6), "cache_other_form": other,
            "plain_sweep": sweep,
            "reference": {"spp": 16384, "pixels": REF_PIXELS, "fp64": True,
@@ -501,8 +558,17 @@ def bench_batch(a):
         c.render(p, want_accum=False)
     seq_s = time.perf_counter() - t
     batch = capi.Batch(ctxs, params)
+    # each form once at a fixed configuration, then the autotuner's pick
+    forms = {}
+    cus = 256
+    for name, cfg in (("lane queue", (cus * 16, 1, 0)), ("wave tiles", (cus * 16, 1, 1)),
+                      ("ray wavefront", (cus * 32, 2, 2))):
+        batch.set_config(*cfg)
+        batch.render()
+        forms[name] = round(batch.render(), 2)
+        print(f"[bench] form {name} {cfg}: {forms[name]} ms", file=sys.stderr, flush=True)
     cfg = batch.autotune(tune_spp=2)
-    print(f"[bench] autotuned grid/chunk/form: {cfg}", file=sys.stderr, flush=True)
+    print(f"[bench] autotuned grid/chunk/form/graph: {cfg}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         batch.render()
     t0 = time.perf_counter()
@@ -512,12 +578,13 @@ def bench_batch(a):
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": round(wall * 1e3 / a.steps, 2), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "batch64 wavefront (persistent queue, hipGraph)", "scenes": n,
+           "config": {"workload": "batch64: 64 scenes in one batched launch (hipGraph)", "scenes": n,
                       "triangles": tris, "spp": params[0].spp, "bounces": params[0].bounces,
                       "grid_blocks": cfg[0], "chunk": cfg[1],
-                      "form": "wave tiles" if cfg[2] == 1 else "lane queue",
+                      "form": ("lane queue", "wave tiles", "ray wavefront")[cfg[2]],
                       "parallelism": "single GPU"},
            "rays_per_step": rays, "graph_replay_ms": [round(x, 2) for x in ms],
+           "form_ms_at_fixed_config": forms,
            "sequential_plain_ms": round(seq_s * 1e3, 2),
            "speedup_vs_sequential_plain": round(seq_s / (wall / a.steps), 3)}
     print(json.dumps(out), flush=True)

@@ -62,6 +62,11 @@ typedef struct lisa_render_params {
 #define LISA_FLAG_COUNT 2u       /* also count rays / node visits / triangle tests */
 #define LISA_FLAG_PERSISTENT 4u  /* one persistent launch whose lanes pull pixels from a queue
                                     (same image; not with COUNT or FP64_ACCUM) */
+#define LISA_FLAG_WAVEFRONT 8u   /* the ray-level wavefront form: path states in HBM, rounds of a
+                                    persistent traversal kernel (per-wave LDS request queues, lanes
+                                    refilled as their walks end) and a shading kernel (ballot /
+                                    prefix-sum compaction of the next requests); the same image
+                                    bit for bit (not with COUNT, FP64_ACCUM or PERSISTENT) */
 
 /* Counters (lisa_get_counters), in order. */
 #define LISA_NCOUNTERS 8
@@ -141,7 +146,14 @@ typedef struct lisa_batch lisa_batch;
 typedef struct lisa_batch_config {
   int32_t grid_blocks; /* persistent workgroups (64 work-items each) */
   int32_t chunk;       /* form 0: pixels a lane takes from the queue per atomic */
-  int32_t form;        /* 0: every lane pulls pixels (any scene); 1: every wave pulls 8x8 tiles */
+  int32_t form;        /* 0: every lane pulls pixels (any scene); 1: every wave pulls 8x8 tiles;
+                          2: ray-level wavefront (LISA_FLAG_WAVEFRONT's form over the batch's work
+                          list: grid_blocks = the traversal kernel's persistent grid, chunk =
+                          paths in flight per traversal lane) */
+  int32_t graph;       /* 1: the launch is captured once into a hipGraph and replayed; 0: plain
+                          launches on the stream.  A capture that fails (e.g. the stream is
+                          already being captured by the caller) falls back to plain launches
+                          and lisa_batch_get_config then reports graph = 0. */
 } lisa_batch_config;
 
 lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,
@@ -151,7 +163,7 @@ lisa_status lisa_batch_set_config(lisa_batch* b, const lisa_batch_config* cfg);
 lisa_status lisa_batch_get_config(const lisa_batch* b, lisa_batch_config* cfg);
 /* Replays the graph; elapsed_ms (optional) = HIP-event time of the replay. */
 lisa_status lisa_batch_render(lisa_batch* b, float* elapsed_ms);
-/* Autotuner: times every (grid, chunk) candidate on a copy of the batch
+/* Autotuner: times every (form, grid, chunk) candidate on a copy of the batch
  * with spp reduced to tune_spp and keeps the fastest configuration. */
 lisa_status lisa_batch_autotune(lisa_batch* b, int32_t tune_spp, lisa_batch_config* best_out);
 
@@ -205,7 +217,7 @@ typedef struct lisa_cache_frame_params {
   uint32_t train_paths;    /* training paths per frame (0: no training) */
   uint32_t train_bounces;  /* bounces of a training path */
   uint32_t train_max;      /* at most this many training rows (suffixes) per frame; 0: no cap */
-  uint32_t train_steps;    /* Adam steps per frame over the frame's rows */
+  uint32_t train_steps;    /* Adam steps per frame over the frame's rows (at most 64) */
   float fallback_loss;     /* > 0: plain frame when the last training loss exceeds it */
   uint32_t flags;          /* LISA_CACHE_* */
   uint32_t nee_loops;      /* with LISA_CACHE_EMISSION: NEE runs averaged at the camera ray's
@@ -237,6 +249,14 @@ typedef struct lisa_cache_frame_params {
  * train_kernel_ms include the wait for the paths.  This flag keeps the two
  * apart (for timing the training kernels alone). */
 #define LISA_CACHE_SERIAL 8u
+/* No network: the frame's paths run all p->bounces radiance rays with the
+ * prefix estimators (LISA_CACHE_EMISSION, nee_loops, nee_loops_deep) at every
+ * vertex and end without a query; no training, no inference.  The frames are
+ * the variance-reduced plain tracer -- the method the cache is measured
+ * against to tell what the network itself gains.  Without LISA_CACHE_EMISSION
+ * they are the plain tracer's frames bit for bit (another RNG stream per
+ * subframe, as in progressive rendering). */
+#define LISA_CACHE_FULL_PATHS 16u
 
 typedef struct lisa_cache_frame_stats {
   float trace_train_ms, train_ms, render_ms, infer_ms, total_ms;  /* host wall time */
@@ -259,7 +279,8 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
  * dielectric flag.
  *
  * lisa_cache_trace_queries: the prefix paths of a frame with cp's
- * prefix_bounces, flags (LISA_CACHE_EMISSION), nee_loops and nee_loops_deep
+ * prefix_bounces, flags (LISA_CACHE_EMISSION, LISA_CACHE_FULL_PATHS: all
+ * p->bounces rays, no query), nee_loops and nee_loops_deep
  * (the other fields are not read): p->spp samples
  * per listed pixel (seeded by p->subframe), at most prefix_bounces + 1
  * radiance rays; the surface hit of radiance ray prefix_bounces + 1 ends the
@@ -309,12 +330,6 @@ lisa_status lisa_trace_subset(lisa_ctx* ctx, const lisa_render_params* p,
                               float* sample_rgb, int32_t* sample_bounces,
                               int32_t* sample_occl, uint32_t* sample_hash,
                               int32_t* rad_hits, int32_t record_bounces);
-
-/* Streams `bytes` through a plain device copy kernel (read + write, float4,
- * grid-stride) `iters` times and returns the average kernel time; the
- * known-size reference for HBM counters (FETCH_SIZE / WRITE_SIZE) and the
- * achievable-bandwidth figure.  No counterpart in the reference. */
-lisa_status lisa_copy_probe(lisa_ctx* ctx, int64_t bytes, int32_t iters, float* avg_ms);
 
 /* ---- host-side pieces of the drop-in (no device needed) ---- */
 

@@ -62,5 +62,17 @@ def build(verbose_resources=False, defines=(), out_dir=OUT, cli=True):
     return lib
 
 
+# Variant of the library with a 2-entry LDS traversal stack: every deeper
+# entry goes to the per-lane private overflow array, so a parity test run
+# against it exercises that path (the default 22-entry stack never
+# overflows on the benchmark scenes).
+SMALL_STACK = os.path.join(OUT, "stack2")
+
+
+def build_small_stack():
+    return build(defines=("LISA_STACK_LDS=2",), out_dir=SMALL_STACK, cli=False)
+
+
 if __name__ == "__main__":
     build(verbose_resources="-v" in sys.argv)
+    build_small_stack()

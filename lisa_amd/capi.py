@@ -29,6 +29,7 @@ MODE_PLAIN = 0
 FLAG_FP64_ACCUM = 1
 FLAG_COUNT = 2
 FLAG_PERSISTENT = 4
+FLAG_WAVEFRONT = 8
 NCOUNTERS = 8
 COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
                  "radiance_hits", "samples", "invalid_dir_rays", "pixels")
@@ -38,7 +39,7 @@ EXPORTS = (
     "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_update_vertices",
     "lisa_render",
     "lisa_copy_accum", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
-    "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset", "lisa_copy_probe",
+    "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset",
     "lisa_batch_create", "lisa_batch_destroy", "lisa_batch_set_config", "lisa_batch_get_config",
     "lisa_batch_render", "lisa_batch_autotune",
     "lisa_cache_create", "lisa_cache_destroy", "lisa_cache_last_error", "lisa_cache_set_weights",
@@ -76,7 +77,8 @@ class Stats(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("grid_blocks", C.c_int32), ("chunk", C.c_int32), ("form", C.c_int32)]
+    _fields_ = [("grid_blocks", C.c_int32), ("chunk", C.c_int32), ("form", C.c_int32),
+                ("graph", C.c_int32)]
 
 
 class SceneInfo(C.Structure):
@@ -116,7 +118,6 @@ def lib():
         "lisa_trace_rays": (st, [vp, f32p, f32p, C.c_int32, C.c_float, C.c_float, i32p, f32p]),
         "lisa_trace_subset": (st, [vp, C.POINTER(RenderParams), i32p, C.c_int32, f32p, f32p,
                                    i32p, i32p, u32p, i32p, C.c_int32]),
-        "lisa_copy_probe": (st, [vp, C.c_int64, C.c_int32, f32p]),
         "lisa_batch_create": (st, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(RenderParams), C.c_int32]),
         "lisa_batch_destroy": (None, [vp]),
@@ -322,11 +323,6 @@ class Context:
         self._ck(lib().lisa_get_counters(self.h, _p(c, C.c_uint64)))
         return c
 
-    def copy_probe(self, nbytes, iters=10):
-        ms = C.c_float()
-        self._ck(lib().lisa_copy_probe(self.h, int(nbytes), int(iters), C.byref(ms)))
-        return ms.value
-
     def stream(self):
         return lib().lisa_get_stream(self.h)
 
@@ -398,13 +394,13 @@ class Batch:
             pass
 
     def config(self):
-        """(grid_blocks, chunk, form)."""
+        """(grid_blocks, chunk, form, graph)."""
         c = BatchConfig()
         _check(lib().lisa_batch_get_config(self.h, C.byref(c)))
-        return c.grid_blocks, c.chunk, c.form
+        return c.grid_blocks, c.chunk, c.form, c.graph
 
-    def set_config(self, grid_blocks, chunk, form=0):
-        c = BatchConfig(grid_blocks, chunk, form)
+    def set_config(self, grid_blocks, chunk, form=0, graph=1):
+        c = BatchConfig(grid_blocks, chunk, form, graph)
         _check(lib().lisa_batch_set_config(self.h, C.byref(c)))
 
     def render(self):
@@ -413,10 +409,10 @@ class Batch:
         return ms.value
 
     def autotune(self, tune_spp=2):
-        """Measured pick of (grid_blocks, chunk, form)."""
+        """Measured pick of (grid_blocks, chunk, form, graph)."""
         c = BatchConfig()
         _check(lib().lisa_batch_autotune(self.h, tune_spp, C.byref(c)))
-        return c.grid_blocks, c.chunk, c.form
+        return c.grid_blocks, c.chunk, c.form, c.graph
 
     def accum(self, i):
         p = self.params[i]
@@ -430,6 +426,7 @@ CACHE_INPUTS = 12
 CACHE_PERSISTENT = 1
 CACHE_EMISSION = 2  # prefix bounces' emission in expectation (same mean, less variance)
 CACHE_SERIAL = 8  # train before the frame's paths (training kernels timed alone)
+CACHE_FULL_PATHS = 16  # no network: whole paths with the prefix estimators (variance-reduced plain)
 
 
 class CacheFrameParams(C.Structure):

@@ -364,7 +364,28 @@ int infer_blocks(int cus, int64_t n) {
                                                      3 * (int64_t)cus));
 }
 
-constexpr int kTrainWaves = 2;
+// Waves per training workgroup: they share one LDS copy of the weights in
+// both layouts (84 KB) and hold 17 KB of activations each, 152 KB in all, so
+// one workgroup per CU; the grid is persistent (one workgroup per CU), each
+// wave looping over 16-row tiles, so the weights are loaded once per CU.
+constexpr int kTrainWaves = 4;
+constexpr int kLossSlots = 64;  // one loss accumulator per step of a training call
+
+// One feature of a 16-row tile (src[i * stride], i < 16) to 16 consecutive
+// BF16 values at dst (32-byte aligned), as two 16-byte stores.
+__device__ inline void store16(const uint16_t* src, int stride, uint16_t* dst) {
+  u32x4 v[2];
+  for (int h = 0; h < 2; ++h)
+    for (int j = 0; j < 4; ++j)
+      v[h][j] = (uint32_t)src[(8 * h + 2 * j) * stride] | ((uint32_t)src[(8 * h + 2 * j + 1) * stride] << 16);
+  reinterpret_cast<u32x4*>(dst)[0] = v[0];
+  reinterpret_cast<u32x4*>(dst)[1] = v[1];
+}
+// Four consecutive BF16 values to dst (8-byte aligned), one store.
+__device__ inline void store4(const uint16_t* v, uint16_t* dst) {
+  *reinterpret_cast<uint2*>(dst) = make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16),
+                                             (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+}
 
 // Forward + relative-L2 loss + backward data pass for 16 samples per wave
 // tile.  Writes feature-major (row = feature, column = sample, row stride npad):
@@ -418,14 +439,14 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
       in_stride = kHStride;
     }
     const f32x4 y = layer_fwd(kLayers - 1, in, in_stride, s_wt, nullptr, lane);
-    // layer inputs, feature-major: X (160 rows), A1..A4 (64 rows)
-    for (int f = lane; f < kEncPad; f += 64)
-      for (int i = 0; i < 16; ++i) act_l[0][(int64_t)f * npad + base + i] = X[i * kXStride + f];
+    // layer inputs, feature-major: X (160 rows), A1..A4 (64 rows); a lane
+    // writes a feature's 16 samples as two 16-byte stores
+    for (int f = lane; f < kEncPad; f += 64) store16(X + f, kXStride, act_l[0] + (int64_t)f * npad + base);
     for (int l = 1; l < kLayers; ++l)
-      for (int i = 0; i < 16; ++i)
-        act_l[l][(int64_t)lane * npad + base + i] = s_a[wave][l - 1][i * kHStride + lane];
+      store16(s_a[wave][l - 1] + lane, kHStride, act_l[l] + (int64_t)lane * npad + base);
     // dLoss/dY for L = mean over samples and channels of (y - t)^2 / (y^2 + 0.01),
     // the denominator held constant (relative L2)
+    uint16_t gq[4];
     for (int i = 0; i < 4; ++i) {
       const int64_t m = base + 4 * q + i;
       float g = 0.0f;
@@ -438,10 +459,10 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
           loss_acc += d * d / den * inv_count;
         }
       }
-      const uint16_t gb = f2bf(g);
-      G[(4 * q + i) * kGStride + r] = gb;
-      dz_l[kLayers - 1][(int64_t)r * npad + m] = gb;
+      gq[i] = f2bf(g);
+      G[(4 * q + i) * kGStride + r] = gq[i];
     }
+    store4(gq, dz_l[kLayers - 1] + (int64_t)r * npad + base + 4 * q);
     for (int e = lane; e < 16 * 16; e += 64) G[(e / 16) * kGStride + 16 + e % 16] = 0;  // K pad 16..31
     __builtin_amdgcn_wave_barrier();
     // backward data pass: dZ_{l-1} = (dZ_l W_l^T) * [A_l > 0], l = 4 .. 1
@@ -459,14 +480,17 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
       }
       __builtin_amdgcn_wave_barrier();
       const uint16_t* A = s_a[wave][l - 1];  // input of layer l = output of layer l-1
-      for (int nt = 0; nt < 4; ++nt)
+      for (int nt = 0; nt < 4; ++nt) {
+        uint16_t gq[4];
+        const int col = nt * 16 + r;
         for (int i = 0; i < 4; ++i) {
-          const int row = 4 * q + i, col = nt * 16 + r;
+          const int row = 4 * q + i;
           const float g = bf2f(A[row * kHStride + col]) > 0.0f ? acc[nt][i] : 0.0f;
-          const uint16_t gb = f2bf(g);
-          G[row * kGStride + col] = gb;
-          dz_l[l - 1][(int64_t)col * npad + base + row] = gb;
+          gq[i] = f2bf(g);
+          G[row * kGStride + col] = gq[i];
         }
+        store4(gq, dz_l[l - 1] + (int64_t)col * npad + base + 4 * q);
+      }
       __builtin_amdgcn_wave_barrier();
     }
   }
@@ -475,16 +499,29 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
   if (lane == 0) atomicAdd(loss, loss_acc);
 }
 
-// dW_l^T (out_pad x in_pad) += dZ_l (out_pad x npad) * in_l^T (npad x in_pad);
-// one wave per (16 x 16 tile, K split).
-__global__ __launch_bounds__(256) void k_mlp_wgrad(const uint16_t* __restrict__ dz,
-                                                   const uint16_t* __restrict__ in, int64_t npad,
-                                                   int out_pad, int in_pad, int ksplit,
-                                                   float* __restrict__ dwt) {
-  const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), lane = threadIdx.x & 63;
+// dW_l^T (out_pad x in_pad) += dZ_l (out_pad x npad) * in_l^T (npad x in_pad)
+// for every layer in one launch; one wave per (layer, 16 x 16 tile, K split):
+// waves [first[l], first[l + 1]) belong to layer l.
+struct WgradPlan {
+  int64_t off_dz[kLayers], off_in[kLayers];
+  int first[kLayers + 1];
+  int ksplit;
+};
+__global__ __launch_bounds__(256) void k_mlp_wgrad(const uint16_t* __restrict__ dz_all,
+                                                   const uint16_t* __restrict__ in_all, int64_t npad,
+                                                   WgradPlan plan, float* __restrict__ g) {
+  int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (wave >= plan.first[kLayers]) return;
+  int l = 0;
+  while (wave >= plan.first[l + 1]) ++l;
+  wave -= plan.first[l];
+  const uint16_t* __restrict__ dz = dz_all + plan.off_dz[l];
+  const uint16_t* __restrict__ in = in_all + plan.off_in[l];
+  const int in_pad = kInDim[l], ksplit = plan.ksplit;
+  float* __restrict__ dwt = g + kOffset[l];
   const int q = lane >> 4, r = lane & 15;
-  const int tiles_in = in_pad / 16, tiles = (out_pad / 16) * tiles_in;
-  if (wave >= tiles * ksplit) return;
+  const int tiles_in = in_pad / 16;
   const int tile = wave / ksplit, split = wave % ksplit;
   const int ot = tile / tiles_in, it = tile % tiles_in;
   const int64_t steps = npad / 32;
@@ -501,17 +538,18 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(const uint16_t* __restrict__ 
 // entries stay zero.  Writes the BF16 forward (W^T) and backward (W) copies.
 __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ w, float* __restrict__ m,
                                                   float* __restrict__ v,
-                                                  const float* __restrict__ g, float lr, float b1,
+                                                  float* __restrict__ g, float lr, float b1,
                                                   float b2, float eps, float c1, float c2,
                                                   uint16_t* __restrict__ wt,
                                                   uint16_t* __restrict__ wb) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= kParams) return;
+  const float gr = g[p];
+  g[p] = 0.0f;  // the next step's weight gradients accumulate from zero
   int l = 0;
   while (p >= kOffset[l + 1]) ++l;
   const int o = (p - kOffset[l]) / kInDim[l], i = (p - kOffset[l]) % kInDim[l];
   if (o >= real_out(l) || i >= real_in(l)) return;
-  const float gr = g[p];
   const float mm = b1 * m[p] + (1.0f - b1) * gr;
   const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
   m[p] = mm;
@@ -643,30 +681,32 @@ lisa_status ensure_train(lisa_cache* c, int64_t npad) {
 }
 
 // Gradients of the mean relative-L2 loss over n device-resident samples
-// into c->g (zeroed first); loss into c->loss.
+// accumulated into c->g, which must be zero (k_mlp_adam leaves it zero; the
+// caller zeroes it before a first step); the loss into *loss_slot (zeroed
+// by the caller).  Two launches: k_mlp_train, k_mlp_wgrad.
 lisa_status grads_device(lisa_cache* c, const float* d_feat, int feat_stride, const float* d_tgt,
-                         int64_t n) {
+                         int64_t n, float* loss_slot) {
   const int64_t npad = (n + 31) / 32 * 32;
   lisa_status rc = ensure_train(c, npad);
   if (rc != LISA_OK) return rc;
   hipStream_t st = c->stream;
-  k_zero_f32<<<64, 256, 0, st>>>(c->g, kParams);
-  k_zero_f32<<<1, 256, 0, st>>>(c->loss, 1);
   const int64_t tiles = npad / 16;
-  const int blocks = (int)std::min<int64_t>((tiles + kTrainWaves - 1) / kTrainWaves, 2048);
+  const int blocks = (int)std::min<int64_t>((tiles + kTrainWaves - 1) / kTrainWaves, c->cus);
   k_mlp_train<<<blocks, 64 * kTrainWaves, 0, st>>>(d_feat, feat_stride, d_tgt, n, npad,
                                                     1.0f / (3.0f * (float)n), c->wt, c->wb, c->act,
-                                                    c->dz, c->loss);
+                                                    c->dz, loss_slot);
+  WgradPlan plan;
+  plan.ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(npad / 32 / 16, 64));
   int64_t off_a = 0, off_d = 0;
+  plan.first[0] = 0;
   for (int l = 0; l < kLayers; ++l) {
-    const int tiles_l = (kOutDim[l] / 16) * (kInDim[l] / 16);
-    const int ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(npad / 32 / 16, 64));
-    const int waves = tiles_l * ksplit;
-    k_mlp_wgrad<<<(waves + 3) / 4, 256, 0, st>>>(c->dz + off_d, c->act + off_a, npad, kOutDim[l],
-                                                 kInDim[l], ksplit, c->g + kOffset[l]);
+    plan.off_dz[l] = off_d;
+    plan.off_in[l] = off_a;
+    plan.first[l + 1] = plan.first[l] + (kOutDim[l] / 16) * (kInDim[l] / 16) * plan.ksplit;
     off_a += (int64_t)kInDim[l] * npad;
     off_d += (int64_t)kOutDim[l] * npad;
   }
+  k_mlp_wgrad<<<(plan.first[kLayers] + 3) / 4, 256, 0, st>>>(c->dz, c->act, npad, plan, c->g);
   CHIP(c, hipGetLastError());
   return LISA_OK;
 }
@@ -688,8 +728,7 @@ lisa_status lisa_cache_create(lisa_cache** out, int32_t device, uint64_t seed, f
       (e = hipMalloc(reinterpret_cast<void**>(&c->w), 4 * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->m), 4 * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->v), 4 * kParams)) != hipSuccess ||
-      (e = hipMalloc(reinterpret_cast<void**>(&c->g), 4 * kParams)) != hipSuccess ||
-      (e = hipMalloc(reinterpret_cast<void**>(&c->loss), 4)) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->g), 4 * (kParams + kLossSlots))) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->wt), 2 * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->wb), 2 * kWbGlobal)) != hipSuccess ||
       (e = hipMemset(c->m, 0, 4 * kParams)) != hipSuccess ||
@@ -700,6 +739,7 @@ lisa_status lisa_cache_create(lisa_cache** out, int32_t device, uint64_t seed, f
     lisa_cache_destroy(c);
     return cfail(nullptr, LISA_ERR_HIP, msg);
   }
+  c->loss = c->g + kParams;  // the per-step loss slots follow the gradients (one zeroing launch)
   std::vector<float> w;
   init_weights(w, seed);
   lisa_status rc = upload_weights(c, w);
@@ -716,7 +756,7 @@ void lisa_cache_destroy(lisa_cache* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (void* p : {(void*)c->w, (void*)c->m, (void*)c->v, (void*)c->g, (void*)c->loss, (void*)c->wt,
+  for (void* p : {(void*)c->w, (void*)c->m, (void*)c->v, (void*)c->g, (void*)c->wt,
                   (void*)c->wb, (void*)c->act, (void*)c->dz, (void*)c->feat, (void*)c->tgt,
                   (void*)c->out})
     (void)hipFree(p);
@@ -775,7 +815,8 @@ lisa_status lisa_cache_gradients(lisa_cache* c, const float* feat, const float* 
   if (rc != LISA_OK) return rc;
   CHIP(c, hipMemcpy(c->feat, feat, 4 * kInputs * (size_t)n, hipMemcpyHostToDevice));
   CHIP(c, hipMemcpy(c->tgt, target, 4 * kOut * (size_t)n, hipMemcpyHostToDevice));
-  rc = grads_device(c, c->feat, kInputs, c->tgt, n);
+  k_zero_f32<<<64, 256, 0, c->stream>>>(c->g, kParams + kLossSlots);
+  rc = grads_device(c, c->feat, kInputs, c->tgt, n, c->loss);
   if (rc != LISA_OK) return rc;
   CHIP(c, hipStreamSynchronize(c->stream));
   if (grad) CHIP(c, hipMemcpy(grad, c->g, 4 * kParams, hipMemcpyDeviceToHost));
@@ -792,10 +833,14 @@ lisa_status lisa::cache_train_device(lisa_cache* c, const float* d_feat, int fea
                                      const float* d_tgt, int64_t n, float* loss, int steps) {
   if (!c) return cfail(nullptr, LISA_ERR_INVALID_ARG, "cache is NULL");
   if (n <= 0 || steps <= 0) return cfail(c, LISA_ERR_INVALID_ARG, "n and steps must be > 0");
+  if (steps > kLossSlots) return cfail(c, LISA_ERR_INVALID_ARG, "at most 64 training steps per call");
   CHIP(c, hipSetDevice(c->device));
   CHIP(c, hipEventRecord(c->ev0, c->stream));
+  // gradients and every step's loss slot from zero (each Adam step leaves
+  // the gradients zero again): three launches per step
+  k_zero_f32<<<64, 256, 0, c->stream>>>(c->g, kParams + kLossSlots);
   for (int k = 0; k < steps; ++k) {
-    lisa_status rc = grads_device(c, d_feat, feat_stride, d_tgt, n);
+    lisa_status rc = grads_device(c, d_feat, feat_stride, d_tgt, n, c->loss + k);
     if (rc != LISA_OK) return rc;
     ++c->step;
     const float c1 = 1.0f - std::pow(c->b1, (float)c->step),
@@ -806,7 +851,7 @@ lisa_status lisa::cache_train_device(lisa_cache* c, const float* d_feat, int fea
   }
   CHIP(c, hipEventRecord(c->ev1, c->stream));
   if (loss) {
-    CHIP(c, hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));
+    CHIP(c, hipMemcpyAsync(loss, c->loss + (steps - 1), 4, hipMemcpyDeviceToHost, c->stream));
   }
   CHIP(c, hipStreamSynchronize(c->stream));
   CHIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));

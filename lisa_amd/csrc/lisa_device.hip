@@ -47,7 +47,7 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 // a private spill array.  The depth of a Karras tree over index-augmented
 // 30-bit keys is below 64, so kStackMax entries always suffice.
 #ifndef LISA_STACK_LDS
-#define LISA_STACK_LDS 24
+#define LISA_STACK_LDS 20
 #endif
 #ifndef LISA_PATH_BLOCK
 #define LISA_PATH_BLOCK 64
@@ -78,6 +78,14 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #endif
 constexpr int kStackLds = LISA_STACK_LDS;
 constexpr int kStackMax = 64;
+// LDS words per lane of a path kernel: the traversal stack, then the pixel's
+// running sum of sample radiance (3 floats, touched once per sample) and the
+// NEE light material (written when a NEE ray finds a light) -- in LDS so
+// that they hold no register through the path loop.  At 20 + 4 words a
+// 64-lane workgroup takes 6 KB, 24 of them (6 waves per SIMD) 144 KB
+// (6.5 KB per workgroup measured 7 % slower: fewer workgroups fit).  The
+// benchmark scenes' traversals need at most 18 stack entries.
+constexpr int kLaneLds = kStackLds + 4;
 // Path workgroup: kPathBlock work-items on a kTileW x kTileH pixel tile
 // (64: 8x8, 128: 16x8, 256: 16x16), so a wave covers a compact 8-row or
 // 4-row block of pixels whose camera rays start coherent.
@@ -165,14 +173,14 @@ __device__ __forceinline__ bool watertight(const RayT& r, F3 a, F3 b, F3 c, floa
   const float By = sel(B, r.ky) - r.Sy * Bkz;
   const float Cx = sel(C, r.kx) - r.Sx * Ckz;
   const float Cy = sel(C, r.ky) - r.Sy * Ckz;
-  float U = Cx * By - Cy * Bx;
-  float V = Ax * Cy - Ay * Cx;
-  float W = Bx * Ay - By * Ax;
-  if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
-  }
+  // FP32 edge functions only (no double-precision recomputation of a zero
+  // edge function): the edge shared by two triangles gets exactly negated
+  // values in the two (the same per-vertex products, subtracted in the other
+  // order), so a ray on either side is accepted by one of them and a ray on
+  // the edge (a zero) by both -- none falls through.
+  const float U = Cx * By - Cy * Bx;
+  const float V = Ax * Cy - Ay * Cx;
+  const float W = Bx * Ay - By * Ax;
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
   const float det = U + V + W;
   if (det == 0.0f) return false;
@@ -230,6 +238,20 @@ __device__ __forceinline__ bool emit_culled(float4 c, float4 H, F3 O, F3 D) {
   return line_culled(c, H, O, D);
 }
 
+// The prefix light sample's estimate p^ = q |cos_y| A / r^2 stays below 1/2
+// at O: q <= 3 sqrt(3) / 12 < 0.4331 (the cube-hemisphere density at the
+// cube's diagonal), |cos_y| <= 1, and every emitter point y lies in the box
+// (c, H), so r^2 >= d^2, d = O's distance to the box; 0.9 A <= d^2 gives
+// p^ <= 0.4331 / 0.9 < 0.49.  A function of the vertex alone, so choosing
+// the estimator by it changes no expectation.
+__device__ __forceinline__ bool light_sample_bounded(float4 c, float4 H, F3 O, float A) {
+  const float dx = fmaxf(fabsf(O.x - c.x) - H.x, 0.0f);
+  const float dy = fmaxf(fabsf(O.y - c.y) - H.y, 0.0f);
+  const float dz = fmaxf(fabsf(O.z - c.z) - H.z, 0.0f);
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  return 0.9f * A <= d2;
+}
+
 struct TravCount {
   uint64_t visits = 0, tests = 0;
 };
@@ -246,7 +268,7 @@ template <bool kCount, int kStride>
 __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, float tmin,
                                            float tlim, uint32_t lim_prim, bool any, float& t_out,
                                            uint32_t* stk, TravCount& cnt) {
-  volatile uint32_t spill[kStackMax - kStackLds];
+  uint32_t spill[kStackMax - kStackLds];
   float best_t = tlim;
   float cull_t = tlim * kWiden;
   uint32_t best_prim = lim_prim;
@@ -311,13 +333,15 @@ __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, flo
     } else {
       if (sp == 0) break;
       --sp;
-      // separate reads: the spill array is volatile, so the compiler neither
-      // keeps it in registers nor merges the two reads into one generic
-      // (flat) load through a selected pointer
-      if (sp < kStackLds)
+      if (sp < kStackLds) {
         node = stk[sp * kStride];
-      else
+      } else {
+        // an empty asm statement keeps the two reads apart, so the compiler
+        // does not merge them into one generic (flat) load through a
+        // selected LDS / private pointer
+        asm volatile("" ::: "memory");
         node = spill[sp - kStackLds];
+      }
     }
   }
   t_out = best_t;
@@ -397,7 +421,8 @@ struct PathArgs {
   int32_t nee_loops;        // kMode 3: NEE runs averaged at the camera ray's hit (>= 1)
   int32_t nee_loops_deep;   // kMode 3: ... at the later vertices (>= 1)
   const float* emit_cdf;    // area prefix sums of the emitting triangles (emitter tree order)
-  float emit_area;          // their total area
+  float emit_area;          // their total area (the last prefix sum)
+  float emit_area_below;    // the largest float below emit_area
   F3 bmin, binv;      // position normalisation for the features
 };
 
@@ -477,7 +502,11 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   uint64_t c_rad = 0, c_occ = 0, c_rhit = 0, c_inv = 0, c_smp = 0, c_px = 0;
   TravCount tc;
 
-  F3 accum = mk(0.0f, 0.0f, 0.0f);
+  // the pixel's sum of sample radiance, in LDS after the lane's stack
+  float* const acc = reinterpret_cast<float*>(my_stk + kStackLds * kPathBlock);
+  acc[0] = 0.0f;
+  acc[kPathBlock] = 0.0f;
+  acc[2 * kPathBlock] = 0.0f;
   double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0;
 
   // Per-lane path state (RayState, shader.cu:30-42, plus the NEE loop index).
@@ -486,7 +515,10 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   F3 O = mk(0.0f, 0.0f, 0.0f), D = O, color = O, att = O, N = O, Din = O;
   bool hitf = false;
   int32_t qstage = 0;  // 0 radiance ray, 1 NEE emitter query, 2 NEE any-hit query
-  int32_t light_mat = 0, tries = 0, matE = -1;
+  int32_t tries = 0, matE = -1;
+  // RayState::material of the NEE's light (shader.cu:38), in LDS after acc
+  int32_t* const light_mat = reinterpret_cast<int32_t*>(my_stk + (kStackLds + 3) * kPathBlock);
+  *light_mat = 0;
   uint32_t primE = 0xFFFFFFFFu;
   float tE = kTmax;
   float rough = 0.0f;
@@ -525,7 +557,9 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       t_used = 0;
     }
     if (kCount) c_smp++;
-    accum = accum + color;
+    acc[0] = acc[0] + color.x;
+    acc[kPathBlock] = acc[kPathBlock] + color.y;
+    acc[2 * kPathBlock] = acc[2 * kPathBlock] + color.z;
     if (kF64) {
       ad0 += (double)color.x;
       ad1 += (double)color.y;
@@ -599,7 +633,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // or give up after kNeeTries (shader.cu:200-208).
   auto nee_after = [&](int32_t code) -> bool {
     if (kRecord) hash = (hash ^ ((uint32_t)code ^ 0x5bd1e995u)) * 16777619u;
-    if (hitf) return end_nee(xyz(scp->mats[light_mat].emission_emit) * Bsdf::eval(N, D));
+    if (hitf) return end_nee(xyz(scp->mats[*light_mat].emission_emit) * Bsdf::eval(N, D));
     if (++tries == kNeeTries) return end_nee(mk(0.0f, 0.0f, 0.0f));
     D = hemisphere_raw(seed);  // finished in the emitter-query loop
     qstage = 1;
@@ -614,7 +648,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       out = mk((float)(ad0 * inv), (float)(ad1 * inv), (float)(ad2 * inv));
     } else {
       const float inv = 1.0f / (float)pp->spp;
-      out = accum * inv;
+      out = mk(acc[0], acc[kPathBlock], acc[2 * kPathBlock]) * inv;
     }
     if (kMode == 0 && pp->subframe > 0 && pp->accum_prev) {
       const float a = 1.0f / (float)(pp->subframe + 1);
@@ -635,7 +669,9 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       y = job.y;
       item = job.item;
       seed = tea16(y * pp->width + x, pp->subframe);
-      accum = mk(0.0f, 0.0f, 0.0f);
+      acc[0] = 0.0f;
+      acc[kPathBlock] = 0.0f;
+      acc[2 * kPathBlock] = 0.0f;
       ad0 = ad1 = ad2 = 0.0;
       s = 0;
       rec_base = (size_t)item * pp->spp;
@@ -693,13 +729,14 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       // then the light sample or the bounce
       if (++nl < (b > 1 ? pp->nee_loops_deep : pp->nee_loops)) {
         hitf = (ent & 1) != 0;
-        light_mat = ent >> 1;
+        *light_mat = ent >> 1;
         tries = 0;
         D = hemisphere_raw(seed);
         qstage = 1;
         continue;
       }
-      if (rough == 1.0f && scp->emit.T > 0 && pp->emit_area > 0.0f && b < pp->bounces) {
+      if (rough == 1.0f && scp->emit.T > 0 && pp->emit_area > 0.0f && b < pp->bounces &&
+          light_sample_bounded(scp->ecull_c, scp->ecull_h, O, pp->emit_area)) {
         // Expected emission of this vertex's bounce ray.  The reference adds
         // Le * att when the bounce (a hemisphere draw at roughness 1, density
         // q = 1 / (12 max|w_i|^3) on N's side) hits an emitter, and ends the
@@ -708,9 +745,14 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
         // a^ = Le p^; the path adds a^ * att, continues with att * (1 - p^),
         // and its bounce is drawn again whenever it hits an emitter (a draw on
         // the condition that it does not).  The expectation is the reference's.
+        // light_sample_bounded keeps p^ < 1/2 (so the throughput stays
+        // positive and the redraw limit ends a path with probability < 2^-64);
+        // vertices closer to the emitters take the reference's bounce.
         const float u0 = rnd(seed), u1 = rnd(seed), u2 = rnd(seed);
         const int32_t T = scp->emit.T;
-        const float ta = u0 * pp->emit_area;
+        // u0 * A can round up to A: keep ta below the last prefix sum, so the
+        // search ends on a triangle whose prefix sum grows (positive area)
+        const float ta = fminf(u0 * pp->emit_area, pp->emit_area_below);
         int32_t lo = 0, hi = T - 1;  // first triangle whose prefix sum exceeds ta
         while (lo < hi) {
           const int32_t mid = (lo + hi) >> 1;
@@ -877,7 +919,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
             qstage = 1;
             tries = 0;
             if (kEmit) {  // the NEE loops of this vertex start from this flag and light
-              ent = (light_mat << 1) | (hitf ? 1 : 0);
+              ent = (*light_mat << 1) | (hitf ? 1 : 0);
               nl = 0;
             }
             D = hemisphere_raw(seed);
@@ -903,7 +945,7 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
       int32_t code;
       if (matE >= 0) {
         if (!blocked) {
-          light_mat = matE;
+          *light_mat = matE;
           hitf = true;
           code = (int32_t)primE;
         } else {
@@ -961,7 +1003,7 @@ struct OnePixel {
 
 template <bool kRecord, bool kCount, bool kF64 = false>
 __global__ __launch_bounds__(kPathBlock, (kRecord || kCount) ? 4 : LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
-  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  __shared__ uint32_t stk[kLaneLds * kPathBlock];
   OnePixel src{sc, p, false};
   path_lane<kRecord, kCount, OnePixel, 0, kF64>(src, stk + threadIdx.x);
 }
@@ -971,7 +1013,7 @@ __global__ __launch_bounds__(kPathBlock, (kRecord || kCount) ? 4 : LISA_PATH_MIN
 template <int kMode>
 __global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : LISA_MODE_WAVES) void k_path_mode(
     SceneView sc, PathArgs p) {
-  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  __shared__ uint32_t stk[kLaneLds * kPathBlock];
   OnePixel src{sc, p, false};
   path_lane<false, false, OnePixel, kMode>(src, stk + threadIdx.x);
 }
@@ -1028,7 +1070,7 @@ struct PixelQueue {
 template <int kMode>
 __global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : 1) void k_path_pq(
     SceneView sc, PathArgs p, uint32_t* counter, uint32_t total, uint32_t chunk) {
-  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  __shared__ uint32_t stk[kLaneLds * kPathBlock];
   PixelQueue src{sc, p, counter, total, chunk, 0u, 0u};
   path_lane<false, false, PixelQueue, kMode>(src, stk + threadIdx.x);
 }
@@ -1138,7 +1180,7 @@ struct QueueSrc {
 __global__ __launch_bounds__(kPathBlock) void k_path_queue(const BatchScene* scenes, int32_t n,
                                                            uint32_t* counter, uint32_t total,
                                                            uint32_t chunk) {
-  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  __shared__ uint32_t stk[kLaneLds * kPathBlock];
   QueueSrc src;
   src.scenes = scenes;
   src.n_scenes = n;
@@ -1175,7 +1217,7 @@ struct TileSrc {
 
 __global__ __launch_bounds__(kPathBlock, LISA_TILE_WAVES) void k_path_tiles(const BatchScene* scenes, int32_t n,
                                                            uint32_t* counter, uint32_t total) {
-  __shared__ uint32_t stk[kStackLds * kPathBlock];
+  __shared__ uint32_t stk[kLaneLds * kPathBlock];
   const uint32_t n_tiles = total / 64u;
   for (;;) {
     uint32_t t = 0;
@@ -1198,6 +1240,491 @@ __global__ __launch_bounds__(kPathBlock, LISA_TILE_WAVES) void k_path_tiles(cons
 
 // Queue reset inside the captured graph (a kernel, not a memset node).
 __global__ void k_queue_reset(uint32_t* counter) { *counter = 0u; }
+
+// ---------------------------------------------------------------------------
+// Ray-level wavefront form (LISA_FLAG_WAVEFRONT, batch form 2).  The per-lane
+// state machine of path_lane is split at its one full traversal: every path
+// in flight keeps its state in a slot of HBM arrays (SoA, 16-byte records),
+// and each round of the host loop runs
+//   k_wf_trace: persistent; each wave holds a queue of traversal requests in
+//     LDS, filled 64 at a time from the round's request list, and every lane
+//     whose traversal ends takes the next request from it at once (dynamic
+//     refill), so lanes do not idle behind the wave's longest traversal;
+//   k_wf_shade: the closest-hit / miss / NEE logic of the request's result,
+//     then the path's cheap work up to its next full traversal (the NEE
+//     tries resolved by the emitter tree, the next sample or the next pixel
+//     of the work list); the paths that need a traversal are packed by a
+//     wave ballot and prefix sum into LDS and appended to the next round's
+//     list with one atomic per wave.
+// Each pixel's samples still run in order through its own RNG stream, so
+// every image is bit-identical to k_path's.
+// ---------------------------------------------------------------------------
+constexpr int kWfBlock = 64;   // one wave per workgroup
+constexpr int kWfStack = 20;   // LDS stack entries per lane of k_wf_trace (8 waves/SIMD: 5 KB each)
+constexpr uint32_t kWfAny = 0x80000000u;  // request bit: any-hit query (else closest hit)
+constexpr uint32_t kWfIdle = 0xFFFFFFFFu;
+
+// Slot state: seven float4 records (xyz + one 32-bit field in .w) and one
+// uint4 of integer state; results of the round's traversal in `res`.
+struct WfState {
+  float4* O;      // ray origin / hit point | tE (the occlusion query's limit)
+  float4* D;      // ray direction (raw NEE draws while trying) | primE
+  float4* col;    // radiance of the sample | roughness of the vertex
+  float4* att;    // throughput | matE
+  float4* N;      // shading normal | light material (RayState::material)
+  float4* Din;    // direction into the vertex | LCG state
+  float4* acc;    // the pixel's sum of samples | samples done
+  uint4* st;      // x: b | tries << 8 | qstage << 16 | hit flag << 20; y: work index (kWfIdle:
+                  // none); z: scene index; w: 0
+  uint2* res;     // sorted triangle index (or -1) and t of the round's traversal
+  uint32_t* q[2]; // request lists (slot | kWfAny)
+  uint32_t* cnt;  // [0], [1]: list lengths; [3]: next work index
+  uint32_t slots;
+};
+
+__device__ __forceinline__ const BatchScene* wf_scene_of(const BatchScene* scenes, int32_t n, uint32_t idx) {
+  int32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (scenes[mid].first <= idx)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return scenes + lo;
+}
+
+// One node visit of a resumable closest / any-hit walk (the loop body of
+// closest(): the same order of visits and tests, so the same answer and
+// counts).  Returns false when the walk has ended.
+struct WfWalk {
+  float best_t, cull_t;
+  uint32_t best_prim;
+  int32_t best_k;
+  uint32_t node;
+  int32_t sp;
+  uint32_t any;
+};
+
+template <int kStride>
+__device__ __forceinline__ bool wf_step(const BvhView& bv, const RayT& r, WfWalk& w, uint32_t* stk,
+                                        uint32_t* spill) {
+  const float4* np = bv.nodes + 4 * (size_t)w.node;
+  const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+  const uint32_t r0 = __float_as_uint(d.x), r1 = __float_as_uint(d.y);
+  float tn0 = 0.0f, tn1 = 0.0f;
+  bool h0 = (r0 != kEmpty) & slab(a.x, a.y, a.z, a.w, b.x, b.y, r, kTmin, w.cull_t, tn0);
+  bool h1 = (r1 != kEmpty) & slab(b.z, b.w, c.x, c.y, c.z, c.w, r, kTmin, w.cull_t, tn1);
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    const bool hit = ch == 0 ? h0 : h1;
+    const uint32_t ref = ch == 0 ? r0 : r1;
+    if (hit && (ref & kLeaf)) {
+      const uint32_t k = ref & ~kLeaf;
+      const float4* tp = bv.tris + 3 * (size_t)k;
+      const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+      const uint32_t prim = __float_as_uint(t0.w);
+      float t;
+      if (watertight(r, xyz(t0), xyz(t1), xyz(t2), t) && t > kTmin &&
+          (t < w.best_t || (t == w.best_t && prim < w.best_prim))) {
+        w.best_t = t;
+        w.cull_t = t * kWiden;
+        w.best_prim = prim;
+        w.best_k = (int32_t)k;
+      }
+      if (ch == 0)
+        h0 = false;
+      else
+        h1 = false;
+    }
+  }
+  if (w.any && w.best_k >= 0) return false;
+  if (h0 && h1) {
+    uint32_t nr = r0, fr = r1;
+    if (tn1 < tn0) {
+      nr = r1;
+      fr = r0;
+    }
+    if (w.sp == kStackMax) return false;  // unreachable for a valid tree (depth < 64)
+    if (w.sp < kWfStack) {
+      stk[w.sp * kStride] = fr;
+    } else {
+      asm volatile("" ::: "memory");
+      spill[w.sp - kWfStack] = fr;
+    }
+    ++w.sp;
+    w.node = nr;
+  } else if (h0) {
+    w.node = r0;
+  } else if (h1) {
+    w.node = r1;
+  } else {
+    if (w.sp == 0) return false;
+    --w.sp;
+    if (w.sp < kWfStack) {
+      w.node = stk[w.sp * kStride];
+    } else {
+      asm volatile("" ::: "memory");
+      w.node = spill[w.sp - kWfStack];
+    }
+  }
+  return true;
+}
+
+// k_wf_trace: the round's requests q[cur][0 .. cnt[cur]), split evenly over
+// the grid's waves (no shared counter: thousands of waves taking requests
+// from one atomic serialise on it).  A wave loads its share 64 requests at a
+// time into its LDS queue; a lane with no walk takes the queue's next request
+// (ballot + rank) whenever at least kRefill lanes are free or none is
+// walking.
+constexpr int kWfRefill = 16;
+__global__ __launch_bounds__(kWfBlock, 8) void k_wf_trace(const BatchScene* __restrict__ scenes, int32_t n_scenes,
+                                                          WfState ws, uint32_t cur) {
+  __shared__ uint32_t stk[kWfStack * kWfBlock];
+  __shared__ uint32_t lq[kWfBlock];
+  uint32_t spill[kStackMax - kWfStack];
+  uint32_t* my_stk = stk + threadIdx.x;
+  const uint32_t lane = threadIdx.x;
+  if (blockIdx.x == 0 && lane == 0) ws.cnt[cur ^ 1u] = 0u;  // the next list (k_wf_shade appends to it)
+  const uint32_t n = ws.cnt[cur];
+  const uint32_t* __restrict__ q = ws.q[cur];
+  // wave-uniform: this wave's share [next, last) and the LDS queue's unread range
+  uint32_t next = (uint32_t)((uint64_t)n * blockIdx.x / gridDim.x);
+  const uint32_t last = (uint32_t)((uint64_t)n * (blockIdx.x + 1) / gridDim.x);
+  uint32_t q_pos = 0, q_end = 0;
+  bool exhausted = false;
+  bool walking = false;
+  uint32_t slot = 0;
+  RayT r;
+  WfWalk w;
+  BvhView bv;
+  for (;;) {
+    const uint64_t free_lanes = __ballot(!walking);
+    const uint32_t n_free = (uint32_t)__popcll(free_lanes);
+    if (!exhausted && n_free > 0 && (n_free >= kWfRefill || n_free == 64u)) {
+      if (q_pos >= q_end) {  // the LDS queue is empty: the share's next 64 requests
+        if (next >= last) {
+          exhausted = true;
+        } else {
+          q_end = min((uint32_t)kWfBlock, last - next);
+          q_pos = 0;
+          if (lane < q_end) lq[lane] = q[next + lane];
+          next += q_end;
+          __syncthreads();
+        }
+      }
+      if (!exhausted) {
+        const uint32_t rank = (uint32_t)__popcll(free_lanes & ((1ull << lane) - 1ull));
+        const uint32_t avail = q_end - q_pos;
+        if (!walking && rank < avail) {
+          const uint32_t req = lq[q_pos + rank];
+          slot = req & ~kWfAny;
+          const float4 o4 = ws.O[slot], d4 = ws.D[slot];
+          const uint4 st = ws.st[slot];
+          const BatchScene* bs = scenes + st.z;
+          bv = bs->sc.all;
+          const F3 Dd = xyz(d4);
+          w.any = req & kWfAny;
+          w.best_t = w.any ? o4.w : kTmax;
+          w.cull_t = w.best_t * kWiden;
+          w.best_prim = w.any ? __float_as_uint(d4.w) : 0xFFFFFFFFu;
+          w.best_k = -1;
+          w.node = 0;
+          w.sp = 0;
+          if (ray_valid(Dd) && bv.T > 0) {
+            r = make_ray(xyz(o4), Dd);
+            walking = true;
+          } else {  // a zero or non-finite direction misses without traversal
+            ws.res[slot] = make_uint2(0xFFFFFFFFu, 0u);
+          }
+        }
+        q_pos += min(n_free, avail);
+      }
+    }
+    if (!__any(walking)) {
+      if (exhausted) break;
+      continue;
+    }
+    if (walking && !wf_step<kWfBlock>(bv, r, w, my_stk, spill)) {
+      ws.res[slot] = make_uint2((uint32_t)w.best_k, __float_as_uint(w.best_t));
+      walking = false;
+    }
+  }
+}
+
+// k_wf_shade (fresh = false): for each request of list `cur`, the result of
+// its traversal, then the path up to its next traversal; k_wf_start
+// (fresh = true): slots [0, slots) take their first pixel.  The steps and
+// their order are path_lane's (plain mode): the same RNG draws, the same
+// FP32 operations.
+template <class Bsdf = LISA_BSDF>
+__device__ __forceinline__ bool wf_advance(const BatchScene* __restrict__ scenes, int32_t n_scenes,
+                                           uint32_t total, const WfState& ws, uint32_t slot, bool fresh,
+                                           uint32_t* my_stk, uint32_t& req) {
+  const float4 o4 = ws.O[slot], d4 = ws.D[slot], c4 = ws.col[slot], a4 = ws.att[slot];
+  const float4 n4 = ws.N[slot], i4 = ws.Din[slot], ac4 = ws.acc[slot];
+  const uint4 st = ws.st[slot];
+  F3 O = xyz(o4), D = xyz(d4), color = xyz(c4), att = xyz(a4), N = xyz(n4), Din = xyz(i4);
+  F3 acc = xyz(ac4);
+  float tE = o4.w, rough = c4.w;
+  uint32_t primE = __float_as_uint(d4.w), seed = __float_as_uint(i4.w), s = __float_as_uint(ac4.w);
+  int32_t matE = (int32_t)__float_as_uint(a4.w), light_mat = (int32_t)__float_as_uint(n4.w);
+  uint32_t b = st.x & 0xFFu, tries = (st.x >> 8) & 0xFFu, qstage = (st.x >> 16) & 0xFu;
+  bool hitf = ((st.x >> 20) & 1u) != 0;
+  uint32_t widx = st.y, scene = fresh ? 0u : st.z;  // a fresh slot's state is not yet written
+  const BatchScene* bs = scenes + scene;
+  const SceneView* scp = &bs->sc;
+  const PathArgs* pp = &bs->p;
+  TravCount tc;
+
+  auto finish_sample = [&]() {
+    acc = acc + color;
+    ++s;
+  };
+  auto begin_sample = [&](uint32_t x, uint32_t y) -> bool {
+    while (s < pp->spp) {
+      const float jx = rng(seed);
+      const float jy = rng(seed);
+      const float dx = (2.0f * (float)x + jx) / (float)pp->width - 1.0f;
+      const float dy = (2.0f * (float)y + jy) / (float)pp->height - 1.0f;
+      D = normalize(dx * pp->U + dy * pp->V + pp->W);
+      O = pp->eye;
+      color = mk(0.0f, 0.0f, 0.0f);
+      att = mk(1.0f, 1.0f, 1.0f);
+      hitf = false;
+      qstage = 0;
+      b = 0;
+      if (pp->bounces > 0) return true;
+      finish_sample();
+    }
+    return false;
+  };
+  // the pixel of work index widx (x, y): false outside a partial edge tile
+  // (bs's own launch: during a fetch bs is already the new pixel's scene)
+  auto pixel_of = [&](uint32_t& x, uint32_t& y) -> bool {
+    const uint32_t local = widx - bs->first;
+    const uint32_t tile = local >> 6, wl = local & 63u;
+    const uint32_t tx = bs->p.tiles_x;
+    x = (tile % tx) * 8 + (wl & 7u);
+    y = (tile / tx) * 8 + (wl >> 3);
+    return x < bs->p.width && y < bs->p.height;
+  };
+  auto sample_xy = [&](uint32_t& x, uint32_t& y) { (void)pixel_of(x, y); };
+  auto end_nee = [&](F3 nee) -> bool {
+    color = color + nee * att;
+    D = Bsdf::sample(Din, N, seed, rough);
+    qstage = 0;
+    if (b == pp->bounces) {
+      finish_sample();
+      uint32_t x, y;
+      sample_xy(x, y);
+      return begin_sample(x, y);
+    }
+    return true;
+  };
+  auto nee_after = [&]() -> bool {
+    if (hitf) return end_nee(xyz(scp->mats[light_mat].emission_emit) * Bsdf::eval(N, D));
+    if (++tries == (uint32_t)kNeeTries) return end_nee(mk(0.0f, 0.0f, 0.0f));
+    D = hemisphere_raw(seed);
+    qstage = 1;
+    return true;
+  };
+
+  bool alive = true;
+  if (fresh) {
+    alive = false;
+    widx = kWfIdle;
+  } else {
+    const uint2 rk = ws.res[slot];
+    const int32_t k = (int32_t)rk.x;
+    const float t = __uint_as_float(rk.y);
+    if (qstage == 0) {  // radiance: miss (shader.cu:189-194) / closest hit (211-255)
+      ++b;
+      bool finish = false;
+      if (k < 0) {
+        color = color + mk(0.0f, 0.0f, 0.0f);
+        finish = true;
+      } else {
+        const float4 t0 = scp->all.tris[3 * (size_t)k], t1 = scp->all.tris[3 * (size_t)k + 1];
+        const DevMat m = scp->mats[__float_as_uint(t1.w)];
+        if (m.emission_emit.w != 0.0f) {
+          color = color + xyz(m.emission_emit) * att;
+          finish = true;
+        } else {
+          O = O + t * D;
+          Din = D;
+          {
+            const float4 t2 = scp->all.tris[3 * (size_t)k + 2];
+            const float4* nq = scp->nrm + 3 * (size_t)k;
+            N = bary_normal(O, xyz(nq[0]), xyz(nq[1]), xyz(nq[2]), xyz(t0), xyz(t1), xyz(t2));
+          }
+          if (m.diffuse_alpha.w < 1.0f) {
+            D = Dielectric::scatter(D, N, m.rough_n.y, xyz(m.diffuse_alpha), att, seed);
+            if (b == pp->bounces) finish = true;
+          } else {
+            att = att * xyz(m.diffuse_alpha);
+            rough = m.rough_n.x;
+            qstage = 1;
+            tries = 0;
+            D = hemisphere_raw(seed);
+          }
+        }
+      }
+      if (finish) {
+        finish_sample();
+        uint32_t x, y;
+        sample_xy(x, y);
+        alive = begin_sample(x, y);
+      }
+    } else {  // occlusion, second query: blocked before (tE, primE)?
+      const bool blocked = k >= 0;
+      if (matE >= 0) {
+        if (!blocked) {
+          light_mat = matE;
+          hitf = true;
+        }
+      } else if (!blocked) {
+        hitf = false;
+      }
+      alive = nee_after();
+    }
+  }
+  for (;;) {
+    if (!alive) {
+      if (widx != kWfIdle) {  // shader.cu:156-166: the pixel's mean
+        uint32_t x, y;
+        sample_xy(x, y);
+        const uint32_t item = y * pp->width + x;
+        F3 out = acc * (1.0f / (float)pp->spp);
+        if (pp->subframe > 0 && pp->accum_prev) {
+          const float a = 1.0f / (float)(pp->subframe + 1);
+          out = lerp(xyz(pp->accum_prev[item]), out, a);
+        }
+        pp->accum[item] = make_float4(out.x, out.y, out.z, 1.0f);
+      }
+      // the next pixel of the work list (one atomic per wave)
+      uint32_t x = 0, y = 0;
+      bool got = false;
+      for (;;) {
+        widx = wave_reserve(&ws.cnt[3]);
+        if (widx >= total) break;
+        bs = wf_scene_of(scenes, n_scenes, widx);
+        if (pixel_of(x, y)) {
+          got = true;
+          break;
+        }
+      }
+      if (!got) {  // no work left: the slot idles
+        ws.st[slot] = make_uint4(0u, kWfIdle, 0u, 0u);
+        return false;
+      }
+      scene = (uint32_t)(bs - scenes);
+      scp = &bs->sc;
+      pp = &bs->p;
+      seed = tea16(y * pp->width + x, pp->subframe);  // shader.cu:141
+      acc = mk(0.0f, 0.0f, 0.0f);
+      s = 0;
+      alive = begin_sample(x, y);
+      continue;
+    }
+    // NEE occlusion rays, first query: the closest emitter (emitter tree);
+    // with the flag false a ray that hits no emitter is resolved here
+    while (qstage == 1) {
+      if (!hitf && line_culled(scp->ecull_c, scp->ecull_h, O, D)) {
+        alive = nee_after();
+        if (!alive) break;
+        continue;
+      }
+      D = hemisphere_dir(D, N);
+      const bool valid = ray_valid(D);
+      int32_t k = -1;
+      float t = 0.0f;
+      if (valid && scp->emit.T > 0 && !emit_culled(scp->ecull_c, scp->ecull_h, O, D)) {
+        const RayT r = make_ray(O, D);
+        k = closest<false, kWfBlock>(scp->emit, r, kTmin, kTmax, 0xFFFFFFFFu, false, t, my_stk, tc);
+      }
+      if (!valid) {  // counts as a ray, misses without traversal
+        hitf = false;
+        alive = nee_after();
+      } else if (k >= 0) {  // an emitter: is anything in front of it?
+        const float4 e0 = scp->emit.tris[3 * (size_t)k], e1 = scp->emit.tris[3 * (size_t)k + 1];
+        tE = t;
+        primE = __float_as_uint(e0.w);
+        matE = (int32_t)__float_as_uint(e1.w);
+        qstage = 2;
+      } else if (!hitf) {
+        alive = nee_after();
+      } else {  // stale hit flag: a miss clears it, any other hit keeps it
+        tE = kTmax;
+        primE = 0xFFFFFFFFu;
+        matE = -1;
+        qstage = 2;
+      }
+      if (!alive) break;
+    }
+    if (!alive) continue;
+    break;
+  }
+  ws.O[slot] = make_float4(O.x, O.y, O.z, tE);
+  ws.D[slot] = make_float4(D.x, D.y, D.z, __uint_as_float(primE));
+  ws.col[slot] = make_float4(color.x, color.y, color.z, rough);
+  ws.att[slot] = make_float4(att.x, att.y, att.z, __uint_as_float((uint32_t)matE));
+  ws.N[slot] = make_float4(N.x, N.y, N.z, __uint_as_float((uint32_t)light_mat));
+  ws.Din[slot] = make_float4(Din.x, Din.y, Din.z, __uint_as_float(seed));
+  ws.acc[slot] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(s));
+  ws.st[slot] = make_uint4(b | (tries << 8) | (qstage << 16) | ((hitf ? 1u : 0u) << 20), widx, scene, 0u);
+  req = slot | (qstage == 2 ? kWfAny : 0u);
+  return true;
+}
+
+// Appends the requests of the lanes with `need` to list `nxt`: a wave ballot
+// and prefix sum place them in an LDS row, one atomic reserves the wave's
+// range, and the row is written out in one coalesced store.
+__device__ __forceinline__ void wf_enqueue(const WfState& ws, uint32_t nxt, bool need, uint32_t req,
+                                           uint32_t* row) {
+  const uint64_t m = __ballot(need);
+  if (m == 0) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t cnt = (uint32_t)__popcll(m);
+  if (need) row[__popcll(m & ((1ull << lane) - 1ull))] = req;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&ws.cnt[nxt], cnt);
+  base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+  __syncthreads();
+  if (lane < cnt) ws.q[nxt][base + lane] = row[lane];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(const BatchScene* __restrict__ scenes, int32_t n_scenes,
+                                                          uint32_t total, WfState ws, uint32_t cur) {
+  __shared__ uint32_t stk[(kStackLds + 1) * kWfBlock];
+  uint32_t* row = stk + kStackLds * kWfBlock;
+  const uint32_t n = ws.cnt[cur];
+  for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t req = 0;
+    bool need = false;
+    if (i < n) need = wf_advance(scenes, n_scenes, total, ws, ws.q[cur][i] & ~kWfAny, false, stk + threadIdx.x, req);
+    wf_enqueue(ws, cur ^ 1u, need, req, row);
+  }
+}
+
+__global__ __launch_bounds__(kWfBlock, 4) void k_wf_start(const BatchScene* __restrict__ scenes, int32_t n_scenes,
+                                                          uint32_t total, WfState ws) {
+  __shared__ uint32_t stk[(kStackLds + 1) * kWfBlock];
+  uint32_t* row = stk + kStackLds * kWfBlock;
+  const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+  uint32_t req = 0;
+  bool need = false;
+  if (i < ws.slots) need = wf_advance(scenes, n_scenes, total, ws, i, true, stk + threadIdx.x, req);
+  wf_enqueue(ws, 0u, need, req, row);
+}
+
+// Zeroes the list lengths and work counter before k_wf_start (a kernel, so
+// the launch sequence can be captured into a graph without memset nodes).
+__global__ void k_wf_reset(uint32_t* cnt) {
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0u;
+}
 
 // ---------------------------------------------------------------------------
 // LBVH build (Karras, HPG 2012).
@@ -1446,7 +1973,7 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
 
 __global__ __launch_bounds__(kBuildBlock) void k_karras(const uint32_t* __restrict__ keys,
                                                         int32_t n, float4* nodes, int32_t* parent,
-                                                        int32_t* leaf_parent) {
+                                                        int32_t* leaf_parent, uint2* range) {
   const int64_t i = (int64_t)blockIdx.x * kBuildBlock + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -1457,6 +1984,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_karras(const uint32_t* __restri
   for (int64_t t = lmax / 2; t >= 1; t /= 2)
     if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
   const int64_t j = i + l * d;
+  range[i] = make_uint2((uint32_t)(i < j ? i : j), (uint32_t)(i > j ? i : j));  // the node's leaves
   const int dnode = delta(keys, n, i, j);
   int64_t s = 0, t = 0, div = 2;
   do {
@@ -1499,44 +2027,66 @@ __device__ __forceinline__ void store_slot(float4* nodes, int32_t node, int slot
   for (int c = 0; c < 6; ++c) f[c] = b[c];
 }
 
-// Bottom-up refit: one work-item per leaf climbs while it is the second of
-// its parent's two children to arrive (agent-scope acq_rel counter per node).
-__global__ __launch_bounds__(kBuildBlock) void k_refit(const float4* __restrict__ tris, int32_t n,
-                                                       float4* nodes,
-                                                       const int32_t* __restrict__ parent,
-                                                       const int32_t* __restrict__ leaf_parent,
-                                                       uint32_t* flags) {
+// Refit by levels of a union table, without atomics: node i of a Karras
+// tree covers the sorted leaves range[i] = [lo, hi], so its box is the
+// union of their boxes.  Level k of the table holds, for every leaf i, the
+// union of the 2^k boxes from i on (clamped at the last leaf); a node with
+// L = hi - lo + 1 leaves, k = floor(log2 L), takes table_k[lo] | table_k[hi -
+// 2^k + 1] (two windows that cover the range).  Each level is one streaming
+// pass; the kernel boundary orders the levels.  min / max are exact, so the
+// boxes equal a bottom-up union's bit for bit.
+__device__ __forceinline__ void box_union(const float4* __restrict__ tab, uint32_t a, uint32_t c, float* b) {
+  const float4 l0 = tab[2 * (size_t)a], h0 = tab[2 * (size_t)a + 1];
+  const float4 l1 = tab[2 * (size_t)c], h1 = tab[2 * (size_t)c + 1];
+  b[0] = fminf(l0.x, l1.x);
+  b[1] = fminf(l0.y, l1.y);
+  b[2] = fminf(l0.z, l1.z);
+  b[3] = fmaxf(h0.x, h1.x);
+  b[4] = fmaxf(h0.y, h1.y);
+  b[5] = fmaxf(h0.z, h1.z);
+}
+
+// Level 0: the leaf boxes, also stored in their parents' child slots.
+__global__ __launch_bounds__(kBuildBlock) void k_rf_leaves(const float4* __restrict__ tris, int32_t n,
+                                                           float4* nodes,
+                                                           const int32_t* __restrict__ leaf_parent,
+                                                           float4* __restrict__ tab) {
   const int32_t k = blockIdx.x * kBuildBlock + threadIdx.x;
   if (k >= n) return;
   float b[6];
   leaf_box(tris, k, b);
-  int32_t p = leaf_parent[k];
+  tab[2 * (size_t)k] = make_float4(b[0], b[1], b[2], 0.0f);
+  tab[2 * (size_t)k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
+  const int32_t p = leaf_parent[k];
   if (p < 0 || p >= n - 1) return;
-  {
-    const uint32_t r0 = __float_as_uint(nodes[4 * (size_t)p + 3].x);
-    store_slot(nodes, p, r0 == (kLeaf | (uint32_t)k) ? 0 : 1, b);
+  const uint32_t r0 = __float_as_uint(nodes[4 * (size_t)p + 3].x);
+  store_slot(nodes, p, r0 == (kLeaf | (uint32_t)k) ? 0 : 1, b);
+}
+
+// Level k from level k - 1 (when `next`), and the boxes of the internal
+// nodes of level k - 1 into their parents' slots (the root has none).
+__global__ __launch_bounds__(kBuildBlock) void k_rf_level(int32_t n, int32_t k, const float4* __restrict__ prev,
+                                                          float4* __restrict__ next,
+                                                          const uint2* __restrict__ range,
+                                                          const int32_t* __restrict__ parent, float4* nodes) {
+  const int32_t i = blockIdx.x * kBuildBlock + threadIdx.x;
+  if (i >= n) return;
+  if (next) {
+    float b[6];
+    box_union(prev, (uint32_t)i, (uint32_t)min(i + (1 << (k - 1)), n - 1), b);
+    next[2 * (size_t)i] = make_float4(b[0], b[1], b[2], 0.0f);
+    next[2 * (size_t)i + 1] = make_float4(b[3], b[4], b[5], 0.0f);
   }
-  for (int level = 0; level < 128; ++level) {
-    const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    const float* f = reinterpret_cast<const float*>(nodes) + 16 * (size_t)p;
-    float u[12];
-    for (int c = 0; c < 12; ++c)
-      u[c] = __hip_atomic_load(&f[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b[0] = fminf(u[0], u[6]);
-    b[1] = fminf(u[1], u[7]);
-    b[2] = fminf(u[2], u[8]);
-    b[3] = fmaxf(u[3], u[9]);
-    b[4] = fmaxf(u[4], u[10]);
-    b[5] = fmaxf(u[5], u[11]);
-    if (p == 0) return;
-    const int32_t q = parent[p];
-    if (q < 0 || q >= n - 1) return;
-    const uint32_t r0 = __float_as_uint(nodes[4 * (size_t)q + 3].x);
-    store_slot(nodes, q, r0 == (uint32_t)p ? 0 : 1, b);
-    p = q;
-  }
+  if (i == 0 || i >= n - 1) return;
+  const uint2 r = range[i];
+  const int32_t lev = 31 - __clz((int)(r.y - r.x + 1));
+  if (lev != k - 1) return;
+  float b[6];
+  box_union(prev, r.x, r.y - (1u << lev) + 1u, b);
+  const int32_t p = parent[i];
+  if (p < 0 || p >= n - 1) return;  // unreachable for a valid tree
+  const uint32_t r0 = __float_as_uint(nodes[4 * (size_t)p + 3].x);
+  store_slot(nodes, p, r0 == (uint32_t)i ? 0 : 1, b);
 }
 
 // New positions (and normals) for sorted triangle k = global prim[k]; the
@@ -1563,13 +2113,6 @@ __global__ __launch_bounds__(kBuildBlock) void k_regather(const float* __restric
   }
 }
 
-// Plain copy (HBM calibration): float4 grid-stride.
-__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src,
-                                                    float4* __restrict__ dst, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-    dst[i] = src[i];
-}
-
 // T == 1: a root with the single leaf as child 0 and an empty child 1.
 __global__ void k_single_root(const float4* __restrict__ tris, float4* nodes) {
   float b[6];
@@ -1594,6 +2137,7 @@ struct DevBvh {
   uint32_t* prim = nullptr;       // global triangle ids, Morton order
   int32_t* parent = nullptr;      // parent of each internal node (kept for refits)
   int32_t* leaf_parent = nullptr; // parent of each leaf
+  uint2* range = nullptr;         // sorted leaves [lo, hi] of each internal node (kept for refits)
   uint64_t bytes = 0;
 };
 
@@ -1617,6 +2161,92 @@ struct DevBuf {
     cap = 0;
   }
 };
+
+// Device buffers of the wavefront form: the state of `slots` paths in
+// flight (WfState), the two request lists, the counters, and (for a single
+// render) the one-entry scene table.  Grown on demand, one allocation.
+struct WfBuffers {
+  void* mem = nullptr;
+  size_t bytes = 0;
+  BatchScene* d_scenes = nullptr;
+  int32_t scene_cap = 0;
+  WfState ws{};
+  hipError_t ensure(uint32_t slots) {
+    const size_t P = ((size_t)slots + 63) / 64 * 64;
+    const size_t want = P * (7 * 16 + 16 + 8 + 4 + 4) + 256;
+    if (!mem || want > bytes) {
+      (void)hipFree(mem);
+      mem = nullptr;
+      bytes = 0;
+      const hipError_t e = hipMalloc(&mem, want);
+      if (e != hipSuccess) return e;
+      bytes = want;
+    }
+    char* c = static_cast<char*>(mem);
+    float4** f4[7] = {&ws.O, &ws.D, &ws.col, &ws.att, &ws.N, &ws.Din, &ws.acc};
+    for (float4** f : f4) {
+      *f = reinterpret_cast<float4*>(c);
+      c += 16 * P;
+    }
+    ws.st = reinterpret_cast<uint4*>(c);
+    c += 16 * P;
+    ws.res = reinterpret_cast<uint2*>(c);
+    c += 8 * P;
+    ws.q[0] = reinterpret_cast<uint32_t*>(c);
+    c += 4 * P;
+    ws.q[1] = reinterpret_cast<uint32_t*>(c);
+    c += 4 * P;
+    ws.cnt = reinterpret_cast<uint32_t*>(c);
+    ws.slots = slots;
+    return hipSuccess;
+  }
+  hipError_t ensure_scenes(int32_t n) {
+    if (d_scenes && n <= scene_cap) return hipSuccess;
+    (void)hipFree(d_scenes);
+    d_scenes = nullptr;
+    scene_cap = 0;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_scenes), sizeof(BatchScene) * (size_t)n);
+    if (e == hipSuccess) scene_cap = n;
+    return e;
+  }
+  void release() {
+    (void)hipFree(mem);
+    (void)hipFree(d_scenes);
+    mem = nullptr;
+    d_scenes = nullptr;
+    bytes = 0;
+    scene_cap = 0;
+  }
+};
+
+// The rounds of the wavefront form on `st`: reset + start, then rounds of
+// (k_wf_trace, k_wf_shade) in blocks of kWfRoundsPerCheck, after each block
+// reading the request count (the loop ends when no path needs a traversal).
+constexpr int kWfRoundsPerCheck = 16;
+constexpr int kWfWavesPerCu = 32;   // k_wf_trace: 8 waves per SIMD
+constexpr int kWfShadePerCu = 16;   // k_wf_shade: 4 waves per SIMD
+hipError_t wf_enqueue_start(hipStream_t st, const BatchScene* d_scenes, int32_t n, uint32_t total,
+                            const WfState& ws) {
+  k_wf_reset<<<1, 64, 0, st>>>(ws.cnt);
+  k_wf_start<<<(ws.slots + kWfBlock - 1) / kWfBlock, kWfBlock, 0, st>>>(d_scenes, n, total, ws);
+  return hipGetLastError();
+}
+hipError_t wf_enqueue_rounds(hipStream_t st, const BatchScene* d_scenes, int32_t n, uint32_t total,
+                             const WfState& ws, int32_t trace_blocks, int32_t shade_blocks) {
+  uint32_t cur = 0;
+  for (int r = 0; r < kWfRoundsPerCheck; ++r) {  // an even count: list 0 is the input again
+    k_wf_trace<<<trace_blocks, kWfBlock, 0, st>>>(d_scenes, n, ws, cur);
+    k_wf_shade<<<shade_blocks, kWfBlock, 0, st>>>(d_scenes, n, total, ws, cur);
+    cur ^= 1u;
+  }
+  return hipGetLastError();
+}
+// Reads the pending request count after a block of rounds (list 0).
+hipError_t wf_pending(hipStream_t st, const WfState& ws, uint32_t* pending) {
+  hipError_t e = hipMemcpyAsync(pending, ws.cnt, 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
+}
 
 struct lisa_ctx {
   int32_t device = 0;
@@ -1652,9 +2282,18 @@ struct lisa_ctx {
   std::vector<double> j_sum, j_sq, j_csum, j_csq;
   uint32_t j_n = 0, j_f = 0;
   int64_t j_from = -1;
+  // the cached sequence in the accum buffer: started by a successful cached
+  // frame 0 (false after any other write of the buffer), with j_nj judge
+  // pixels; later frames must continue it with the same count
+  bool j_seq = false;
+  uint32_t j_nj = 0;
+  // bumped whenever a device buffer a batch captured may have moved or its
+  // captured values changed (scene, refit, accum buffer)
+  uint64_t gen = 0;
   DevBuf<int32_t> c_jpix;
   DevBuf<float4> c_jout, c_jacc;
   DevBuf<float> c_jrgb;
+  WfBuffers wf;  // the wavefront form of lisa_render (LISA_FLAG_WAVEFRONT)
 };
 
 struct lisa_batch {
@@ -1662,14 +2301,20 @@ struct lisa_batch {
   std::vector<lisa_ctx*> ctxs;
   std::vector<lisa_render_params> params;
   std::vector<size_t> accum_px;
+  std::vector<uint64_t> gens;  // each context's generation when the batch was made
   uint32_t total = 0;
   hipStream_t stream = nullptr;
   BatchScene* d_scenes = nullptr;
   uint32_t* d_counter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // form 2: the start and a block of rounds, captured separately (the host
+  // replays the rounds until no path needs a traversal)
+  hipGraph_t graph2 = nullptr;
+  hipGraphExec_t exec2 = nullptr;
+  WfBuffers wf;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  lisa_batch_config cfg{0, 1, 0};
+  lisa_batch_config cfg{0, 1, 0, 1};
   float last_ms = 0.0f;
 };
 
@@ -1697,10 +2342,13 @@ void free_bvh(DevBvh& b) {
   (void)hipFree(b.prim);
   (void)hipFree(b.parent);
   (void)hipFree(b.leaf_parent);
+  (void)hipFree(b.range);
   b = DevBvh();
 }
 
 void free_scene(lisa_ctx* c) {
+  c->gen++;
+  c->j_seq = false;
   free_bvh(c->all);
   free_bvh(c->emit);
   (void)hipFree(c->d_nrm);
@@ -1737,10 +2385,27 @@ struct Scratch {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// The context's accum buffer for npx pixels (reallocated: a new generation).
+lisa_status ensure_accum(lisa_ctx* ctx, size_t npx) {
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  (void)hipFree(ctx->d_accum);
+  ctx->d_accum = nullptr;
+  ctx->accum_px = 0;
+  ctx->gen++;
+  ctx->j_seq = false;
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_accum), npx * sizeof(float4)));
+  ctx->accum_px = npx;
+  ctx->accum_bytes = npx * sizeof(float4);
+  return LISA_OK;
+}
+
 lisa_status check_params(lisa_ctx* ctx, const lisa_render_params* p) {
   if (!p) return fail(ctx, LISA_ERR_INVALID_ARG, "render params is NULL");
   if ((p->flags & LISA_FLAG_PERSISTENT) && (p->flags & (LISA_FLAG_COUNT | LISA_FLAG_FP64_ACCUM)))
     return fail(ctx, LISA_ERR_UNSUPPORTED, "the persistent launch has no counting or FP64 form");
+  if ((p->flags & LISA_FLAG_WAVEFRONT) &&
+      (p->flags & (LISA_FLAG_COUNT | LISA_FLAG_FP64_ACCUM | LISA_FLAG_PERSISTENT)))
+    return fail(ctx, LISA_ERR_UNSUPPORTED, "the wavefront form has no counting, FP64 or persistent variant");
   if (p->width == 0 || p->height == 0) return fail(ctx, LISA_ERR_INVALID_ARG, "width/height must be > 0");
   if ((uint64_t)p->width * p->height > (1ull << 31))
     return fail(ctx, LISA_ERR_INVALID_ARG, "image larger than 2^31 pixels");
@@ -1766,6 +2431,35 @@ PathArgs make_args(const lisa_render_params* p) {
   return a;
 }
 
+// Every child box of b (T >= 2) from its triangles: the leaf level, then one
+// k_rf_level pass per level of the union table (scratch: two tables of T
+// entries, 32 bytes each).  Enqueued on ctx->stream; waits before returning
+// (the scratch is freed).
+// The same passes on caller-provided tables (2 * T float4 each); no allocation, no wait.
+lisa_status refit_levels_on(lisa_ctx* ctx, DevBvh& b, float4* ta, float4* tb) {
+  hipStream_t st = ctx->stream;
+  const int32_t n = b.T;
+  k_rf_leaves<<<grid_for(n, kBuildBlock), kBuildBlock, 0, st>>>(b.tris, n, b.nodes, b.leaf_parent, ta);
+  const int32_t K = 31 - __builtin_clz((unsigned)n);  // the root's level
+  for (int32_t k = 1; k <= K + 1; ++k) {
+    k_rf_level<<<grid_for(n, kBuildBlock), kBuildBlock, 0, st>>>(n, k, ta, k <= K ? tb : nullptr, b.range,
+                                                               b.parent, b.nodes);
+    std::swap(ta, tb);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  return LISA_OK;
+}
+lisa_status refit_levels(lisa_ctx* ctx, DevBvh& b) {
+  Scratch S;
+  float4 *ta = nullptr, *tb = nullptr;
+  HIP_TRY(ctx, S.alloc(&ta, 2 * (size_t)b.T));
+  HIP_TRY(ctx, S.alloc(&tb, 2 * (size_t)b.T));
+  lisa_status rc = refit_levels_on(ctx, b, ta, tb);
+  if (rc != LISA_OK) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the tables are freed on return
+  return LISA_OK;
+}
+
 // LBVH over the n triangles gid[0..n) (all T when gid is null) of the soup
 // already on the device: centroid bounds -> 30-bit Morton codes -> 4-pass LDS
 // radix sort -> Morton-ordered triangle records -> Karras hierarchy -> refit.
@@ -1782,13 +2476,14 @@ lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.prim), 4 * (size_t)std::max(n, 1)));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.parent), 4 * (size_t)std::max(n, 1)));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.leaf_parent), 4 * (size_t)std::max(n, 1)));
+  HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.range), 8 * (size_t)std::max(n, 1)));
   b.bytes = nodes_b + tris_b + 16 * (size_t)std::max(n, 1);
   HIP_TRY(ctx, hipMemsetAsync(b.nodes, 0, nodes_b, st));
   if (n == 0) return LISA_OK;
   Scratch S;
   float *d_part = nullptr, *d_bounds = nullptr;
   int32_t *d_parent = b.parent, *d_leafp = b.leaf_parent;
-  uint32_t *d_k2 = nullptr, *d_v2 = nullptr, *d_hist = nullptr, *d_tot = nullptr, *d_flags = nullptr;
+  uint32_t *d_k2 = nullptr, *d_v2 = nullptr, *d_hist = nullptr, *d_tot = nullptr;
   const int32_t n_tiles = (n + kSortTile - 1) / kSortTile;
   const int32_t n_part = std::min<int32_t>(1024, (n + kBuildBlock - 1) / kBuildBlock);
   HIP_TRY(ctx, S.alloc(&d_part, 6 * (size_t)n_part));
@@ -1797,7 +2492,6 @@ lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   HIP_TRY(ctx, S.alloc(&d_v2, (size_t)n));
   HIP_TRY(ctx, S.alloc(&d_hist, 256 * (size_t)n_tiles));
   HIP_TRY(ctx, S.alloc(&d_tot, 256));
-  HIP_TRY(ctx, S.alloc(&d_flags, (size_t)n));
   k_centroid_partial<<<n_part, kBuildBlock, 0, st>>>(d_v, d_gid, n, d_part);
   k_centroid_final<<<1, kBuildBlock, 0, st>>>(d_part, n_part, d_bounds);
   k_morton<<<grid_for(n, kBuildBlock), kBuildBlock, 0, st>>>(d_v, d_gid, n, d_bounds, b.morton, b.prim);
@@ -1816,13 +2510,12 @@ lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   if (n == 1) {
     k_single_root<<<1, 1, 0, st>>>(b.tris, b.nodes);
   } else {
-    HIP_TRY(ctx, hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * (size_t)n, st));
     HIP_TRY(ctx, hipMemsetAsync(d_parent, 0xFF, sizeof(int32_t) * (size_t)n, st));
     HIP_TRY(ctx, hipMemsetAsync(d_leafp, 0xFF, sizeof(int32_t) * (size_t)n, st));
     k_karras<<<grid_for(n - 1, kBuildBlock), kBuildBlock, 0, st>>>(b.morton, n, b.nodes, d_parent,
-                                                                  d_leafp);
-    k_refit<<<grid_for(n, kBuildBlock), kBuildBlock, 0, st>>>(b.tris, n, b.nodes, d_parent, d_leafp,
-                                                             d_flags);
+                                                                  d_leafp, b.range);
+    lisa_status rc = refit_levels(ctx, b);
+    if (rc != LISA_OK) return rc;
   }
   HIP_TRY(ctx, hipGetLastError());
   // scratch is freed on return: wait for the kernels that use it
@@ -1839,13 +2532,8 @@ lisa_status refit_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   if (b.T == 1) {
     k_single_root<<<1, 1, 0, st>>>(b.tris, b.nodes);
   } else {
-    Scratch S;
-    uint32_t* d_flags = nullptr;
-    HIP_TRY(ctx, S.alloc(&d_flags, (size_t)b.T));
-    HIP_TRY(ctx, hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * (size_t)b.T, st));
-    k_refit<<<grid_for(b.T, kBuildBlock), kBuildBlock, 0, st>>>(b.tris, b.T, b.nodes, b.parent,
-                                                               b.leaf_parent, d_flags);
-    HIP_TRY(ctx, hipGetLastError());
+    lisa_status rc = refit_levels(ctx, b);
+    if (rc != LISA_OK) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(st));
   }
   HIP_TRY(ctx, hipGetLastError());
@@ -1877,18 +2565,69 @@ lisa_status launch_persistent(lisa_ctx* ctx, PathArgs a, int slot) {
   return LISA_OK;
 }
 
-// Scene box of the cache features: union of the root's two child boxes.
-// Area prefix sums of the emitting triangles in emitter-tree order (one
-// thread: the emitter set is small, and the sums are sequential).
-__global__ void k_emit_cdf(const float4* __restrict__ tris, int32_t T, float* cdf) {
+// The plain render `a` (full image) through the wavefront form: the work
+// list is the image's 8x8 tiles; `a.tiles_x` counts 8-pixel tiles here.
+lisa_status render_wavefront(lisa_ctx* ctx, PathArgs a) {
+  hipStream_t st = ctx->stream;
+  a.tiles_x = (a.width + 7) / 8;
+  const uint32_t total = a.tiles_x * ((a.height + 7) / 8) * 64u;
+  const uint32_t slots = std::min<uint32_t>(total, (uint32_t)ctx->cus * kWfWavesPerCu * kWfBlock * 2u);
+  HIP_TRY(ctx, ctx->wf.ensure(slots));
+  HIP_TRY(ctx, ctx->wf.ensure_scenes(1));
+  BatchScene hs;
+  hs.sc = view(ctx);
+  hs.p = a;
+  hs.first = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->wf.d_scenes, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, wf_enqueue_start(st, ctx->wf.d_scenes, 1, total, ctx->wf.ws));
+  // every path ends: at most spp * (bounces * (kNeeTries + 1) + 1) + 1
+  // traversals per pixel, pixels in turn on at most `total` slots
+  const uint64_t max_rounds = (uint64_t)a.spp * ((uint64_t)a.bounces * (kNeeTries + 1) + 1) *
+                              (((uint64_t)total + slots - 1) / slots) + 2 * kWfRoundsPerCheck;
+  for (uint64_t r = 0;; r += kWfRoundsPerCheck) {
+    if (r > max_rounds) return fail(ctx, LISA_ERR_HIP, "wavefront form: paths did not finish");
+    HIP_TRY(ctx, wf_enqueue_rounds(st, ctx->wf.d_scenes, 1, total, ctx->wf.ws, ctx->cus * kWfWavesPerCu,
+                                   ctx->cus * kWfShadePerCu));
+    uint32_t pending = 0;
+    HIP_TRY(ctx, wf_pending(st, ctx->wf.ws, &pending));
+    if (pending == 0) break;
+  }
+  return LISA_OK;
+}
+
+// Area prefix sums of the emitting triangles in emitter-tree order, in a
+// fixed FP32 summation order: chunks of kCdfChunk triangles are summed
+// sequentially from 0 (p_k, one work-item per chunk), the chunk totals
+// sequentially into bases b_c (one work-item: T / 64 additions), and
+// cdf[k] = b_c + p_k.  With one chunk these are the plain sequential sums.
+constexpr int32_t kCdfChunk = 64;
+__global__ __launch_bounds__(256) void k_emit_chunks(const float4* __restrict__ tris, int32_t T,
+                                                     float* cdf, float* chunk_tot) {
+  const int32_t c = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+  const int32_t k0 = c * kCdfChunk;
+  if (k0 >= T) return;
+  const int32_t k1 = min(k0 + kCdfChunk, T);
   float acc = 0.0f;
-  for (int32_t k = 0; k < T; ++k) {
+  for (int32_t k = k0; k < k1; ++k) {
     const F3 v0 = xyz(tris[3 * (size_t)k]), v1 = xyz(tris[3 * (size_t)k + 1]), v2 = xyz(tris[3 * (size_t)k + 2]);
     const F3 e1 = v1 - v0, e2 = v2 - v0;
-    const F3 c = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
-    acc = acc + 0.5f * sqrtf(dot(c, c));
+    const F3 x = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+    acc = acc + 0.5f * sqrtf(dot(x, x));
     cdf[k] = acc;
   }
+  chunk_tot[c] = acc;
+}
+__global__ void k_emit_bases(float* chunk_tot, int32_t n_chunks) {
+  float base = 0.0f;
+  for (int32_t c = 0; c < n_chunks; ++c) {
+    const float t = chunk_tot[c];
+    chunk_tot[c] = base;
+    base = base + t;
+  }
+}
+__global__ __launch_bounds__(256) void k_emit_add(float* cdf, int32_t T, const float* __restrict__ base) {
+  const int32_t k = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+  if (k < T) cdf[k] = base[k / kCdfChunk] + cdf[k];
 }
 
 // Union of the child boxes of a root node (its 16 words).
@@ -1911,7 +2650,7 @@ void root_box(const float* rb, float lo[3], float hi[3]) {
 // The scene box (root of `all`) and the emitter cull box (emit_culled): the
 // emitter root box inflated by S * 2^-11 per side, S the largest |coordinate|
 // of the scene box.
-lisa_status read_scene_box(lisa_ctx* ctx) {
+lisa_status read_scene_box(lisa_ctx* ctx, bool may_alloc = true) {
   const int64_t T = ctx->all.T;
   for (int a = 0; a < 3; ++a) {
     ctx->ecull_c[a] = 0.0f;
@@ -1930,10 +2669,17 @@ lisa_status read_scene_box(lisa_ctx* ctx) {
   HIP_TRY(ctx, hipMemcpy(rb, ctx->emit.nodes, 64, hipMemcpyDeviceToHost));
   root_box(rb, lo, hi);
   // light sampling of LISA_CACHE_EMISSION: area prefix sums of the emitters
-  HIP_TRY(ctx, ctx->c_ecdf.ensure((size_t)ctx->emit.T));
-  k_emit_cdf<<<1, 1, 0, ctx->stream>>>(ctx->emit.tris, ctx->emit.T, ctx->c_ecdf.p);
+  const int32_t TE = ctx->emit.T, n_chunks = (TE + kCdfChunk - 1) / kCdfChunk;
+  if (may_alloc)
+    HIP_TRY(ctx, ctx->c_ecdf.ensure((size_t)TE + (size_t)n_chunks));
+  else if (ctx->c_ecdf.cap < (size_t)TE + (size_t)n_chunks)  // sized by lisa_set_scene for these emitters
+    return fail(ctx, LISA_ERR_ALLOC, "emitter prefix-sum buffer smaller than the scene's emitters");
+  float* chunk_tot = ctx->c_ecdf.p + TE;
+  k_emit_chunks<<<grid_for(n_chunks, 256), 256, 0, ctx->stream>>>(ctx->emit.tris, TE, ctx->c_ecdf.p, chunk_tot);
+  k_emit_bases<<<1, 1, 0, ctx->stream>>>(chunk_tot, n_chunks);
+  k_emit_add<<<grid_for(TE, 256), 256, 0, ctx->stream>>>(ctx->c_ecdf.p, TE, chunk_tot);
   HIP_TRY(ctx, hipGetLastError());
-  HIP_TRY(ctx, hipMemcpyAsync(&ctx->emit_area, ctx->c_ecdf.p + (ctx->emit.T - 1), 4, hipMemcpyDeviceToHost,
+  HIP_TRY(ctx, hipMemcpyAsync(&ctx->emit_area, ctx->c_ecdf.p + (TE - 1), 4, hipMemcpyDeviceToHost,
                               ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (int a = 0; a < 3; ++a) {
@@ -1957,6 +2703,7 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
   a.nee_loops_deep = 1;
   a.emit_cdf = ctx->c_ecdf.p;
   a.emit_area = ctx->emit_area;
+  a.emit_area_below = std::nextafter(ctx->emit_area, 0.0f);
   return a;
 }
 
@@ -1999,9 +2746,9 @@ lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, si
 // returns without waiting; trace_queries_finish waits for it, reads the
 // query count (*nq) and turns the query records into features.
 lisa_status trace_queries_launch(lisa_ctx* ctx, PathArgs& a, const int32_t* d_pix, size_t n, uint32_t prefix,
-                                 float4* frame, bool persistent) {
+                                 float4* frame, bool persistent, bool full = false) {
   hipStream_t st = ctx->stream;
-  const size_t qcap = n * a.spp;
+  const size_t qcap = full ? 1 : n * a.spp;
   HIP_TRY(ctx, ctx->c_qfeat.ensure(qcap * 12));
   HIP_TRY(ctx, ctx->c_qatt.ensure(qcap * 3));
   HIP_TRY(ctx, ctx->c_qout.ensure(qcap * 3));
@@ -2012,8 +2759,10 @@ lisa_status trace_queries_launch(lisa_ctx* ctx, PathArgs& a, const int32_t* d_pi
   a.n_items = (int32_t)n;
   a.accum = frame;
   a.accum_prev = nullptr;
-  a.bounces = prefix + 1;
-  a.cache_at = prefix + 1;
+  // full paths (LISA_CACHE_FULL_PATHS): a.bounces radiance rays and no
+  // query (radiance ray numbers start at 1)
+  if (!full) a.bounces = prefix + 1;
+  a.cache_at = full ? 0u : prefix + 1;
   a.q_feat = ctx->c_qfeat.p;
   a.q_att = ctx->c_qatt.p;
   a.q_item = ctx->c_qitem.p;
@@ -2053,8 +2802,8 @@ lisa_status trace_queries_finish(lisa_ctx* ctx, const PathArgs& a, uint32_t* nq)
 }
 
 lisa_status trace_queries(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, uint32_t prefix,
-                          float4* frame, uint32_t* nq, bool persistent = false) {
-  lisa_status rc = trace_queries_launch(ctx, a, d_pix, n, prefix, frame, persistent);
+                          float4* frame, uint32_t* nq, bool persistent = false, bool full = false) {
+  lisa_status rc = trace_queries_launch(ctx, a, d_pix, n, prefix, frame, persistent, full);
   if (rc != LISA_OK) return rc;
   return trace_queries_finish(ctx, a, nq);
 }
@@ -2085,6 +2834,7 @@ lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float*
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
+  ctx->gen++;  // the cull box a batch captured may have changed
   return read_scene_box(ctx);
 }
 
@@ -2132,6 +2882,7 @@ void lisa_destroy(lisa_ctx* c) {
   c->c_jout.release();
   c->c_jacc.release();
   c->c_jrgb.release();
+  c->wf.release();
   free_scene(c);
   (void)hipFree(c->d_accum);
   (void)hipFree(c->d_counters);
@@ -2217,14 +2968,10 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
   if (ctx->accum_px != npx) {
     if (p->subframe > 0)
       return fail(ctx, LISA_ERR_INVALID_ARG, "subframe > 0 but the accum buffer has another size");
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(ctx->d_accum);
-    ctx->d_accum = nullptr;
-    ctx->accum_px = 0;
-    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_accum), npx * sizeof(float4)));
-    ctx->accum_px = npx;
-    ctx->accum_bytes = npx * sizeof(float4);
+    rc = ensure_accum(ctx, npx);
+    if (rc != LISA_OK) return rc;
   }
+  ctx->j_seq = false;  // the buffer now holds a plain render
   PathArgs a = make_args(p);
   a.tiles_x = (p->width + kTileW - 1) / kTileW;
   const uint32_t tiles_y = (p->height + kTileH - 1) / kTileH;
@@ -2238,7 +2985,10 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters, 0, LISA_NCOUNTERS * sizeof(unsigned long long), st));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
   const unsigned grid = a.tiles_x * tiles_y;
-  if (p->flags & LISA_FLAG_PERSISTENT) {
+  if (p->flags & LISA_FLAG_WAVEFRONT) {
+    lisa_status rc2 = render_wavefront(ctx, a);
+    if (rc2 != LISA_OK) return rc2;
+  } else if (p->flags & LISA_FLAG_PERSISTENT) {
     lisa_status rc2 = launch_persistent<0>(ctx, a, 4);
     if (rc2 != LISA_OK) return rc2;
   } else if (count && a.fp64)
@@ -2406,29 +3156,6 @@ lisa_status lisa_copy_accum(lisa_ctx* ctx, float* rgba_out, int64_t n_pixels) {
   return LISA_OK;
 }
 
-lisa_status lisa_copy_probe(lisa_ctx* ctx, int64_t bytes, int32_t iters, float* avg_ms) {
-  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
-  if (bytes < 16 || iters < 1 || !avg_ms) return fail(ctx, LISA_ERR_INVALID_ARG, "bad copy probe args");
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
-  const int64_t n = bytes / 16;
-  Scratch S;
-  float4 *a = nullptr, *b = nullptr;
-  HIP_TRY(ctx, S.alloc(&a, (size_t)n));
-  HIP_TRY(ctx, S.alloc(&b, (size_t)n));
-  HIP_TRY(ctx, hipMemsetAsync(a, 0, (size_t)n * 16, ctx->stream));
-  hipStream_t st = ctx->stream;
-  k_copy_probe<<<4096, 256, 0, st>>>(a, b, n);  // warm
-  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  for (int32_t i = 0; i < iters; ++i) k_copy_probe<<<4096, 256, 0, st>>>(a, b, n);
-  HIP_TRY(ctx, hipGetLastError());
-  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  float ms = 0.0f;
-  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  *avg_ms = ms / (float)iters;
-  return LISA_OK;
-}
-
 lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,
                               const lisa_render_params* params, int32_t n) {
   if (!out) return fail(nullptr, LISA_ERR_INVALID_ARG, "out is NULL");
@@ -2454,15 +3181,9 @@ lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,
       if (ctxs[k] == c) return fail(nullptr, LISA_ERR_INVALID_ARG, "a context appears twice in the batch");
     if (hipSetDevice(dev) != hipSuccess) return fail(nullptr, LISA_ERR_HIP, "hipSetDevice");
     const size_t npx = (size_t)params[i].width * params[i].height;
-    if (c->accum_px != npx) {
-      (void)hipStreamSynchronize(c->stream);
-      (void)hipFree(c->d_accum);
-      c->d_accum = nullptr;
-      c->accum_px = 0;
-      if (hipMalloc(reinterpret_cast<void**>(&c->d_accum), npx * sizeof(float4)) != hipSuccess)
-        return fail(nullptr, LISA_ERR_ALLOC, "accum buffer allocation failed");
-      c->accum_px = npx;
-      c->accum_bytes = npx * sizeof(float4);
+    if (c->accum_px != npx && ensure_accum(c, npx) != LISA_OK) {
+      lisa::set_global_error("batch scene " + std::to_string(i) + ": " + c->err);
+      return LISA_ERR_ALLOC;
     }
     PathArgs a = make_args(&params[i]);
     a.tiles_x = (params[i].width + 7) / 8;
@@ -2480,7 +3201,11 @@ lisa_status lisa_batch_create(lisa_batch** out, lisa_ctx* const* ctxs,
   b->ctxs.assign(ctxs, ctxs + n);
   b->params.assign(params, params + n);
   b->accum_px.resize(n);
-  for (int32_t i = 0; i < n; ++i) b->accum_px[i] = ctxs[i]->accum_px;
+  b->gens.resize(n);
+  for (int32_t i = 0; i < n; ++i) {
+    b->accum_px[i] = ctxs[i]->accum_px;
+    b->gens[i] = ctxs[i]->gen;
+  }
   b->device = dev;
   b->total = (uint32_t)total;
   b->stream = ctxs[0]->stream;
@@ -2507,6 +3232,9 @@ void lisa_batch_destroy(lisa_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->exec2) (void)hipGraphExecDestroy(b->exec2);
+  if (b->graph2) (void)hipGraphDestroy(b->graph2);
+  b->wf.release();
   (void)hipFree(b->d_scenes);
   (void)hipFree(b->d_counter);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -2516,15 +3244,21 @@ void lisa_batch_destroy(lisa_batch* b) {
 
 lisa_status lisa_batch_set_config(lisa_batch* b, const lisa_batch_config* cfg) {
   if (!b || !cfg) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL argument");
-  if (cfg->grid_blocks <= 0 || cfg->chunk <= 0 || cfg->chunk > 4096 || cfg->form < 0 || cfg->form > 1)
+  if (cfg->grid_blocks <= 0 || cfg->chunk <= 0 || cfg->chunk > 4096 || cfg->form < 0 || cfg->form > 2 ||
+      (cfg->graph != 0 && cfg->graph != 1))
     return fail(nullptr, LISA_ERR_INVALID_ARG,
-                "grid_blocks and chunk must be positive (chunk <= 4096), form 0 or 1");
-  if (cfg->grid_blocks != b->cfg.grid_blocks || cfg->chunk != b->cfg.chunk || cfg->form != b->cfg.form) {
+                "grid_blocks and chunk must be positive (chunk <= 4096), form 0, 1 or 2, graph 0 or 1");
+  if (cfg->grid_blocks != b->cfg.grid_blocks || cfg->chunk != b->cfg.chunk || cfg->form != b->cfg.form ||
+      cfg->graph != b->cfg.graph) {
     b->cfg = *cfg;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->exec2) (void)hipGraphExecDestroy(b->exec2);
+    if (b->graph2) (void)hipGraphDestroy(b->graph2);
     b->exec = nullptr;
     b->graph = nullptr;
+    b->exec2 = nullptr;
+    b->graph2 = nullptr;
   }
   return LISA_OK;
 }
@@ -2535,31 +3269,99 @@ lisa_status lisa_batch_get_config(const lisa_batch* b, lisa_batch_config* cfg) {
   return LISA_OK;
 }
 
+// The batch's launch on its stream: the queue reset (a kernel, so a captured
+// graph holds no memset node) and the persistent path kernel of its form.
+static uint32_t wf_batch_slots(const lisa_batch* b) {
+  return (uint32_t)std::min<uint64_t>((uint64_t)b->total, (uint64_t)b->cfg.grid_blocks * kWfBlock * b->cfg.chunk);
+}
+static hipError_t enqueue_batch_rounds(lisa_batch* b) {
+  return wf_enqueue_rounds(b->stream, b->d_scenes, b->n, b->total, b->wf.ws, b->cfg.grid_blocks,
+                           b->cus * kWfShadePerCu);
+}
+static hipError_t enqueue_batch(lisa_batch* b) {
+  if (b->cfg.form == 2) {  // the start and the first block of rounds
+    const hipError_t e = wf_enqueue_start(b->stream, b->d_scenes, b->n, b->total, b->wf.ws);
+    return e != hipSuccess ? e : enqueue_batch_rounds(b);
+  }
+  k_queue_reset<<<1, 1, 0, b->stream>>>(b->d_counter);
+  if (b->cfg.form == 1)
+    k_path_tiles<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter, b->total);
+  else
+    k_path_queue<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter, b->total,
+                                                                   (uint32_t)b->cfg.chunk);
+  return hipGetLastError();
+}
+
 lisa_status lisa_batch_render(lisa_batch* b, float* elapsed_ms) {
   if (!b) return fail(nullptr, LISA_ERR_INVALID_ARG, "batch is NULL");
   for (int32_t i = 0; i < b->n; ++i)
-    if (b->ctxs[i]->accum_px != b->accum_px[i] || !b->ctxs[i]->has_scene)
-      return fail(nullptr, LISA_ERR_INVALID_ARG, "a context of the batch changed: re-create the batch");
+    if (b->ctxs[i]->accum_px != b->accum_px[i] || !b->ctxs[i]->has_scene || b->ctxs[i]->gen != b->gens[i])
+      return fail(nullptr, LISA_ERR_INVALID_ARG,
+                  "a context of the batch changed (scene, vertices or accum buffer): re-create the batch");
   if (hipSetDevice(b->device) != hipSuccess) return fail(nullptr, LISA_ERR_HIP, "hipSetDevice");
   hipError_t e;
-  if (!b->exec) {  // capture reset + persistent launch once, replay afterwards
-    if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-      return fail(nullptr, LISA_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
-    k_queue_reset<<<1, 1, 0, b->stream>>>(b->d_counter);
-    if (b->cfg.form == 1)
-      k_path_tiles<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter,
-                                                                     b->total);
-    else
-      k_path_queue<<<b->cfg.grid_blocks, kPathBlock, 0, b->stream>>>(b->d_scenes, b->n, b->d_counter,
-                                                                     b->total, (uint32_t)b->cfg.chunk);
-    e = hipStreamEndCapture(b->stream, &b->graph);
-    if (e != hipSuccess) return fail(nullptr, LISA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
-      return fail(nullptr, LISA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  if (!b->exec && b->cfg.graph) {  // capture reset + persistent launch once, replay afterwards
+    if (b->cfg.form == 2 && (e = b->wf.ensure(wf_batch_slots(b))) != hipSuccess)
+      return fail(nullptr, LISA_ERR_ALLOC, std::string("wavefront buffers: ") + hipGetErrorString(e));
+    e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t ce = enqueue_batch(b);
+      e = hipStreamEndCapture(b->stream, &b->graph);
+      if (e == hipSuccess) e = ce;
+      if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    }
+    if (e == hipSuccess && b->cfg.form == 2) {
+      e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        const hipError_t ce = enqueue_batch_rounds(b);
+        e = hipStreamEndCapture(b->stream, &b->graph2);
+        if (e == hipSuccess) e = ce;
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec2, b->graph2, nullptr, nullptr, 0);
+      }
+    }
+    if (e != hipSuccess) {
+      // capture refused (e.g. the caller is capturing this stream itself):
+      // plain launches from now on
+      (void)hipGetLastError();
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(b->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+      }
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      if (b->exec2) (void)hipGraphExecDestroy(b->exec2);
+      if (b->graph2) (void)hipGraphDestroy(b->graph2);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      b->exec2 = nullptr;
+      b->graph2 = nullptr;
+      b->cfg.graph = 0;
+    }
   }
-  if ((e = hipEventRecord(b->ev0, b->stream)) != hipSuccess ||
-      (e = hipGraphLaunch(b->exec, b->stream)) != hipSuccess ||
-      (e = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
+  if (b->cfg.form == 2 && (e = b->wf.ensure(wf_batch_slots(b))) != hipSuccess)
+    return fail(nullptr, LISA_ERR_ALLOC, std::string("wavefront buffers: ") + hipGetErrorString(e));
+  if ((e = hipEventRecord(b->ev0, b->stream)) != hipSuccess)
+    return fail(nullptr, LISA_ERR_HIP, std::string("lisa_batch_render: ") + hipGetErrorString(e));
+  if (b->exec)
+    e = hipGraphLaunch(b->exec, b->stream);
+  else
+    e = enqueue_batch(b);
+  if (e == hipSuccess && b->cfg.form == 2) {  // further blocks of rounds until every path has ended
+    uint64_t max_spp_b = 1;
+    for (const lisa_render_params& q : b->params)
+      max_spp_b = std::max<uint64_t>(max_spp_b, (uint64_t)q.spp * ((uint64_t)q.bounces * (kNeeTries + 1) + 1));
+    const uint32_t slots = b->wf.ws.slots;
+    const uint64_t max_rounds = max_spp_b * (((uint64_t)b->total + slots - 1) / slots) + 2 * kWfRoundsPerCheck;
+    for (uint64_t r = kWfRoundsPerCheck; e == hipSuccess; r += kWfRoundsPerCheck) {
+      uint32_t pending = 0;
+      if ((e = wf_pending(b->stream, b->wf.ws, &pending)) != hipSuccess || pending == 0) break;
+      if (r > max_rounds) return fail(nullptr, LISA_ERR_HIP, "wavefront form: paths did not finish");
+      e = b->exec2 ? hipGraphLaunch(b->exec2, b->stream) : enqueue_batch_rounds(b);
+    }
+  }
+  if (e != hipSuccess || (e = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(b->stream)) != hipSuccess)
     return fail(nullptr, LISA_ERR_HIP, std::string("lisa_batch_render: ") + hipGetErrorString(e));
   float ms = 0.0f;
@@ -2584,9 +3386,13 @@ lisa_status lisa_batch_autotune(lisa_batch* b, int32_t tune_spp, lisa_batch_conf
   lisa_batch_config best_cfg = b->cfg;
   std::vector<lisa_batch_config> cands;
   for (int32_t pc : per_cu) {
-    for (int32_t ch : chunks) cands.push_back(lisa_batch_config{b->cus * pc, ch, 0});
-    cands.push_back(lisa_batch_config{b->cus * pc, 1, 1});
+    for (int32_t ch : chunks) cands.push_back(lisa_batch_config{b->cus * pc, ch, 0, b->cfg.graph});
+    cands.push_back(lisa_batch_config{b->cus * pc, 1, 1, b->cfg.graph});
   }
+  // form 2: the traversal kernel's persistent grid and the paths in flight
+  // per traversal lane (queue depth)
+  for (int32_t pc : {16, 32})
+    for (int32_t ch : {1, 2, 4}) cands.push_back(lisa_batch_config{b->cus * pc, ch, 2, b->cfg.graph});
   for (const lisa_batch_config& c : cands) {
     float ms = 0.0f, ms2 = 0.0f;
     if ((rc = lisa_batch_set_config(t, &c)) != LISA_OK || (rc = lisa_batch_render(t, &ms)) != LISA_OK ||
@@ -2633,6 +3439,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   if (rc != LISA_OK) return rc;
   if (cp->train_paths > 0 && (cp->train_bounces == 0 || cp->train_bounces > 64))
     return fail(ctx, LISA_ERR_INVALID_ARG, "train_bounces must be in 1..64");
+  if (cp->train_steps > 64) return fail(ctx, LISA_ERR_INVALID_ARG, "train_steps must be at most 64");
   const size_t npx = (size_t)p->width * p->height;
   if (npx * p->spp >= (1ull << 31)) return fail(ctx, LISA_ERR_INVALID_ARG, "width * height * spp >= 2^31");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -2640,19 +3447,26 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   const auto t_start = std::chrono::steady_clock::now();
   lisa_cache_frame_stats out_stats;
   std::memset(&out_stats, 0, sizeof(out_stats));
-  if (ctx->accum_px != npx) {
-    if (p->subframe > 0) return fail(ctx, LISA_ERR_INVALID_ARG, "frame > 0 but the image size changed");
-    HIP_TRY(ctx, hipStreamSynchronize(st));
-    (void)hipFree(ctx->d_accum);
-    ctx->d_accum = nullptr;
-    ctx->accum_px = 0;
-    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_accum), npx * sizeof(float4)));
-    ctx->accum_px = npx;
-    ctx->accum_bytes = npx * sizeof(float4);
-  }
   if (cp->judge_pixels > npx || cp->judge_pixels > (1u << 20))
     return fail(ctx, LISA_ERR_INVALID_ARG, "judge_pixels must be at most the pixel count and 2^20");
+  if (p->subframe > 0) {
+    // frames after the first continue the sequence frame 0 started: same
+    // image size and judge pixels (the judge's buffers are sized at frame 0)
+    if (ctx->accum_px != npx) return fail(ctx, LISA_ERR_INVALID_ARG, "frame > 0 but the image size changed");
+    if (!ctx->j_seq)
+      return fail(ctx, LISA_ERR_INVALID_ARG,
+                  "frame > 0 but no cached frame 0 started this sequence (the accum buffer was "
+                  "written by another call since)");
+    if (cp->judge_pixels != ctx->j_nj)
+      return fail(ctx, LISA_ERR_INVALID_ARG, "judge_pixels changed within a cached sequence");
+  }
+  if (ctx->accum_px != npx) {
+    rc = ensure_accum(ctx, npx);
+    if (rc != LISA_OK) return rc;
+  }
   if (p->subframe == 0) {  // a new sequence: the judge starts over
+    ctx->j_seq = false;  // set again when this frame succeeds
+    ctx->j_nj = cp->judge_pixels;
     ctx->j_from = -1;
     ctx->j_n = 0;
     ctx->j_f = 0;
@@ -2691,9 +3505,10 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   qa.nee_loops = pre.nee_loops;
   qa.nee_loops_deep = pre.nee_loops_deep;
   const bool persistent = (cp->flags & LISA_CACHE_PERSISTENT) != 0;
+  const bool full = (cp->flags & LISA_CACHE_FULL_PATHS) != 0;
   bool launched = false;
   // 1. online training on suffixes of full-length paths traced by the plain loop
-  if (cp->train_paths > 0 && cp->train_steps > 0 && !taken_over) {
+  if (cp->train_paths > 0 && cp->train_steps > 0 && !taken_over && !full) {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t nt = cp->train_paths;
     const size_t cap = cp->train_max > 0 ? cp->train_max : nt * cp->train_bounces;
@@ -2757,7 +3572,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   } else {
     uint32_t nq = 0;
     if (!launched) {
-      rc = trace_queries_launch(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, persistent);
+      rc = trace_queries_launch(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, persistent, full);
       if (rc != LISA_OK) return rc;
     }
     rc = trace_queries_finish(ctx, qa, &nq);
@@ -2857,6 +3672,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
       ctx->j_from = (int64_t)p->subframe + 1;
   }
   out_stats.total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (p->subframe == 0) ctx->j_seq = true;
   if (stats) *stats = out_stats;
   return LISA_OK;
 }
@@ -2897,7 +3713,8 @@ lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
   a.nee_loops = pre.nee_loops;
   a.nee_loops_deep = pre.nee_loops_deep;
   uint32_t nq = 0;
-  rc = trace_queries(ctx, a, ctx->c_tpix.p, (size_t)n, cp->prefix_bounces, ctx->c_frame.p, &nq);
+  rc = trace_queries(ctx, a, ctx->c_tpix.p, (size_t)n, cp->prefix_bounces, ctx->c_frame.p, &nq, false,
+                     (cp->flags & LISA_CACHE_FULL_PATHS) != 0);
   if (rc != LISA_OK) return rc;
   HIP_TRY(ctx, hipMemcpy(accum_out, ctx->c_frame.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
   const size_t k = std::min<size_t>(nq, (size_t)q_cap);

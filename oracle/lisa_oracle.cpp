@@ -239,24 +239,34 @@ struct Scene {
   Bvh all;   // every triangle
   Bvh emit;  // the triangles whose material emits (light queries of NEE)
   EmitCull ecull;
+  EmitCull ebox;  // the same box, never switched off (the light sample's bound)
   // area prefix sums of the emitting triangles in emitter-tree order, and the
   // total (the light sample of the cache's prefix-emission estimator)
   std::vector<float> ecdf;
   float earea = 0.0f;
 };
 
-// The emitter area prefix sums, accumulated in tree order in FP32.
+// The emitter area prefix sums in tree order, in FP32, in the order the
+// device sums them: chunks of kCdfChunk triangles are summed sequentially
+// from 0 (p_k), the chunk totals sequentially into bases b_c, and
+// cdf[k] = b_c + p_k (for one chunk: the plain sequential sums).
+constexpr int32_t kCdfChunk = 64;
 void emit_cdf(Scene& s) {
-  s.ecdf.assign((size_t)s.emit.T, 0.0f);
-  float acc = 0.0f;
-  for (int32_t k = 0; k < s.emit.T; ++k) {
-    const float* t = &s.emit.tv[9 * (size_t)k];
-    const V3 v0 = ld3(t), v1 = ld3(t + 3), v2 = ld3(t + 6);
-    const V3 c = cross(v1 - v0, v2 - v0);
-    acc = acc + 0.5f * std::sqrt(dot(c, c));
-    s.ecdf[(size_t)k] = acc;
+  const int32_t T = s.emit.T;
+  s.ecdf.assign((size_t)T, 0.0f);
+  float base = 0.0f;
+  for (int32_t k0 = 0; k0 < T; k0 += kCdfChunk) {
+    float acc = 0.0f;
+    for (int32_t k = k0; k < std::min(k0 + kCdfChunk, T); ++k) {
+      const float* t = &s.emit.tv[9 * (size_t)k];
+      const V3 v0 = ld3(t), v1 = ld3(t + 3), v2 = ld3(t + 6);
+      const V3 c = cross(v1 - v0, v2 - v0);
+      acc = acc + 0.5f * std::sqrt(dot(c, c));
+      s.ecdf[(size_t)k] = base + acc;
+    }
+    base = base + acc;
   }
-  s.earea = acc;
+  s.earea = T > 0 ? s.ecdf[(size_t)T - 1] : 0.0f;
 }
 
 inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
@@ -445,14 +455,14 @@ inline bool watertight(const Ray& r, const float* tri, float& t) {
   const float By = comp(B, r.ky) - r.Sy * Bkz;
   const float Cx = comp(C, r.kx) - r.Sx * Ckz;
   const float Cy = comp(C, r.ky) - r.Sy * Ckz;
-  float U = Cx * By - Cy * Bx;
-  float V = Ax * Cy - Ay * Cx;
-  float W = Bx * Ay - By * Ax;
-  if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
-  }
+  // FP32 edge functions only (no double-precision recomputation of a zero
+  // edge function): the edge shared by two triangles gets exactly negated
+  // values in the two (the same per-vertex products, subtracted in the other
+  // order), so a ray on either side is accepted by one of them and a ray on
+  // the edge (a zero) by both -- none falls through.
+  const float U = Cx * By - Cy * Bx;
+  const float V = Ax * Cy - Ay * Cx;
+  const float W = Bx * Ay - By * Ax;
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
   const float det = U + V + W;
   if (det == 0.0f) return false;
@@ -669,6 +679,18 @@ bool emit_culled(const EmitCull& e, V3 O, V3 D) {
   return line_culled(e, O, D);
 }
 
+// The prefix light sample's estimate p^ = q |cos_y| A / r^2 stays below 1/2
+// at O: q <= 3 sqrt(3) / 12 < 0.4331, |cos_y| <= 1 and r^2 >= d^2 (d: O's
+// distance to the emitters' box), so 0.9 A <= d^2 gives p^ < 0.49.  Depends
+// on the vertex only, so choosing the estimator by it changes no expectation.
+bool light_sample_bounded(const EmitCull& e, V3 O, float A) {
+  const float dx = std::fmax(std::fabs(O.x - e.c[0]) - e.H[0], 0.0f);
+  const float dy = std::fmax(std::fabs(O.y - e.c[1]) - e.H[1], 0.0f);
+  const float dz = std::fmax(std::fabs(O.z - e.c[2]) - e.H[2], 0.0f);
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  return 0.9f * A <= d2;
+}
+
 // The occlusion query of NEE.  Returns the material of the closest hit when
 // it emits, -1 on a miss, kNonEmitter when the closest hit does not emit --
 // exactly the outcomes shader.cu:172-184 distinguish, computed as:
@@ -857,7 +879,8 @@ struct Tracer {
   // nothing lies before it.
   void emit_sample(RayState& st) {
     const float u0 = rnd(seed), u1 = rnd(seed), u2 = rnd(seed);
-    const float ta = u0 * s.earea;
+    // below the last prefix sum: the search ends on a triangle of positive area
+    const float ta = std::fmin(u0 * s.earea, std::nextafter(s.earea, 0.0f));
     int32_t lo = 0, hi = s.emit.T - 1;  // first prefix sum above ta
     while (lo < hi) {
       const int32_t mid = (lo + hi) >> 1;
@@ -982,7 +1005,8 @@ struct Tracer {
         const V3 nee = shoot_ray_to_light(st);
         st.color = st.color + nee * st.att;
       }
-      if (emit_expect && m.roughness == 1.0f && s.emit.T > 0 && s.earea > 0.0f && rec.bounces < max_b) {
+      if (emit_expect && m.roughness == 1.0f && s.emit.T > 0 && s.earea > 0.0f && rec.bounces < max_b &&
+          light_sample_bounded(s.ebox, st.origin, s.earea)) {
         emit_sample(st);
         cond = true;
         rejects = 0;
@@ -1159,6 +1183,7 @@ orc_scene* orc_scene_create(const float* verts, const float* normals, const int3
   build_bvh(s.all, s.v, gid_all);
   build_bvh(s.emit, s.v, gid_emit);
   s.ecull = emit_cull(s);
+  s.ebox = s.ecull;
   emit_cdf(s);
   return o;
 }
@@ -1171,6 +1196,7 @@ void orc_scene_update_vertices(orc_scene* o, const float* verts, const float* no
   update_bvh(s.all, s.v);
   update_bvh(s.emit, s.v);
   s.ecull = emit_cull(s);
+  s.ebox = s.ecull;
   emit_cdf(s);
 }
 int32_t orc_num_tris(const orc_scene* s) { return s->s.T; }
@@ -1251,6 +1277,49 @@ void orc_cache_bounds(const orc_scene* o, float* bmin, float* binv) {
   const CacheBox c = cache_box(o->s);
   st3(bmin, c.bmin);
   st3(binv, c.binv);
+}
+
+int orc_render_full_paths(const orc_scene* o, const orc_render_params* p, const int32_t* pixel_ids,
+                          int32_t n, float* accum_out, int threads, int emit_expect, int nee_loops,
+                          int nee_loops_deep) {
+  const Scene& s = o->s;
+  if (p->width == 0 || p->height == 0 || p->spp == 0 || n <= 0) return -1;
+  if (nee_loops > 16 || nee_loops_deep > 16 || ((nee_loops > 1 || nee_loops_deep > 1) && !emit_expect))
+    return -3;
+  const int64_t npx = (int64_t)p->width * p->height;
+  for (int32_t i = 0; i < n; ++i)
+    if (pixel_ids[i] < 0 || pixel_ids[i] >= npx) return -2;
+  const CacheBox box = cache_box(s);
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+  {
+    Counters cnt;
+#pragma omp for schedule(dynamic, 4)
+    for (int32_t i = 0; i < n; ++i) {
+      const uint32_t pix = (uint32_t)pixel_ids[i];
+      const uint32_t x = pix % p->width, y = pix / p->width;
+      uint32_t seed = tea16(y * p->width + x, p->subframe);
+      const V3 eye = ld3(p->eye), U = ld3(p->U), V = ld3(p->V), W = ld3(p->W);
+      V3 accum = mk(0.0f, 0.0f, 0.0f);
+      for (uint32_t k = 0; k < p->spp; ++k) {
+        const float jx = rng(seed);
+        const float jy = rng(seed);
+        const float dx = (2.0f * (float)x + jx) / (float)p->width - 1.0f;
+        const float dy = (2.0f * (float)y + jy) / (float)p->height - 1.0f;
+        const V3 D = normalize(dx * U + dy * V + W);
+        SampleRec rec;
+        Tracer tr{s, cnt, seed, rec};
+        tr.cache_at = 0;  // no query: the whole path
+        tr.box = &box;
+        tr.emit_expect = emit_expect != 0;
+        tr.nee_loops = nee_loops > 1 ? nee_loops : 1;
+        tr.nee_loops_deep = nee_loops_deep > 1 ? nee_loops_deep : 1;
+        accum = accum + tr.trace(eye, D, (int)p->bounces);
+      }
+      st3(accum_out + 4 * (size_t)i, accum * (1.0f / (float)p->spp));
+      accum_out[4 * (size_t)i + 3] = 1.0f;
+    }
+  }
+  return 0;
 }
 
 int orc_cache_queries(const orc_scene* o, const orc_render_params* p, uint32_t prefix_bounces,

@@ -143,6 +143,13 @@ int orc_cache_queries(const orc_scene* s, const orc_render_params* p, uint32_t p
                       const int32_t* pixel_ids, int32_t n, float* accum_out, float* q_feat,
                       float* q_att, int32_t* q_item, int32_t q_cap, int32_t* n_queries, int threads,
                       int emit_expect, int nee_loops, int nee_loops_deep);
+/* The variance-reduced plain tracer (the estimators of orc_cache_queries'
+ * emit_expect / nee_loops on whole paths of p->bounces radiance rays, no
+ * query): per-pixel means into accum_out (4n).  Without emit_expect these are
+ * the plain tracer's samples. */
+int orc_render_full_paths(const orc_scene* s, const orc_render_params* p, const int32_t* pixel_ids,
+                          int32_t n, float* accum_out, int threads, int emit_expect, int nee_loops,
+                          int nee_loops_deep);
 /* Training rows of plain paths (p->bounces) on the listed pixels: one row per
  * surface hit (depth 0) or per surface hit of radiance ray number `depth`,
  * 12 features and the target (final - gathered before the hit) / throughput

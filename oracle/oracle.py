@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liblisa_oracle.so")
+# LISA_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build of `make sanitize`)
+LIB_PATH = os.environ.get("LISA_ORACLE_LIB") or os.path.join(HERE, "_build", "liblisa_oracle.so")
 NCOUNTERS = 8
 COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
                  "radiance_hits", "samples", "invalid_dir_rays", "pixels")
@@ -100,6 +102,9 @@ def lib():
         L.orc_cache_queries.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_uint32, i32p,
                                         C.c_int32, f32p, f32p, f32p, i32p, C.c_int32, i32p, C.c_int,
                                         C.c_int, C.c_int, C.c_int]
+        L.orc_render_full_paths.restype = C.c_int
+        L.orc_render_full_paths.argtypes = [C.c_void_p, C.POINTER(RenderParams), i32p, C.c_int32,
+                                            f32p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_cache_train_rows.restype = C.c_int
         L.orc_cache_train_rows.argtypes = [C.c_void_p, C.POINTER(RenderParams), i32p, C.c_int32,
                                            C.c_uint32, f32p, f32p, i32p, C.c_int32, i32p, C.c_int]
@@ -372,6 +377,22 @@ class Scene:
             raise RuntimeError(f"orc_cache_queries failed: {rc}")
         k = nq.value
         return {"accum": acc, "feat": f[:k], "att": at[:k], "item": it[:k], "n": k}
+
+    def render_full_paths(self, width, height, spp, bounces, eye, U, V, W, pixels, subframe=0,
+                          threads=1, emit_expect=True, nee_loops=1, nee_loops_deep=None):
+        """The variance-reduced plain tracer (orc_render_full_paths): whole paths with
+        the cache prefix's estimators and no query; per-pixel means (n, 4)."""
+        if nee_loops_deep is None:
+            nee_loops_deep = nee_loops
+        p = self._params(width, height, spp, bounces, eye, U, V, W, subframe)
+        px = np.ascontiguousarray(pixels, dtype=np.int32)
+        acc = np.zeros((px.shape[0], 4), np.float32)
+        rc = lib().orc_render_full_paths(self.h, C.byref(p), _p(px, C.c_int32), px.shape[0],
+                                         _p(acc, C.c_float), threads, 1 if emit_expect else 0,
+                                         nee_loops, nee_loops_deep)
+        if rc != 0:
+            raise RuntimeError(f"orc_render_full_paths failed: {rc}")
+        return acc
 
     def cache_train_rows(self, width, height, spp, bounces, eye, U, V, W, pixels, subframe=0,
                          depth=0, threads=1):

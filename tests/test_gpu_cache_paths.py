@@ -314,3 +314,74 @@ def test_plain_tracer_judges_the_cache(capi, oracle_mod, ctx):
                 assert took is None and e > 0.05
         finally:
             cache.close()
+
+
+def test_cached_sequence_must_continue_frame0(capi, oracle_mod, ctx):
+    """Frames after the first continue the sequence a cached frame 0 started:
+    the judge's buffers are sized at frame 0, so a frame > 0 with another
+    judge_pixels, after a plain render wrote the accum buffer, or with no
+    cached frame 0 at all is refused before anything runs on the device."""
+    W = H = 32
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    cache = capi.Cache(0, seed=1)
+    try:
+        cp = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0,
+                                     judge_pixels=64, judge_rmse=1e9)
+        p = lambda f: capi.render_params(W, H, 2, 7, eye, U, V, Wv, subframe=f)  # noqa: E731
+        fresh = capi.Context(0)
+        try:
+            fresh.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+            with pytest.raises(capi.LisaError, match="frame > 0"):
+                capi.render_cached(fresh, cache, p(1), cp)
+        finally:
+            fresh.close()
+        capi.render_cached(ctx, cache, p(0), cp)
+        capi.render_cached(ctx, cache, p(1), cp)
+        more = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0,
+                                       judge_pixels=128, judge_rmse=1e9)
+        with pytest.raises(capi.LisaError, match="judge_pixels changed"):
+            capi.render_cached(ctx, cache, p(2), more)
+        none = capi.cache_frame_params(prefix_bounces=2, train_paths=0, train_steps=0)
+        with pytest.raises(capi.LisaError, match="judge_pixels changed"):
+            capi.render_cached(ctx, cache, p(2), none)
+        capi.render_cached(ctx, cache, p(2), cp)  # the sequence itself goes on
+        ctx.render(p(0))  # a plain render takes the accum buffer
+        with pytest.raises(capi.LisaError, match="no cached frame 0"):
+            capi.render_cached(ctx, cache, p(1), cp)
+        capi.render_cached(ctx, cache, p(0), none)  # a new sequence without the judge
+        capi.render_cached(ctx, cache, p(1), none)
+    finally:
+        cache.close()
+
+
+def test_full_paths_match_oracle(capi, oracle_mod, ctx):
+    """LISA_CACHE_FULL_PATHS (the variance-reduced plain tracer, no network):
+    without the estimators the per-pixel means are the plain tracer's bit for
+    bit; with them (light samples, 8 / 1 NEE runs) they equal the oracle's
+    orc_render_full_paths bit for bit, on the Cornell box and a small island."""
+    from lisa_amd import scenes
+    W = H = 64
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    px = np.sort(np.random.default_rng(5).choice(W * H, 1024, replace=False)).astype(np.int32)
+    p = capi.render_params(W, H, 8, 7, eye, U, V, Wv, subframe=2)
+    cp = capi.cache_frame_params(prefix_bounces=2, flags=capi.CACHE_FULL_PATHS)
+    g = capi.cache_trace_queries(ctx, p, cp, px, cap=0)
+    plain = ctx.trace_subset(p, px, records=False)["accum"]
+    assert g["n"] == 0
+    assert np.array_equal(g["accum"].view(np.uint32), plain.view(np.uint32))
+    isl = scenes.island(n=128, width=96, height=64)
+    for sc_, W_, H_ in ((sc, W, H), (isl, 96, 64)):
+        ctx.set_scene(sc_["verts"], sc_["normals"], sc_["mat_idx"], sc_["materials"])
+        o_ = oracle_mod.Scene(sc_["verts"], sc_["normals"], sc_["mat_idx"], sc_["materials"])
+        cam = sc_["camera"]
+        U_, V_, W2 = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W_ / H_)
+        px_ = np.sort(np.random.default_rng(6).choice(W_ * H_, 1024, replace=False)).astype(np.int32)
+        p_ = capi.render_params(W_, H_, 8, 12, cam["eye"], U_, V_, W2, subframe=5)
+        cp_ = capi.cache_frame_params(prefix_bounces=2, flags=capi.CACHE_FULL_PATHS | capi.CACHE_EMISSION,
+                                      nee_loops=8, nee_loops_deep=1)
+        g = capi.cache_trace_queries(ctx, p_, cp_, px_, cap=0)
+        r = o_.render_full_paths(W_, H_, 8, 12, cam["eye"], U_, V_, W2, px_, subframe=5, threads=8,
+                                 emit_expect=True, nee_loops=8, nee_loops_deep=1)
+        assert np.array_equal(g["accum"].view(np.uint32), r.view(np.uint32))

@@ -186,6 +186,13 @@ def test_hit_ids_edges_and_axis_rays(capi, oracle_mod, ctx):
         assert np.array_equal(gp, rp) and np.array_equal(gt.view(np.uint32), rt.view(np.uint32))
 
 
+def _nan_tris(v):
+    v = np.array(v, np.float32).reshape(-1, 3, 3)
+    for i in range(v.shape[0]):
+        v[i, : 1 + i % 3] = np.nan
+    return v.reshape(-1, 9)
+
+
 def test_edge_case_scenes(capi, oracle_mod, ctx):
     sc = cornell(32, 16, 32, 32, 4, 5)
     U, V, W = camera(sc, capi)
@@ -200,6 +207,10 @@ def test_edge_case_scenes(capi, oracle_mod, ctx):
         "degenerate": (np.concatenate([sc["verts"], np.repeat(sc["verts"][:5, :3], 3, 1)]),
                        np.concatenate([sc["normals"], sc["normals"][:5]]),
                        np.concatenate([sc["mat_idx"], sc["mat_idx"][:5]])),
+        # triangles with NaN vertices (one, two or all three) mixed into the box
+        "nan": (np.concatenate([sc["verts"], _nan_tris(sc["verts"][:6])]),
+                np.concatenate([sc["normals"], sc["normals"][:6]]),
+                np.concatenate([sc["mat_idx"], sc["mat_idx"][:6]])),
     }
     for name, (v, n, m) in cases.items():
         ctx.set_scene(v, n, m, mats)
@@ -334,7 +345,11 @@ def test_batched_wavefront_bit_exact(capi, oracle_mod):
         ctxs.append(c)
         params.append(p)
     batch = capi.Batch(ctxs, params)
-    for cfg in [None, (64, 1), (1024, 4), (300, 16), (64, 1, 1), (1024, 1, 1), (7, 1, 1)]:
+    for cfg in [None, (64, 1), (1024, 4), (300, 16), (64, 1, 1), (1024, 1, 1), (7, 1, 1),
+                (256, 4, 0, 0), (256, 1, 1, 0),
+                # the ray-level wavefront form: slots for every pixel, few slots
+                # (pixels refill freed slots), plain launches
+                (4096, 2, 2), (16, 1, 2), (300, 3, 2, 0)]:
         if cfg:
             batch.set_config(*cfg)
         for rep in range(2):  # graph replays
@@ -343,10 +358,27 @@ def test_batched_wavefront_bit_exact(capi, oracle_mod):
                 got = batch.accum(i)
                 assert np.array_equal(got.view(np.uint32), refs[i].view(np.uint32)), (cfg, rep, i)
     best = batch.autotune(tune_spp=1)
-    assert best[0] > 0 and best[1] > 0 and best[2] in (0, 1)
+    assert best[0] > 0 and best[1] > 0 and best[2] in (0, 1, 2)
     batch.render()
     for i in range(3):
         assert np.array_equal(batch.accum(i).view(np.uint32), refs[i].view(np.uint32))
+    # a context whose scene, vertices or accum buffer changed makes the
+    # captured launch stale: the batch refuses to run until re-created
+    sc = cornell(*specs[1][:2], *specs[1][2:])
+    for change in ("vertices", "scene", "accum"):
+        b2 = capi.Batch(ctxs, params)
+        b2.render()
+        if change == "vertices":
+            ctxs[1].update_vertices(sc["verts"], sc["normals"])
+        elif change == "scene":
+            ctxs[1].set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        else:
+            p = params[1]
+            ctxs[1].render(capi.render_params(8, 8, 1, 1, sc["camera"]["eye"], p.U, p.V, p.W))
+            ctxs[1].render(p)
+        with pytest.raises(capi.LisaError, match="re-create the batch"):
+            b2.render()
+        b2.close()
     batch.close()
     for c in ctxs:
         c.close()
@@ -426,6 +458,47 @@ def test_persistent_launch_same_image(capi, oracle_mod, ctx):
             got, _ = ctx.render(p)
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f
         p = capi.render_params(100, 60, 4, 7, eye, U, V, W, flags=capi.FLAG_PERSISTENT | capi.FLAG_COUNT)
+        with pytest.raises(capi.LisaError):
+            ctx.render(p)
+    finally:
+        plain.close()
+
+
+def test_wavefront_form_same_image(capi, oracle_mod, ctx):
+    """LISA_FLAG_WAVEFRONT (path states in HBM, rounds of the traversal
+    kernel with per-wave request queues and the shading kernel with ballot /
+    prefix-sum compaction) renders every image bit for bit as the
+    megakernel: partial edge tiles, progressive subframes, an open scene with
+    dielectric water, a displaced glass icosphere at 12 bounces, and the
+    edge-case scenes (empty, one triangle, NaN vertices)."""
+    isl = scenes.island(n=128, width=96, height=64)
+    ico = scenes.displaced_icosphere(level=5, width=72, height=40, spp=3, bounces=12)
+    base = cornell(32, 16, 100, 60, 4, 7)
+    cases = [("cornell", base, 100, 60, 4, 7), ("island", isl, 96, 64, 4, 12),
+             ("icosphere", ico, 72, 40, 3, 12)]
+    mats = base["materials"]
+    cases.append(("empty", dict(base, verts=np.zeros((0, 9), np.float32),
+                                normals=np.zeros((0, 9), np.float32),
+                                mat_idx=np.zeros(0, np.int32)), 40, 24, 2, 5))
+    cases.append(("nan", dict(base, verts=np.concatenate([base["verts"], _nan_tris(base["verts"][:6])]),
+                              normals=np.concatenate([base["normals"], base["normals"][:6]]),
+                              mat_idx=np.concatenate([base["mat_idx"], base["mat_idx"][:6]])),
+                  40, 24, 2, 5))
+    plain = capi.Context(0)
+    try:
+        for name, sc, W, H, spp, B in cases:
+            for c in (ctx, plain):
+                c.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc.get("materials", mats))
+            cam = sc["camera"]
+            U, V, Wv = capi.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+            for sub in range(2):
+                p = capi.render_params(W, H, spp, B, cam["eye"], U, V, Wv, subframe=sub)
+                want, _ = plain.render(p)
+                p.flags = capi.FLAG_WAVEFRONT
+                got, _ = ctx.render(p)
+                print(name, sub, flush=True)
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (name, sub)
+        p = capi.render_params(8, 8, 1, 1, (0, 0, 1), U, V, Wv, flags=capi.FLAG_WAVEFRONT | capi.FLAG_COUNT)
         with pytest.raises(capi.LisaError):
             ctx.render(p)
     finally:

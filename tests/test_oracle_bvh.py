@@ -141,6 +141,30 @@ def test_traversal_equals_brute_force(oracle_mod, seg, rings, inside):
     assert cnt[2] > 0 and cnt[3] > 0
 
 
+def test_nan_vertices_are_never_hit(oracle_mod):
+    """Triangles with one, two or three NaN vertices in the Cornell box: the
+    LBVH over them still answers every ray as the brute-force loop does (NaN
+    boxes drop out of the unions, a NaN edge function or t never accepts),
+    no NaN triangle is ever the closest hit, and the other triangles' hits
+    are those of the scene without them."""
+    sc = cornell(32, 16, 8, 8, 1, 1)
+    v = np.array(sc["verts"][:30], np.float32).reshape(-1, 3, 3)
+    for i in range(v.shape[0]):
+        v[i, : 1 + i % 3] = np.nan
+    T0 = sc["verts"].shape[0]
+    verts = np.concatenate([sc["verts"], v.reshape(-1, 9)])
+    o = oracle_mod.Scene(verts, np.concatenate([sc["normals"], sc["normals"][:30]]),
+                         np.concatenate([sc["mat_idx"], sc["mat_idx"][:30]]), sc["materials"])
+    ref = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    O, D = random_rays(4000, seed=9)
+    p1, t1, _ = o.closest_hit(O, D, threads=4)
+    p2, t2, _ = o.closest_hit(O, D, brute=True)
+    p3, t3, _ = ref.closest_hit(O, D, brute=True)
+    assert np.array_equal(p1, p2) and np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
+    assert np.all(p1 < T0) and (p1 >= 0).mean() > 0.8
+    assert np.array_equal(p1, p3) and np.array_equal(t1.view(np.uint32), t3.view(np.uint32))
+
+
 def test_zero_and_nan_directions_miss(oracle_mod):
     sc = cornell(32, 16, 8, 8, 1, 1)
     o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])

@@ -124,3 +124,60 @@ def test_no_emitter_is_the_plain_prefix(oracle_mod):
     assert a["n"] == b["n"] > 0
     for k in ("accum", "feat", "att", "item"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_prefix_light_sample_is_bounded_in_the_cornell_box(oracle_mod):
+    """In the closed Cornell box many vertices lie close to the ceiling light,
+    where the light sample's hit estimate p^ = q |cos| A / r^2 could exceed 1
+    (and the throughput factor 1 - p^ turn negative).  The estimator is used
+    only where a bound on p^ from the vertex's distance to the emitters' box
+    keeps it below 1/2 (elsewhere the reference's bounce), so every query
+    throughput and pixel mean stays non-negative, and the estimator's
+    per-pixel variance is no larger than the plain prefix's.  Without the
+    bound this scene gives negative throughputs."""
+    W = H = 48
+    sc = cornell(32, 16, W, H, 4, 7)
+    o = _scene(oracle_mod, sc)
+    var = {}
+    for emit in (False, True):
+        x, _, r = _pixel_sums(o, oracle_mod, sc, W, H, 64, 2, 5, emit)
+        y = _pixel_sums(o, oracle_mod, sc, W, H, 64, 2, 6, emit)[0]
+        assert r["n"] > 0
+        assert np.all(r["att"] >= 0) and np.all(r["accum"] >= 0), emit
+        var[emit] = float(np.var(x - y) / 2)
+    print("Cornell prefix variance (plain, estimator):", var, flush=True)
+    assert var[True] <= 1.05 * var[False]
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_full_paths_variance_reduced_plain(oracle_mod, case):
+    """The variance-reduced plain tracer (orc_render_full_paths: the prefix
+    estimators on whole 7-bounce paths, no network): without the estimators
+    it is the plain tracer bit for bit; with them (light samples, 8 NEE runs
+    at the camera ray's hit, 1 later) its pixel means keep the plain ones'
+    expectation (paired z-test) at less variance on the island."""
+    make, W, H, spp, _ = CASES[case]
+    sc = make()
+    o = _scene(oracle_mod, sc)
+    cam = sc["camera"]
+    U, V, Wv = oracle_mod.camera_uvw(cam["eye"], cam["look_at"], cam["fov"], W / H)
+    px = np.arange(W * H, dtype=np.int32)
+    spp = 64
+    plain = o.render(W, H, spp, 7, cam["eye"], U, V, Wv, subframe=3, pixels=px, threads=8)["accum"]
+    same = o.render_full_paths(W, H, spp, 7, cam["eye"], U, V, Wv, px, subframe=3, threads=8,
+                               emit_expect=False)
+    assert np.array_equal(same.view(np.uint32), plain.view(np.uint32))
+    vr = o.render_full_paths(W, H, spp, 7, cam["eye"], U, V, Wv, px, subframe=3, threads=8,
+                             emit_expect=True, nee_loops=8, nee_loops_deep=1)
+    vr2 = o.render_full_paths(W, H, spp, 7, cam["eye"], U, V, Wv, px, subframe=4, threads=8,
+                              emit_expect=True, nee_loops=8, nee_loops_deep=1)
+    plain2 = o.render(W, H, spp, 7, cam["eye"], U, V, Wv, subframe=4, pixels=px, threads=8)["accum"]
+    d = vr[:, :3].astype(np.float64).sum(1) - plain[:, :3].astype(np.float64).sum(1)
+    z = d.mean() / (d.std() / np.sqrt(d.size))
+    v_vr = float(np.var(vr[:, :3].astype(np.float64) - vr2[:, :3]) / 2)
+    v_pl = float(np.var(plain[:, :3].astype(np.float64) - plain2[:, :3]) / 2)
+    print(f"{case}: z {z:+.2f}, per-pixel variance plain {v_pl:.3g} vr {v_vr:.3g}", flush=True)
+    assert abs(z) < Z_BOUND
+    assert np.all(vr >= 0)
+    if case == "island":
+        assert v_vr < 0.7 * v_pl
