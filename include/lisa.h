@@ -111,6 +111,17 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
  * reference, whose GAS is built once (optix_wrapper.cc:104-106). */
 lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float* normals);
 
+/* lisa_update_vertices for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight animation / serving loops): the same refit,
+ * bit-identical trees, with the new positions (9*T floats) and optional
+ * normals already in DEVICE memory (caller-owned), and no hipMalloc / hipFree
+ * of its own -- the refit's union tables live in a device array of the library
+ * (64 MB per device, held until the process ends).  At most 1048576
+ * triangles (LISA_ERR_INVALID_ARG beyond, before anything runs).  Every call
+ * on one device shares that array: calls that overlap on one GPU (from
+ * several threads or contexts) race.  No counterpart in the reference. */
+lisa_status lisa_update_vertices_noalloc(lisa_ctx* ctx, const float* d_verts, const float* d_normals);
+
 /* Replaces render() -> launchSubframe() -> optixLaunch(w, h, 1)
  * (src/LiSA/src/render.cc:35-58, 133-148).  The float4 accum buffer stays on
  * the device (as OptixParams::accum_buffer); accum_rgba_out (w*h*4 floats)

@@ -37,6 +37,7 @@ COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
 # Every function declared in include/lisa.h.
 EXPORTS = (
     "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_update_vertices",
+    "lisa_update_vertices_noalloc",
     "lisa_render",
     "lisa_copy_accum", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
     "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset",
@@ -108,6 +109,7 @@ def lib():
         "lisa_last_error": (C.c_char_p, [vp]),
         "lisa_set_scene": (st, [vp, f32p, f32p, i32p, C.c_int32, C.POINTER(Material), C.c_int32]),
         "lisa_update_vertices": (st, [vp, f32p, f32p]),
+        "lisa_update_vertices_noalloc": (st, [vp, vp, vp]),
         "lisa_render": (st, [vp, C.POINTER(RenderParams), f32p, u8p]),
         "lisa_copy_accum": (st, [vp, f32p, C.c_int64]),
         "lisa_synchronize": (st, [vp]),
@@ -294,6 +296,11 @@ class Context:
             raise ValueError("update_vertices: triangle count differs from the scene")
         n = None if normals is None else f32(normals).reshape(-1, 9)
         self._ck(lib().lisa_update_vertices(self.h, _p(v, C.c_float), _p(n, C.c_float)))
+
+    def update_vertices_noalloc(self, d_verts, d_normals=None):
+        """lisa_update_vertices_noalloc: new positions (and normals) already in device
+        memory, given as integer device addresses; no allocation by the library."""
+        self._ck(lib().lisa_update_vertices_noalloc(self.h, d_verts, d_normals))
 
     def render(self, params, want_accum=True, want_rgba8=False):
         npx = params.width * params.height

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -2162,6 +2163,18 @@ struct DevBuf {
   }
 };
 
+// Device memory of lisa_update_vertices_noalloc (the refit for callers that
+// cannot allocate): the two union tables of the level refit, 2 float4 per
+// triangle each, for scenes of at most kNoAllocMaxTris triangles (64 MB per
+// device, held until the process ends).  Host code never uses the array's
+// name: the address on each device comes from hipGetSymbolAddress, taken
+// once per device in lisa_create with that device current.
+constexpr int32_t kNoAllocMaxTris = 1 << 20;
+static __device__ __align__(256) float4 scratch[4 * (size_t)kNoAllocMaxTris];
+constexpr int kMaxDevices = 64;
+static float4* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
 // Device buffers of the wavefront form: the state of `slots` paths in
 // flight (WfState), the two request lists, the counters, and (for a single
 // render) the one-entry scene table.  Grown on demand, one allocation.
@@ -2838,6 +2851,44 @@ lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float*
   return read_scene_box(ctx);
 }
 
+lisa_status lisa_update_vertices_noalloc(lisa_ctx* ctx, const float* d_verts, const float* d_normals) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!ctx->has_scene) return fail(ctx, LISA_ERR_NO_SCENE, "lisa_set_scene has not been called");
+  const int32_t T = ctx->T;
+  if (T == 0) return LISA_OK;
+  if (!d_verts) return fail(ctx, LISA_ERR_INVALID_ARG, "d_verts is NULL");
+  if (T > kNoAllocMaxTris)
+    return fail(ctx, LISA_ERR_INVALID_ARG,
+                "lisa_update_vertices_noalloc takes scenes of at most 1048576 triangles (this one has " +
+                    std::to_string(T) + "); use lisa_update_vertices");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  float4* const ptr = g_scratch_ptr[ctx->device];  // set by lisa_create on this device
+  if (!ptr) return fail(ctx, LISA_ERR_HIP, "no refit tables on this device");
+  float4* const ta = ptr;
+  float4* const tb = ptr + 2 * (size_t)kNoAllocMaxTris;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  DevBvh* trees[2] = {&ctx->all, &ctx->emit};
+  for (DevBvh* b : trees) {
+    if (b->T == 0) continue;
+    const bool all = b == &ctx->all;
+    k_regather<<<grid_for(b->T, kBuildBlock), kBuildBlock, 0, st>>>(d_verts, all ? d_normals : nullptr, b->prim,
+                                                                    b->T, b->tris, all ? ctx->d_nrm : nullptr);
+    if (b->T == 1) {
+      k_single_root<<<1, 1, 0, st>>>(b->tris, b->nodes);
+    } else {
+      lisa_status rc = refit_levels_on(ctx, *b, ta, tb);
+      if (rc != LISA_OK) return rc;
+    }
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  HIP_TRY(ctx, hipEventElapsedTime(&ctx->bvh_ms, ctx->ev0, ctx->ev1));
+  ctx->gen++;  // the cull box a batch captured may have changed
+  return read_scene_box(ctx, false);
+}
+
 lisa_status lisa_create(lisa_ctx** out, int32_t device) {
   if (!out) return fail(nullptr, LISA_ERR_INVALID_ARG, "out is NULL");
   *out = nullptr;
@@ -2857,6 +2908,23 @@ lisa_status lisa_create(lisa_ctx** out, int32_t device) {
     const std::string msg = std::string("lisa_create: ") + hipGetErrorString(e);
     lisa_destroy(c);
     return fail(nullptr, LISA_ERR_HIP, msg);
+  }
+  // this device's copy of the allocation-free refit's tables (once per device)
+  if (device >= kMaxDevices) {
+    lisa_destroy(c);
+    return fail(nullptr, LISA_ERR_UNSUPPORTED, "device index >= 64");
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void* ptr = nullptr;
+      if ((e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch))) != hipSuccess || !ptr) {
+        const std::string msg = std::string("lisa_create: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        lisa_destroy(c);
+        return fail(nullptr, LISA_ERR_HIP, msg);
+      }
+      g_scratch_ptr[device] = static_cast<float4*>(ptr);
+    }
   }
   *out = c;
   return LISA_OK;
