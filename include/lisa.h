@@ -58,7 +58,11 @@ typedef struct lisa_render_params {
 } lisa_render_params;
 
 #define LISA_MODE_PLAIN 0        /* one work-item per pixel megakernel (the drop-in) */
-#define LISA_FLAG_FP64_ACCUM 1u  /* per-pixel FP64 accumulation (reference renders >= 16k spp) */
+#define LISA_FLAG_FP64_ACCUM 1u  /* per-pixel FP64 accumulation (reference renders >= 16k spp): the
+                                    image's FP64 sums and sample counts are kept across subframes, so
+                                    subframe k's accum is the exact mean of every sample of subframes
+                                    0..k (instead of shader.cu:160-164's FP32 running lerp);
+                                    resumable with lisa_accum64_save / lisa_accum64_load */
 #define LISA_FLAG_COUNT 2u       /* also count rays / node visits / triangle tests */
 #define LISA_FLAG_PERSISTENT 4u  /* one persistent launch whose lanes pull pixels from a queue
                                     (same image; not with COUNT or FP64_ACCUM) */
@@ -134,6 +138,16 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p,
  * last render) to rgba_out.  No counterpart in the reference, whose
  * frame_buffer is mapped host memory (CUDAOutputBuffer ZERO_COPY). */
 lisa_status lisa_copy_accum(lisa_ctx* ctx, float* rgba_out, int64_t n_pixels);
+
+/* Checkpoint / resume of an FP64 progressive render (LISA_FLAG_FP64_ACCUM):
+ * save copies the image's FP64 state, 4 doubles per pixel (r, g, b sums,
+ * samples so far), after an FP64 render; load puts such a state into a
+ * context (the accum buffer becomes its mean) so that the next FP64 render
+ * with subframe k + 1 continues the sequence exactly where subframe k left
+ * it -- bit for bit as if it had not been interrupted.  No counterpart in the
+ * reference (its display loop restarts from subframe 0). */
+lisa_status lisa_accum64_save(lisa_ctx* ctx, double* sums_out, int64_t n_pixels);
+lisa_status lisa_accum64_load(lisa_ctx* ctx, const double* sums, int64_t n_pixels);
 
 lisa_status lisa_synchronize(lisa_ctx* ctx);
 lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out);

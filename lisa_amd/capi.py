@@ -39,7 +39,7 @@ EXPORTS = (
     "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_update_vertices",
     "lisa_update_vertices_noalloc",
     "lisa_render",
-    "lisa_copy_accum", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
+    "lisa_copy_accum", "lisa_accum64_save", "lisa_accum64_load", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
     "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset",
     "lisa_batch_create", "lisa_batch_destroy", "lisa_batch_set_config", "lisa_batch_get_config",
     "lisa_batch_render", "lisa_batch_autotune",
@@ -112,6 +112,8 @@ def lib():
         "lisa_update_vertices_noalloc": (st, [vp, vp, vp]),
         "lisa_render": (st, [vp, C.POINTER(RenderParams), f32p, u8p]),
         "lisa_copy_accum": (st, [vp, f32p, C.c_int64]),
+        "lisa_accum64_save": (st, [vp, C.POINTER(C.c_double), C.c_int64]),
+        "lisa_accum64_load": (st, [vp, C.POINTER(C.c_double), C.c_int64]),
         "lisa_synchronize": (st, [vp]),
         "lisa_get_stats": (st, [vp, C.POINTER(Stats)]),
         "lisa_get_counters": (st, [vp, u64p]),
@@ -314,6 +316,16 @@ class Context:
         acc = np.zeros((n_pixels, 4), np.float32)
         self._ck(lib().lisa_copy_accum(self.h, _p(acc, C.c_float), n_pixels))
         return acc
+
+    def accum64_save(self, n_pixels):
+        """The FP64 progressive state (n_pixels x 4: r, g, b sums, samples)."""
+        out = np.zeros((n_pixels, 4), np.float64)
+        self._ck(lib().lisa_accum64_save(self.h, _p(out, C.c_double), n_pixels))
+        return out
+
+    def accum64_load(self, sums):
+        a = np.ascontiguousarray(sums, dtype=np.float64).reshape(-1, 4)
+        self._ck(lib().lisa_accum64_load(self.h, _p(a, C.c_double), a.shape[0]))
 
     def synchronize(self):
         self._ck(lib().lisa_synchronize(self.h))

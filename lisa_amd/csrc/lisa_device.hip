@@ -269,7 +269,12 @@ template <bool kCount, int kStride>
 __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, float tmin,
                                            float tlim, uint32_t lim_prim, bool any, float& t_out,
                                            uint32_t* stk, TravCount& cnt) {
-  uint32_t spill[kStackMax - kStackLds];
+  // Entries past the LDS part of the stack.  volatile: the compiler keeps
+  // the pop's LDS read and private read apart instead of merging them into
+  // one generic (flat) load through a selected pointer.  (An empty asm with
+  // a memory clobber does the same but makes the batched kernels re-read
+  // their scene views, which live in global memory, after every pop.)
+  volatile uint32_t spill[kStackMax - kStackLds];
   float best_t = tlim;
   float cull_t = tlim * kWiden;
   uint32_t best_prim = lim_prim;
@@ -334,15 +339,10 @@ __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, flo
     } else {
       if (sp == 0) break;
       --sp;
-      if (sp < kStackLds) {
+      if (sp < kStackLds)
         node = stk[sp * kStride];
-      } else {
-        // an empty asm statement keeps the two reads apart, so the compiler
-        // does not merge them into one generic (flat) load through a
-        // selected LDS / private pointer
-        asm volatile("" ::: "memory");
+      else
         node = spill[sp - kStackLds];
-      }
     }
   }
   t_out = best_t;
@@ -397,6 +397,7 @@ struct PathArgs {
   float4* accum;
   const float4* accum_prev;
   int32_t fp64;  // FP64 per-pixel sums: the launch picks the k_path<.., .., true> instantiation
+  double4* sum64;  // FP64: the image's sums and sample counts across subframes (full renders)
   float* s_rgb;
   int32_t* s_b;
   int32_t* s_o;
@@ -642,16 +643,31 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   };
 
   // shader.cu:156-166: the pixel's mean (and running mean over subframes).
+  // With FP64 sums kept across subframes (sum64), the mean is over every
+  // sample of subframes 0..subframe instead of a running FP32 lerp.
   auto write_pixel = [&]() {
     F3 out;
+    bool exact_mean = false;
     if (kF64) {
-      const double inv = 1.0 / (double)pp->spp;
-      out = mk((float)(ad0 * inv), (float)(ad1 * inv), (float)(ad2 * inv));
+      double s0 = ad0, s1 = ad1, s2 = ad2, n = (double)pp->spp;
+      if (pp->sum64) {
+        if (pp->subframe > 0) {
+          const double4 prev = pp->sum64[item];
+          s0 = prev.x + s0;
+          s1 = prev.y + s1;
+          s2 = prev.z + s2;
+          n = prev.w + n;
+        }
+        pp->sum64[item] = make_double4(s0, s1, s2, n);
+        exact_mean = true;
+      }
+      const double inv = 1.0 / n;
+      out = mk((float)(s0 * inv), (float)(s1 * inv), (float)(s2 * inv));
     } else {
       const float inv = 1.0f / (float)pp->spp;
       out = mk(acc[0], acc[kPathBlock], acc[2 * kPathBlock]) * inv;
     }
-    if (kMode == 0 && pp->subframe > 0 && pp->accum_prev) {
+    if (kMode == 0 && pp->subframe > 0 && pp->accum_prev && !exact_mean) {
       const float a = 1.0f / (float)(pp->subframe + 1);
       out = lerp(xyz(pp->accum_prev[item]), out, a);
     }
@@ -1106,6 +1122,15 @@ __global__ __launch_bounds__(256) void k_pick(const float4* __restrict__ a, cons
                                               int32_t n, float4* __restrict__ out) {
   const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
   if (i < n) out[i] = a[px[i]];
+}
+
+// accum = FP64 sums / sample count (the accum buffer of loaded FP64 sums).
+__global__ __launch_bounds__(256) void k_mean64(const double4* __restrict__ sum64, int64_t n, float4* accum) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double4 v = sum64[i];
+  const double inv = 1.0 / v.w;
+  accum[i] = make_float4((float)(v.x * inv), (float)(v.y * inv), (float)(v.z * inv), 1.0f);
 }
 
 __global__ __launch_bounds__(256) void k_blend(float4* accum, const float4* __restrict__ frame,
@@ -2307,6 +2332,11 @@ struct lisa_ctx {
   DevBuf<float4> c_jout, c_jacc;
   DevBuf<float> c_jrgb;
   WfBuffers wf;  // the wavefront form of lisa_render (LISA_FLAG_WAVEFRONT)
+  // FP64 progressive sums (LISA_FLAG_FP64_ACCUM full renders): r, g, b sums
+  // and the sample count per pixel; valid while they belong to the accum
+  // buffer's current sequence (an FP64 subframe 0, or lisa_accum64_load)
+  DevBuf<double4> c_sum64;
+  bool sum64_valid = false;
 };
 
 struct lisa_batch {
@@ -2406,6 +2436,7 @@ lisa_status ensure_accum(lisa_ctx* ctx, size_t npx) {
   ctx->accum_px = 0;
   ctx->gen++;
   ctx->j_seq = false;
+  ctx->sum64_valid = false;
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_accum), npx * sizeof(float4)));
   ctx->accum_px = npx;
   ctx->accum_bytes = npx * sizeof(float4);
@@ -2951,6 +2982,7 @@ void lisa_destroy(lisa_ctx* c) {
   c->c_jacc.release();
   c->c_jrgb.release();
   c->wf.release();
+  c->c_sum64.release();
   free_scene(c);
   (void)hipFree(c->d_accum);
   (void)hipFree(c->d_counters);
@@ -3040,6 +3072,13 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
     if (rc != LISA_OK) return rc;
   }
   ctx->j_seq = false;  // the buffer now holds a plain render
+  const bool f64 = (p->flags & LISA_FLAG_FP64_ACCUM) != 0;
+  if (f64) {
+    if (p->subframe > 0 && !ctx->sum64_valid)
+      return fail(ctx, LISA_ERR_INVALID_ARG,
+                  "FP64 subframe > 0 needs the FP64 sums of subframe 0 (or lisa_accum64_load)");
+    HIP_TRY(ctx, ctx->c_sum64.ensure(npx));
+  }
   PathArgs a = make_args(p);
   a.tiles_x = (p->width + kTileW - 1) / kTileW;
   const uint32_t tiles_y = (p->height + kTileH - 1) / kTileH;
@@ -3047,6 +3086,7 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
   a.accum = ctx->d_accum;
   a.accum_prev = ctx->d_accum;
   a.counters = ctx->d_counters;
+  a.sum64 = f64 ? ctx->c_sum64.p : nullptr;
   const bool count = (p->flags & LISA_FLAG_COUNT) != 0;
   hipStream_t st = ctx->stream;
   if (count)
@@ -3082,6 +3122,7 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
   }
   HIP_TRY(ctx, hipStreamSynchronize(st));
   HIP_TRY(ctx, hipEventElapsedTime(&ctx->render_ms, ctx->ev0, ctx->ev1));
+  ctx->sum64_valid = f64;  // a plain render leaves sums that no longer match the buffer
   if (count) {
     unsigned long long h[LISA_NCOUNTERS];
     HIP_TRY(ctx, hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
@@ -3221,6 +3262,39 @@ lisa_status lisa_copy_accum(lisa_ctx* ctx, float* rgba_out, int64_t n_pixels) {
   HIP_TRY(ctx, hipMemcpyAsync(rgba_out, ctx->d_accum, (size_t)n_pixels * sizeof(float4),
                               hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LISA_OK;
+}
+
+lisa_status lisa_accum64_save(lisa_ctx* ctx, double* sums_out, int64_t n_pixels) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!sums_out || n_pixels <= 0 || (size_t)n_pixels != ctx->accum_px)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "sums_out NULL or n_pixels != pixels of the last render");
+  if (!ctx->sum64_valid)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "no FP64 sums: the last render was not LISA_FLAG_FP64_ACCUM");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpyAsync(sums_out, ctx->c_sum64.p, (size_t)n_pixels * sizeof(double4), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LISA_OK;
+}
+
+lisa_status lisa_accum64_load(lisa_ctx* ctx, const double* sums, int64_t n_pixels) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (!sums || n_pixels <= 0 || n_pixels > (int64_t)1 << 31)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "sums NULL or n_pixels out of range");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t npx = (size_t)n_pixels;
+  if (ctx->accum_px != npx) {
+    lisa_status rc = ensure_accum(ctx, npx);
+    if (rc != LISA_OK) return rc;
+  }
+  ctx->j_seq = false;
+  HIP_TRY(ctx, ctx->c_sum64.ensure(npx));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->c_sum64.p, sums, npx * sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
+  k_mean64<<<grid_for((int64_t)npx, 256), 256, 0, ctx->stream>>>(ctx->c_sum64.p, (int64_t)npx, ctx->d_accum);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->sum64_valid = true;
   return LISA_OK;
 }
 

@@ -503,3 +503,52 @@ def test_wavefront_form_same_image(capi, oracle_mod, ctx):
             ctx.render(p)
     finally:
         plain.close()
+
+
+def test_fp64_progressive_and_resume(capi, oracle_mod, ctx):
+    """LISA_FLAG_FP64_ACCUM across subframes: subframe k's accum is the exact
+    FP64 mean of every sample of subframes 0..k (the kernel's per-pixel FP64
+    sums in sample order, checked bit for bit against the same sums of the
+    per-sample records); lisa_accum64_save after subframe 1 and
+    lisa_accum64_load into another context make its subframe 2 bit-identical
+    to the uninterrupted sequence (a resumable reference render)."""
+    W, H, spp = 48, 40, 8
+    sc = cornell(32, 16, W, H, spp, 7)
+    _pair(capi, oracle_mod, ctx, sc)
+    U, V, Wv = camera(sc, capi)
+    eye = sc["camera"]["eye"]
+    px = np.arange(W * H, dtype=np.int32)
+    sums = np.zeros((W * H, 3))
+    n = 0
+    saved = None
+    accs = []
+    for sub in range(3):
+        rec = ctx.trace_subset(capi.render_params(W, H, spp, 7, eye, U, V, Wv, subframe=sub), px,
+                               record_bounces=0)["sample_rgb"].astype(np.float64)
+        part = np.zeros((W * H, 3))
+        for k in range(spp):  # the kernel's order: samples in turn, then onto the sums so far
+            part = part + rec[:, k]
+        sums = part if sub == 0 else sums + part
+        n += spp
+        p = capi.render_params(W, H, spp, 7, eye, U, V, Wv, subframe=sub, flags=capi.FLAG_FP64_ACCUM)
+        acc, _ = ctx.render(p)
+        want = (sums * (1.0 / n)).astype(np.float32)
+        assert np.array_equal(acc[:, :3].view(np.uint32), want.view(np.uint32)), sub
+        accs.append(acc)
+        if sub == 1:
+            saved = ctx.accum64_save(W * H)
+            assert np.all(saved[:, 3] == 2 * spp)
+    other = capi.Context(0)
+    try:
+        other.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        p = capi.render_params(W, H, spp, 7, eye, U, V, Wv, subframe=2, flags=capi.FLAG_FP64_ACCUM)
+        other.render(capi.render_params(W, H, spp, 7, eye, U, V, Wv))  # a plain render: no FP64 sums
+        with pytest.raises(capi.LisaError, match="FP64 sums"):
+            other.render(p)
+        other.accum64_load(saved)
+        assert np.array_equal(other.copy_accum(W * H).view(np.uint32), accs[1].view(np.uint32))
+        acc, _ = other.render(p)
+        assert np.array_equal(acc.view(np.uint32), accs[2].view(np.uint32))
+        assert np.array_equal(other.accum64_save(W * H), ctx.accum64_save(W * H))
+    finally:
+        other.close()
