@@ -3,13 +3,15 @@
 //
 //   k_mlp_infer      frequency encoding + 5-layer forward, 16 queries per
 //                    wave-tile, weights and activations in registers
-//   k_mlp_train      the same forward with every activation kept, the
-//                    relative-L2 loss gradient and the backward data pass
-//                    (dZ_l = (dZ_{l+1} W_{l+1}^T) * [A_{l+1} > 0]); writes the
-//                    layer inputs and dZ feature-major for the weight GEMMs
-//   k_mlp_wgrad      dW_l^T = dZ_l * in_l^T over the batch (split-K, atomics)
-//   k_mlp_adam       Adam on the FP32 master weights, BF16 copies for the
-//                    forward (W^T) and backward (W) layouts
+//   k_mlp_encode     the frequency encoding of a training call's rows, once
+//                    for all of its steps (sample- and feature-major copies)
+//   k_mlp_train      the forward pass with the relative-L2 loss gradient and
+//                    the backward data pass (dZ_l = (dZ_{l+1} W_{l+1}^T) *
+//                    [A_{l+1} > 0]); writes the layer inputs and dZ
+//                    feature-major for the weight GEMMs
+//   k_mlp_wgrad      dW_l^T = dZ_l * in_l^T over the batch (split-K partials)
+//   k_mlp_adam       the partials' sum, Adam on the FP32 master weights, BF16
+//                    copies for the forward (W^T) and backward (W) layouts
 //
 // MFMA 16x16x32 lane layout (64 lanes, q = lane / 16, r = lane % 16):
 //   A (16 x 32): lane holds A[r][8q .. 8q+7]
@@ -36,19 +38,13 @@ typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
-// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet), and back.
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
 __host__ __device__ inline uint16_t f2bf(float f) {
   uint32_t u;
   std::memcpy(&u, &f, 4);
   if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
-}
-__host__ __device__ inline float bf2f(uint16_t h) {
-  const uint32_t u = (uint32_t)h << 16;
-  float f;
-  std::memcpy(&f, &u, 4);
-  return f;
 }
 
 __device__ inline bf16x8 ld8(const uint16_t* p) {  // 16-byte aligned
@@ -60,7 +56,6 @@ __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 
 // LDS row strides (elements): padded by 8 so the 16 rows a wave reads
 // together fall on different banks; 16-byte alignment is kept.
-constexpr int kXStride = kEncPad + 8;  // 168
 constexpr int kHStride = kHidden + 8;  // 72
 constexpr int kGStride = kHidden + 8;  // dZ tiles
 __host__ __device__ constexpr int wt_stride(int l) { return kInDim[l] + 8; }
@@ -86,83 +81,53 @@ __host__ __device__ constexpr int wb_lds_offset(int l) {
 }
 constexpr int kWbLds = wb_lds_offset(kLayers);  // 16384 elements
 
-// Frequency encoding of one input: (sin, cos)(pi 2^b p) for b = 0..5, written
-// at features 12 i + 2 b + {0, 1}.
-__device__ inline void encode_one(float p, uint16_t* row, int i) {
+// Frequency encoding of one input: (sin, cos)(pi 2^b p) for b = 0..5 into
+// e[2 b], e[2 b + 1] (input i's features are 12 i + 2 b + {0, 1}).
+__device__ inline void encode_one(float p, uint16_t* e) {
 #pragma unroll
   for (int b = 0; b < kBands; ++b) {
     float s, c;
     sincospif(p * (float)(1 << b), &s, &c);
-    row[kBands * 2 * i + 2 * b] = f2bf(s);
-    row[kBands * 2 * i + 2 * b + 1] = f2bf(c);
+    e[2 * b] = f2bf(s);
+    e[2 * b + 1] = f2bf(c);
   }
 }
 
-// Copies the forward weights W^T (global, row stride in_pad) into LDS
-// (row stride in_pad + 8), and optionally the backward copies.
+// Copies the forward weights W^T (global, row stride in_pad) and the
+// backward copies W (layers 1..4) into LDS (row strides + 8).  The 16-byte
+// chunks are numbered over both arrays; a thread of the kThreads-thread block
+// loads all of its chunks before it stores any (one memory round trip).
+template <int kThreads>
 __device__ void load_weights(const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
                              uint16_t* s_wt, uint16_t* s_wb) {
-  for (int l = 0; l < kLayers; ++l) {
-    const int rows = kOutDim[l], cols = kInDim[l];
-    for (int e = threadIdx.x; e < rows * cols / 8; e += blockDim.x) {
-      const int r = e / (cols / 8), c = (e % (cols / 8)) * 8;
-      *reinterpret_cast<u32x4*>(s_wt + wt_lds_offset(l) + r * wt_stride(l) + c) =
-          *reinterpret_cast<const u32x4*>(wt + kOffset[l] + r * cols + c);
-    }
-  }
-  if (s_wb) {
-    for (int l = 1; l < kLayers; ++l) {
-      const int rows = kHidden, cols = wb_out(l);
-      for (int e = threadIdx.x; e < rows * cols / 8; e += blockDim.x) {
-        const int r = e / (cols / 8), c = (e % (cols / 8)) * 8;
-        *reinterpret_cast<u32x4*>(s_wb + wb_lds_offset(l) + r * wb_stride(l) + c) =
-            *reinterpret_cast<const u32x4*>(wb + wb_global_offset(l) + r * cols + c);
-      }
-    }
-  }
-}
-
-// Encodes 16 samples (rows of `feat`, 12 floats each; rows >= n are zero)
-// into the wave's X tile.
-__device__ inline void encode_tile(const float* __restrict__ feat, int stride, int64_t base,
-                                   int64_t n, uint16_t* X, int lane) {
-  for (int e = lane; e < 16 * kInputs; e += 64) {
-    const int r = e / kInputs, i = e % kInputs;
-    const int64_t m = base + r;
-    const float p = m < n ? feat[m * stride + i] : 0.0f;
-    encode_one(p, X + r * kXStride, i);
-  }
-  for (int e = lane; e < 16 * (kEncPad - kEnc); e += 64) {
-    const int r = e / (kEncPad - kEnc), c = kEnc + e % (kEncPad - kEnc);
-    X[r * kXStride + c] = 0;
-  }
-}
-
-// One layer on a 16-row tile: out = in (16 x K) * W_l (K x out_pad).
-// Hidden layers apply ReLU and store BF16 rows into `out_tile`; the last
-// layer returns the raw FP32 accumulator of the single 16-column tile.
-__device__ inline f32x4 layer_fwd(int l, const uint16_t* in, int in_stride, const uint16_t* s_wt,
-                                  uint16_t* out_tile, int lane) {
-  const int q = lane >> 4, r = lane & 15;
-  const uint16_t* w = s_wt + wt_lds_offset(l);
-  const int K = kInDim[l], ntiles = kOutDim[l] / 16;
-  f32x4 acc_last = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int nt = 0; nt < ntiles; ++nt) {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int ks = 0; ks < K / 32; ++ks) {
-      const bf16x8 a = ld8(in + r * in_stride + ks * 32 + 8 * q);
-      const bf16x8 b = ld8(w + (nt * 16 + r) * wt_stride(l) + ks * 32 + 8 * q);
-      acc = mfma(a, b, acc);
-    }
-    if (l < kLayers - 1) {
+  constexpr int kWtChunks = kParams / 8, kChunks = (kParams + kWbGlobal) / 8;
+  constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
+  u32x4 v[kPer];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        out_tile[(4 * q + i) * kHStride + nt * 16 + r] = f2bf(fmaxf(acc[i], 0.0f));
-    } else {
-      acc_last = acc;
+  for (int k = 0; k < kPer; ++k) {
+    const int e = (int)threadIdx.x + k * kThreads;
+    if (e < kWtChunks)
+      v[k] = reinterpret_cast<const u32x4*>(wt)[e];
+    else if (e < kChunks)
+      v[k] = reinterpret_cast<const u32x4*>(wb)[e - kWtChunks];
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = (int)threadIdx.x + k * kThreads;
+    if (e < kWtChunks) {
+      const int g = 8 * e;
+      int l = 0;
+      while (g >= kOffset[l + 1]) ++l;
+      const int r = (g - kOffset[l]) / kInDim[l], c = (g - kOffset[l]) % kInDim[l];
+      *reinterpret_cast<u32x4*>(s_wt + wt_lds_offset(l) + r * wt_stride(l) + c) = v[k];
+    } else if (e < kChunks) {
+      const int g = 8 * (e - kWtChunks);
+      int l = 1;
+      while (l < kLayers - 1 && g >= wb_global_offset(l + 1)) ++l;
+      const int r = (g - wb_global_offset(l)) / wb_out(l), c = (g - wb_global_offset(l)) % wb_out(l);
+      *reinterpret_cast<u32x4*>(s_wb + wb_lds_offset(l) + r * wb_stride(l) + c) = v[k];
     }
   }
-  return acc_last;
 }
 
 // ---- inference with the weights and activations in registers ----
@@ -180,6 +145,10 @@ __device__ inline f32x4 layer_fwd(int l, const uint16_t* in, int in_stride, cons
 // 6 octaves, which the encoded-feature layout 12 i + 2 b + {0, 1} places at
 // 36 q .. 36 q + 35.  Activations never leave the lane.
 constexpr int kInferWaves = 4;
+
+// FP32 -> BF16 bits, round to nearest even (the hardware conversion; the
+// same bits as f2bf for every non-NaN value).
+__device__ inline uint16_t bf16_rne(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 
 __device__ inline uint32_t pack_bf16(float a, float b) {
   bf16x2 v;
@@ -364,12 +333,42 @@ int infer_blocks(int cus, int64_t n) {
                                                      3 * (int64_t)cus));
 }
 
-// Waves per training workgroup: they share one LDS copy of the weights in
-// both layouts (84 KB) and hold 17 KB of activations each, 152 KB in all, so
-// one workgroup per CU; the grid is persistent (one workgroup per CU), each
-// wave looping over 16-row tiles, so the weights are loaded once per CU.
-constexpr int kTrainWaves = 4;
-constexpr int kLossSlots = 64;  // one loss accumulator per step of a training call
+// Frequency encoding of a training call's rows, once for all of its steps:
+// thread (row m, input i) writes the input's 12 features into the
+// sample-major copy xs (row stride kEncPad; the layer-0 A fragments of
+// k_mlp_train) and the feature-major copy xf (row stride npad; layer 0's
+// input for the weight gradients).  Rows n <= m < npad encode p = 0 (their
+// loss gradient is zero).  Threads of input 0 also write features 144..159.
+constexpr int kEncRows = 64;  // rows per block; blockDim = kEncRows * kInputs
+__global__ __launch_bounds__(kEncRows * kInputs) void k_mlp_encode(
+    const float* __restrict__ feat, int feat_stride, int64_t n, int64_t npad,
+    uint16_t* __restrict__ xs, uint16_t* __restrict__ xf) {
+  const int i = threadIdx.x / kEncRows;
+  const int64_t m = (int64_t)blockIdx.x * kEncRows + threadIdx.x % kEncRows;
+  if (m >= npad) return;
+  const float p = m < n ? feat[m * feat_stride + i] : 0.0f;
+  uint16_t row[2 * kBands];
+  encode_one(p, row);
+  for (int j = 0; j < 2 * kBands; ++j) {
+    xs[m * kEncPad + 2 * kBands * i + j] = row[j];
+    xf[(int64_t)(2 * kBands * i + j) * npad + m] = row[j];
+  }
+  if (i == 0) {
+    for (int f = kEnc; f < kEncPad; ++f) {
+      xs[m * kEncPad + f] = 0;
+      xf[(int64_t)f * npad + m] = 0;
+    }
+  }
+}
+
+// Waves per training workgroup (two per SIMD): they share one LDS copy of
+// the weights in both layouts (84 KB) and hold 6.9 KB each (two activation
+// tiles used in turn through the forward pass and one dZ tile), 140 KB in
+// all, so one workgroup per CU; the grid is persistent (at most one
+// workgroup per CU), each wave looping over 16-row tiles.
+constexpr int kTrainWaves = 8;
+constexpr int kLossSlots = 64;  // one loss per step of a training call
+constexpr int kMaxSplit = 64;   // K splits of the weight-gradient GEMMs (partials, no atomics)
 
 // One feature of a 16-row tile (src[i * stride], i < 16) to 16 consecutive
 // BF16 values at dst (32-byte aligned), as two 16-byte stores.
@@ -388,24 +387,26 @@ __device__ inline void store4(const uint16_t* v, uint16_t* dst) {
 }
 
 // Forward + relative-L2 loss + backward data pass for 16 samples per wave
-// tile.  Writes feature-major (row = feature, column = sample, row stride npad):
-//   act_l (l = 0..4): the input of layer l (X, A1..A4), BF16
+// tile.  Layer 0's input comes from xs (k_mlp_encode); writes feature-major
+// (row = feature, column = sample, row stride npad):
+//   act_l (l = 1..4): the input of layer l (A1..A4), BF16
 //   dz_l  (l = 0..4): dLoss/dZ_l, BF16 (dz_4 has 16 rows, 3 used)
+// and the wave's share of the loss into loss_part[wave of the grid].  The
+// ReLU masks of the backward pass stay in registers: lane (q, r) holds
+// the D elements [4q + i][16 nt + r] of every layer, in the forward and the
+// backward pass alike.
 __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
-    const float* __restrict__ feat, int feat_stride, const float* __restrict__ target, int64_t n,
-    int64_t npad, float inv_count, const uint16_t* __restrict__ wt,
-    const uint16_t* __restrict__ wb, uint16_t* __restrict__ act, uint16_t* __restrict__ dz,
-    float* __restrict__ loss) {
+    const uint16_t* __restrict__ xs, const float* __restrict__ target, int64_t n, int64_t npad,
+    float inv_count, const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
+    uint16_t* __restrict__ act, uint16_t* __restrict__ dz, float* __restrict__ loss_part) {
   __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtLds];
   __shared__ __attribute__((aligned(16))) uint16_t s_wb[kWbLds];
-  __shared__ __attribute__((aligned(16))) uint16_t s_x[kTrainWaves][16 * kXStride];
-  __shared__ __attribute__((aligned(16))) uint16_t s_a[kTrainWaves][4][16 * kHStride];
+  __shared__ __attribute__((aligned(16))) uint16_t s_a[kTrainWaves][2][16 * kHStride];
   __shared__ __attribute__((aligned(16))) uint16_t s_g[kTrainWaves][16 * kGStride];
-  load_weights(wt, wb, s_wt, s_wb);
+  load_weights<64 * kTrainWaves>(wt, wb, s_wt, s_wb);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = lane >> 4, r = lane & 15;
-  uint16_t* X = s_x[wave];
   uint16_t* G = s_g[wave];
   // feature-major global buffers
   uint16_t* act_l[kLayers];
@@ -424,26 +425,56 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
   }
   float loss_acc = 0.0f;
   const int64_t tiles = npad / 16;
-  for (int64_t t = (int64_t)blockIdx.x * kTrainWaves + wave; t < tiles;
-       t += (int64_t)gridDim.x * kTrainWaves) {
+  const int64_t t_step = (int64_t)gridDim.x * kTrainWaves;
+  // layer 0's A fragments straight from the sample-major encoding, one tile ahead
+  bf16x8 x[kEncPad / 32], xn[kEncPad / 32];
+  auto load_x = [&](int64_t t, bf16x8 (&dst)[kEncPad / 32]) {
+#pragma unroll
+    for (int ks = 0; ks < kEncPad / 32; ++ks) dst[ks] = ld8(xs + (t * 16 + r) * kEncPad + ks * 32 + 8 * q);
+  };
+  const int64_t t0 = (int64_t)blockIdx.x * kTrainWaves + wave;
+  if (t0 < tiles) load_x(t0, xn);
+  for (int64_t t = t0; t < tiles; t += t_step) {
     const int64_t base = t * 16;
-    encode_tile(feat, feat_stride, base, n, X, lane);
-    __builtin_amdgcn_wave_barrier();
-    // forward, keeping A1..A4
-    const uint16_t* in = X;
-    int in_stride = kXStride;
-    for (int l = 0; l < kLayers - 1; ++l) {
-      layer_fwd(l, in, in_stride, s_wt, s_a[wave][l], lane);
+#pragma unroll
+    for (int ks = 0; ks < kEncPad / 32; ++ks) x[ks] = xn[ks];
+    if (t + t_step < tiles) load_x(t + t_step, xn);
+    // mask bit 16 l + 4 nt + i: [A_{l+1}[4q + i][16 nt + r] > 0], l = 0..3
+    uint64_t mask = 0;
+    f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int l = 0; l < kLayers; ++l) {
+      const uint16_t* w = s_wt + wt_lds_offset(l);
+      const uint16_t* in = s_a[wave][(l + 1) & 1];  // output of layer l - 1
+      uint16_t* out = s_a[wave][l & 1];
+      const int ntiles = kOutDim[l] / 16;
+      for (int nt = 0; nt < ntiles; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (l == 0) {
+#pragma unroll
+          for (int ks = 0; ks < kEncPad / 32; ++ks)
+            acc = mfma(x[ks], ld8(w + (nt * 16 + r) * wt_stride(0) + ks * 32 + 8 * q), acc);
+        } else {
+#pragma unroll
+          for (int ks = 0; ks < kHidden / 32; ++ks)
+            acc = mfma(ld8(in + r * kHStride + ks * 32 + 8 * q),
+                       ld8(w + (nt * 16 + r) * wt_stride(l) + ks * 32 + 8 * q), acc);
+        }
+        if (l < kLayers - 1) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const uint16_t h = bf16_rne(relu(acc[i]));
+            out[(4 * q + i) * kHStride + nt * 16 + r] = h;
+            if (h != 0) mask |= 1ull << (16 * l + 4 * nt + i);
+          }
+        } else {
+          y = acc;
+        }
+      }
       __builtin_amdgcn_wave_barrier();
-      in = s_a[wave][l];
-      in_stride = kHStride;
+      // the layer's output is the input of layer l + 1: feature-major to HBM
+      if (l < kLayers - 1) store16(out + lane, kHStride, act_l[l + 1] + (int64_t)lane * npad + base);
+      __builtin_amdgcn_wave_barrier();
     }
-    const f32x4 y = layer_fwd(kLayers - 1, in, in_stride, s_wt, nullptr, lane);
-    // layer inputs, feature-major: X (160 rows), A1..A4 (64 rows); a lane
-    // writes a feature's 16 samples as two 16-byte stores
-    for (int f = lane; f < kEncPad; f += 64) store16(X + f, kXStride, act_l[0] + (int64_t)f * npad + base);
-    for (int l = 1; l < kLayers; ++l)
-      store16(s_a[wave][l - 1] + lane, kHStride, act_l[l] + (int64_t)lane * npad + base);
     // dLoss/dY for L = mean over samples and channels of (y - t)^2 / (y^2 + 0.01),
     // the denominator held constant (relative L2)
     uint16_t gq[4];
@@ -459,7 +490,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
           loss_acc += d * d / den * inv_count;
         }
       }
-      gq[i] = f2bf(g);
+      gq[i] = bf16_rne(g);
       G[(4 * q + i) * kGStride + r] = gq[i];
     }
     store4(gq, dz_l[kLayers - 1] + (int64_t)r * npad + base + 4 * q);
@@ -479,29 +510,33 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(
         }
       }
       __builtin_amdgcn_wave_barrier();
-      const uint16_t* A = s_a[wave][l - 1];  // input of layer l = output of layer l-1
       for (int nt = 0; nt < 4; ++nt) {
         uint16_t gq[4];
         const int col = nt * 16 + r;
         for (int i = 0; i < 4; ++i) {
-          const int row = 4 * q + i;
-          const float g = bf2f(A[row * kHStride + col]) > 0.0f ? acc[nt][i] : 0.0f;
-          gq[i] = f2bf(g);
-          G[row * kGStride + col] = gq[i];
+          const float g = (mask >> (16 * (l - 1) + 4 * nt + i)) & 1u ? acc[nt][i] : 0.0f;
+          gq[i] = bf16_rne(g);
+          G[(4 * q + i) * kGStride + col] = gq[i];
         }
         store4(gq, dz_l[l - 1] + (int64_t)col * npad + base + 4 * q);
       }
       __builtin_amdgcn_wave_barrier();
     }
   }
-  // wave-reduce the loss, one atomic per wave
+  // the wave's loss: one plain store per wave (summed by k_mlp_adam / k_mlp_reduce)
   for (int o = 32; o > 0; o >>= 1) loss_acc += __shfl_xor(loss_acc, o, 64);
-  if (lane == 0) atomicAdd(loss, loss_acc);
+  if (lane == 0) loss_part[blockIdx.x * kTrainWaves + wave] = loss_acc;
 }
 
-// dW_l^T (out_pad x in_pad) += dZ_l (out_pad x npad) * in_l^T (npad x in_pad)
-// for every layer in one launch; one wave per (layer, 16 x 16 tile, K split):
-// waves [first[l], first[l + 1]) belong to layer l.
+// dW_l^T (out_pad x in_pad) = dZ_l (out_pad x npad) * in_l^T (npad x in_pad)
+// for every layer in one launch.  One wave per (layer, column block, K
+// split): a column block is all of the layer's output rows (4 tiles of 16, 1
+// for the last layer) times kWgIT input tiles, so each dZ fragment serves
+// kWgIT MFMAs and each input fragment up to 4; waves [first[l], first[l + 1])
+// belong to layer l.  Split s writes its partial sums to part[s * kParams + p]
+// (plain stores, every p of the block).
+constexpr int kWgIT = 2;
+static_assert(kEncPad % (16 * kWgIT) == 0 && kHidden % (16 * kWgIT) == 0, "whole column blocks");
 struct WgradPlan {
   int64_t off_dz[kLayers], off_in[kLayers];
   int first[kLayers + 1];
@@ -509,7 +544,7 @@ struct WgradPlan {
 };
 __global__ __launch_bounds__(256) void k_mlp_wgrad(const uint16_t* __restrict__ dz_all,
                                                    const uint16_t* __restrict__ in_all, int64_t npad,
-                                                   WgradPlan plan, float* __restrict__ g) {
+                                                   WgradPlan plan, float* __restrict__ part) {
   int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (wave >= plan.first[kLayers]) return;
@@ -519,33 +554,124 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(const uint16_t* __restrict__ 
   const uint16_t* __restrict__ dz = dz_all + plan.off_dz[l];
   const uint16_t* __restrict__ in = in_all + plan.off_in[l];
   const int in_pad = kInDim[l], ksplit = plan.ksplit;
-  float* __restrict__ dwt = g + kOffset[l];
+  const int n_ot = kOutDim[l] / 16;  // wave-uniform: 4, or 1 for the last layer
   const int q = lane >> 4, r = lane & 15;
-  const int tiles_in = in_pad / 16;
-  const int tile = wave / ksplit, split = wave % ksplit;
-  const int ot = tile / tiles_in, it = tile % tiles_in;
+  const int blk = wave / ksplit, split = wave % ksplit;
+  float* __restrict__ dwt = part + (int64_t)split * kParams + kOffset[l];
   const int64_t steps = npad / 32;
   const int64_t s0 = steps * split / ksplit, s1 = steps * (split + 1) / ksplit;
-  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  const uint16_t* arow = dz + (int64_t)(ot * 16 + r) * npad + 8 * q;
-  const uint16_t* brow = in + (int64_t)(it * 16 + r) * npad + 8 * q;
-  for (int64_t s = s0; s < s1; ++s) acc = mfma(ld8(arow + 32 * s), ld8(brow + 32 * s), acc);
-  for (int i = 0; i < 4; ++i)
-    atomicAdd(&dwt[(int64_t)(ot * 16 + 4 * q + i) * in_pad + it * 16 + r], acc[i]);
+  f32x4 acc[4][kWgIT];
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+#pragma unroll
+    for (int j = 0; j < kWgIT; ++j) acc[o][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint16_t* arow = dz + (int64_t)r * npad + 8 * q;
+  const uint16_t* brow = in + (int64_t)(blk * kWgIT * 16 + r) * npad + 8 * q;
+  auto step = [&](int64_t s, int nu) {
+    // nu K-steps from s: every load first, then the MFMAs
+    bf16x8 a[4][4], b[4][kWgIT];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (u < nu) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+          if (o < n_ot) a[u][o] = ld8(arow + (int64_t)o * 16 * npad + 32 * (s + u));
+#pragma unroll
+        for (int j = 0; j < kWgIT; ++j) b[u][j] = ld8(brow + (int64_t)j * 16 * npad + 32 * (s + u));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (u < nu) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+          if (o < n_ot) {
+#pragma unroll
+            for (int j = 0; j < kWgIT; ++j) acc[o][j] = mfma(a[u][o], b[u][j], acc[o][j]);
+          }
+      }
+    }
+  };
+  int64_t s = s0;
+  for (; s + 4 <= s1; s += 4) step(s, 4);
+  if (s < s1) step(s, (int)(s1 - s));
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+    if (o < n_ot) {
+#pragma unroll
+      for (int j = 0; j < kWgIT; ++j)
+        for (int i = 0; i < 4; ++i)
+          dwt[(int64_t)(o * 16 + 4 * q + i) * in_pad + (blk * kWgIT + j) * 16 + r] = acc[o][j][i];
+    }
 }
 
-// Adam (Kingma & Ba) on the FP32 master weights (packed W^T layout); padded
-// entries stay zero.  Writes the BF16 forward (W^T) and backward (W) copies.
+// Sum of n floats at p (the block's 256 threads; result valid in thread 0).
+__device__ float block_sum(const float* __restrict__ p, int n, float* s_red) {
+  float a = 0.0f;
+  for (int k = threadIdx.x; k < n; k += 256) a += p[k];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  return s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+// The weight gradient of p from the K-split partials: each of four
+// neighbouring lanes (k = lane % 4) sums the splits k, k + 4, ... in order,
+// then the four sums are added in a fixed tree; every lane of the group
+// returns the total.  kParamsPerBlock weights per 256-thread block.
+constexpr int kParamsPerBlock = 64;
+static_assert(kParams % kParamsPerBlock == 0, "whole blocks of weights");
+__device__ inline float grad_of(const float* __restrict__ part, int ksplit, int p) {
+  const int k = threadIdx.x & 3;
+  float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int s = k;
+  for (; s + 12 < ksplit; s += 16) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g[u] += part[(int64_t)(s + 4 * u) * kParams + p];
+  }
+  for (; s < ksplit; s += 4) g[0] += part[(int64_t)s * kParams + p];
+  float t = (g[0] + g[1]) + (g[2] + g[3]);
+  t += __shfl_xor(t, 1, 64);
+  t += __shfl_xor(t, 2, 64);
+  return t;
+}
+
+// Gradients and loss of one k_mlp_train + k_mlp_wgrad pass into g[0..kParams)
+// and *loss (lisa_cache_gradients).
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, int ksplit,
+                                                    const float* __restrict__ loss_part,
+                                                    int n_loss, float* __restrict__ g,
+                                                    float* __restrict__ loss) {
+  __shared__ float s_red[4];
+  if (blockIdx.x == 0) {
+    const float l = block_sum(loss_part, n_loss, s_red);
+    if (threadIdx.x == 0) *loss = l;
+  }
+  const int p = blockIdx.x * kParamsPerBlock + (threadIdx.x >> 2);
+  const float gr = grad_of(part, ksplit, p);  // kParams is a multiple of kParamsPerBlock
+  if ((threadIdx.x & 3) == 0) g[p] = gr;
+}
+
+// Adam (Kingma & Ba) on the FP32 master weights (packed W^T layout) with the
+// gradients summed from the K-split partials; padded entries stay zero.
+// Writes the BF16 forward (W^T) and backward (W) copies; block 0 also sums
+// the step's loss into *loss.
 __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ w, float* __restrict__ m,
                                                   float* __restrict__ v,
-                                                  float* __restrict__ g, float lr, float b1,
+                                                  const float* __restrict__ part, int ksplit,
+                                                  const float* __restrict__ loss_part, int n_loss,
+                                                  float* __restrict__ loss, float lr, float b1,
                                                   float b2, float eps, float c1, float c2,
                                                   uint16_t* __restrict__ wt,
                                                   uint16_t* __restrict__ wb) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= kParams) return;
-  const float gr = g[p];
-  g[p] = 0.0f;  // the next step's weight gradients accumulate from zero
+  __shared__ float s_red[4];
+  if (blockIdx.x == 0) {
+    const float l = block_sum(loss_part, n_loss, s_red);
+    if (threadIdx.x == 0) *loss = l;
+  }
+  const int p = blockIdx.x * kParamsPerBlock + (threadIdx.x >> 2);
+  const float gr = grad_of(part, ksplit, p);  // kParams is a multiple of kParamsPerBlock
+  if ((threadIdx.x & 3) != 0) return;
   int l = 0;
   while (p >= kOffset[l + 1]) ++l;
   const int o = (p - kOffset[l]) / kInDim[l], i = (p - kOffset[l]) % kInDim[l];
@@ -561,11 +687,6 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ w, float* 
   if (l >= 1) wb[wb_global_offset(l) + i * wb_out(l) + o] = h;
 }
 
-__global__ void k_zero_f32(float* p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-    p[i] = 0.0f;
-}
-
 }  // namespace
 
 // ===========================================================================
@@ -577,12 +698,14 @@ struct lisa_cache {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr, *loss = nullptr;
+  float* part = nullptr;       // kMaxSplit x kParams weight-gradient partials
+  float* loss_part = nullptr;  // kLossSlots x (cus * kTrainWaves) per-wave losses
   uint16_t *wt = nullptr, *wb = nullptr;
   float lr = 1e-3f, b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   int64_t step = 0;
   // training scratch (grown on demand)
   int64_t cap = 0;
-  uint16_t *act = nullptr, *dz = nullptr;
+  uint16_t *act = nullptr, *dz = nullptr, *xs = nullptr;
   float *feat = nullptr, *tgt = nullptr, *out = nullptr;
   int64_t io_cap = 0;
   float last_ms = 0.0f;
@@ -667,7 +790,8 @@ lisa_status ensure_train(lisa_cache* c, int64_t npad) {
   if (npad <= c->cap) return LISA_OK;
   (void)hipFree(c->act);
   (void)hipFree(c->dz);
-  c->act = c->dz = nullptr;
+  (void)hipFree(c->xs);
+  c->act = c->dz = c->xs = nullptr;
   c->cap = 0;
   int64_t rows_act = 0, rows_dz = 0;
   for (int l = 0; l < kLayers; ++l) {
@@ -676,38 +800,50 @@ lisa_status ensure_train(lisa_cache* c, int64_t npad) {
   }
   CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->act), 2 * (size_t)rows_act * npad));
   CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->dz), 2 * (size_t)rows_dz * npad));
+  CHIP(c, hipMalloc(reinterpret_cast<void**>(&c->xs), 2 * (size_t)kEncPad * npad));
   c->cap = npad;
   return LISA_OK;
 }
 
-// Gradients of the mean relative-L2 loss over n device-resident samples
-// accumulated into c->g, which must be zero (k_mlp_adam leaves it zero; the
-// caller zeroes it before a first step); the loss into *loss_slot (zeroed
-// by the caller).  Two launches: k_mlp_train, k_mlp_wgrad.
-lisa_status grads_device(lisa_cache* c, const float* d_feat, int feat_stride, const float* d_tgt,
-                         int64_t n, float* loss_slot) {
+// Encodes n device-resident samples (features with row stride feat_stride)
+// for the steps that follow (k_mlp_encode into xs and layer 0's feature-major
+// input).
+lisa_status encode_device(lisa_cache* c, const float* d_feat, int feat_stride, int64_t n) {
   const int64_t npad = (n + 31) / 32 * 32;
   lisa_status rc = ensure_train(c, npad);
   if (rc != LISA_OK) return rc;
+  k_mlp_encode<<<(int)((npad + kEncRows - 1) / kEncRows), kEncRows * kInputs, 0, c->stream>>>(
+      d_feat, feat_stride, n, npad, c->xs, c->act);
+  CHIP(c, hipGetLastError());
+  return LISA_OK;
+}
+
+// One gradient pass over the n encoded samples (targets n x 3): the weight
+// gradients as ksplit partials in c->part, the per-wave losses in
+// loss_part[0 .. *n_loss).  Two launches: k_mlp_train, k_mlp_wgrad.
+lisa_status grads_step(lisa_cache* c, const float* d_tgt, int64_t n, float* loss_part,
+                       int* ksplit, int* n_loss) {
+  const int64_t npad = (n + 31) / 32 * 32;
   hipStream_t st = c->stream;
   const int64_t tiles = npad / 16;
   const int blocks = (int)std::min<int64_t>((tiles + kTrainWaves - 1) / kTrainWaves, c->cus);
-  k_mlp_train<<<blocks, 64 * kTrainWaves, 0, st>>>(d_feat, feat_stride, d_tgt, n, npad,
-                                                    1.0f / (3.0f * (float)n), c->wt, c->wb, c->act,
-                                                    c->dz, loss_slot);
+  k_mlp_train<<<blocks, 64 * kTrainWaves, 0, st>>>(c->xs, d_tgt, n, npad, 1.0f / (3.0f * (float)n),
+                                                    c->wt, c->wb, c->act, c->dz, loss_part);
   WgradPlan plan;
-  plan.ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(npad / 32 / 16, 64));
+  plan.ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(npad / 32 / 16, kMaxSplit));
   int64_t off_a = 0, off_d = 0;
   plan.first[0] = 0;
   for (int l = 0; l < kLayers; ++l) {
     plan.off_dz[l] = off_d;
     plan.off_in[l] = off_a;
-    plan.first[l + 1] = plan.first[l] + (kOutDim[l] / 16) * (kInDim[l] / 16) * plan.ksplit;
+    plan.first[l + 1] = plan.first[l] + kInDim[l] / 16 / kWgIT * plan.ksplit;
     off_a += (int64_t)kInDim[l] * npad;
     off_d += (int64_t)kOutDim[l] * npad;
   }
-  k_mlp_wgrad<<<(plan.first[kLayers] + 3) / 4, 256, 0, st>>>(c->dz, c->act, npad, plan, c->g);
+  k_mlp_wgrad<<<(plan.first[kLayers] + 3) / 4, 256, 0, st>>>(c->dz, c->act, npad, plan, c->part);
   CHIP(c, hipGetLastError());
+  *ksplit = plan.ksplit;
+  *n_loss = blocks * kTrainWaves;
   return LISA_OK;
 }
 
@@ -729,17 +865,20 @@ lisa_status lisa_cache_create(lisa_cache** out, int32_t device, uint64_t seed, f
       (e = hipMalloc(reinterpret_cast<void**>(&c->m), 4 * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->v), 4 * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->g), 4 * (kParams + kLossSlots))) != hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->part), 4 * (size_t)kMaxSplit * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->wt), 2 * kParams)) != hipSuccess ||
       (e = hipMalloc(reinterpret_cast<void**>(&c->wb), 2 * kWbGlobal)) != hipSuccess ||
       (e = hipMemset(c->m, 0, 4 * kParams)) != hipSuccess ||
       (e = hipMemset(c->v, 0, 4 * kParams)) != hipSuccess ||
       (e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device)) !=
-          hipSuccess) {
+          hipSuccess ||
+      (e = hipMalloc(reinterpret_cast<void**>(&c->loss_part),
+                     4 * (size_t)kLossSlots * c->cus * kTrainWaves)) != hipSuccess) {
     const std::string msg = std::string("lisa_cache_create: ") + hipGetErrorString(e);
     lisa_cache_destroy(c);
     return cfail(nullptr, LISA_ERR_HIP, msg);
   }
-  c->loss = c->g + kParams;  // the per-step loss slots follow the gradients (one zeroing launch)
+  c->loss = c->g + kParams;  // the per-step losses follow the gradients
   std::vector<float> w;
   init_weights(w, seed);
   lisa_status rc = upload_weights(c, w);
@@ -756,9 +895,9 @@ void lisa_cache_destroy(lisa_cache* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (void* p : {(void*)c->w, (void*)c->m, (void*)c->v, (void*)c->g, (void*)c->wt,
-                  (void*)c->wb, (void*)c->act, (void*)c->dz, (void*)c->feat, (void*)c->tgt,
-                  (void*)c->out})
+  for (void* p : {(void*)c->w, (void*)c->m, (void*)c->v, (void*)c->g, (void*)c->part,
+                  (void*)c->loss_part, (void*)c->wt, (void*)c->wb, (void*)c->act, (void*)c->dz,
+                  (void*)c->xs, (void*)c->feat, (void*)c->tgt, (void*)c->out})
     (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -815,9 +954,14 @@ lisa_status lisa_cache_gradients(lisa_cache* c, const float* feat, const float* 
   if (rc != LISA_OK) return rc;
   CHIP(c, hipMemcpy(c->feat, feat, 4 * kInputs * (size_t)n, hipMemcpyHostToDevice));
   CHIP(c, hipMemcpy(c->tgt, target, 4 * kOut * (size_t)n, hipMemcpyHostToDevice));
-  k_zero_f32<<<64, 256, 0, c->stream>>>(c->g, kParams + kLossSlots);
-  rc = grads_device(c, c->feat, kInputs, c->tgt, n, c->loss);
+  rc = encode_device(c, c->feat, kInputs, n);
   if (rc != LISA_OK) return rc;
+  int ksplit = 0, n_loss = 0;
+  rc = grads_step(c, c->tgt, n, c->loss_part, &ksplit, &n_loss);
+  if (rc != LISA_OK) return rc;
+  k_mlp_reduce<<<kParams / kParamsPerBlock, 256, 0, c->stream>>>(c->part, ksplit, c->loss_part, n_loss,
+                                                             c->g, c->loss);
+  CHIP(c, hipGetLastError());
   CHIP(c, hipStreamSynchronize(c->stream));
   if (grad) CHIP(c, hipMemcpy(grad, c->g, 4 * kParams, hipMemcpyDeviceToHost));
   if (loss) CHIP(c, hipMemcpy(loss, c->loss, 4, hipMemcpyDeviceToHost));
@@ -836,17 +980,20 @@ lisa_status lisa::cache_train_device(lisa_cache* c, const float* d_feat, int fea
   if (steps > kLossSlots) return cfail(c, LISA_ERR_INVALID_ARG, "at most 64 training steps per call");
   CHIP(c, hipSetDevice(c->device));
   CHIP(c, hipEventRecord(c->ev0, c->stream));
-  // gradients and every step's loss slot from zero (each Adam step leaves
-  // the gradients zero again): three launches per step
-  k_zero_f32<<<64, 256, 0, c->stream>>>(c->g, kParams + kLossSlots);
+  // the encoding once, then three launches per step
+  lisa_status rc = encode_device(c, d_feat, feat_stride, n);
+  if (rc != LISA_OK) return rc;
   for (int k = 0; k < steps; ++k) {
-    lisa_status rc = grads_device(c, d_feat, feat_stride, d_tgt, n, c->loss + k);
+    float* lp = c->loss_part + (size_t)k * c->cus * kTrainWaves;
+    int ksplit = 0, n_loss = 0;
+    rc = grads_step(c, d_tgt, n, lp, &ksplit, &n_loss);
     if (rc != LISA_OK) return rc;
     ++c->step;
     const float c1 = 1.0f - std::pow(c->b1, (float)c->step),
                 c2 = 1.0f - std::pow(c->b2, (float)c->step);
-    k_mlp_adam<<<(kParams + 255) / 256, 256, 0, c->stream>>>(c->w, c->m, c->v, c->g, c->lr, c->b1,
-                                                            c->b2, c->eps, c1, c2, c->wt, c->wb);
+    k_mlp_adam<<<kParams / kParamsPerBlock, 256, 0, c->stream>>>(
+        c->w, c->m, c->v, c->part, ksplit, lp, n_loss, c->loss + k, c->lr, c->b1, c->b2, c->eps, c1,
+        c2, c->wt, c->wb);
     CHIP(c, hipGetLastError());
   }
   CHIP(c, hipEventRecord(c->ev1, c->stream));
