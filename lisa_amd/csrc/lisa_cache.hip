@@ -81,18 +81,6 @@ __host__ __device__ constexpr int wb_lds_offset(int l) {
 }
 constexpr int kWbLds = wb_lds_offset(kLayers);  // 16384 elements
 
-// Frequency encoding of one input: (sin, cos)(pi 2^b p) for b = 0..5 into
-// e[2 b], e[2 b + 1] (input i's features are 12 i + 2 b + {0, 1}).
-__device__ inline void encode_one(float p, uint16_t* e) {
-#pragma unroll
-  for (int b = 0; b < kBands; ++b) {
-    float s, c;
-    sincospif(p * (float)(1 << b), &s, &c);
-    e[2 * b] = f2bf(s);
-    e[2 * b + 1] = f2bf(c);
-  }
-}
-
 // Copies the forward weights W^T (global, row stride in_pad) and the
 // backward copies W (layers 1..4) into LDS (row strides + 8).  The 16-byte
 // chunks are numbered over both arrays; a thread of the kThreads-thread block
@@ -202,6 +190,24 @@ __device__ inline void sincospi_unit(float x, float& s, float& c) {
   c = sign * pc;
 }
 
+// Frequency encoding of one input p: e[2 b], e[2 b + 1] = (sin, cos)(pi 2^b p)
+// for b = 0..5 (input i's features are 12 i + 2 b + {0, 1}): one sincospi,
+// then the double-angle recurrence sin 2x = 2 sin x cos x,
+// cos 2x = cos^2 x - sin^2 x.  Training and inference both encode with it.
+__device__ inline void encode_input(float p, float* e) {
+  float sn, cs;
+  sincospi_unit(p, sn, cs);
+#pragma unroll
+  for (int b = 0; b < kBands; ++b) {
+    e[2 * b] = sn;
+    e[2 * b + 1] = cs;
+    const float s2 = 2.0f * sn * cs;
+    const float c2 = cs * cs - sn * sn;
+    sn = s2;
+    cs = c2;
+  }
+}
+
 // ReLU as a signed integer max on the float's bits (v_max_i32 with 0): a
 // float with its sign bit set becomes +0, others are kept; one instruction,
 // where fmaxf under IEEE mode needs a quieting step as well.
@@ -257,23 +263,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
   for (int64_t t = first; t < tiles; t += stride) {
     float pn[3] = {0.0f, 0.0f, 0.0f};
     if (t + stride < tiles) load_inputs(t + stride, pn);
-    // frequency encoding: one sincospi per input, then the double-angle
-    // recurrence sin 2x = 2 sin x cos x, cos 2x = cos^2 x - sin^2 x
+    // frequency encoding of the lane's three inputs
     float e[40];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float sn, cs;
-      sincospi_unit(p[k], sn, cs);
-#pragma unroll
-      for (int b = 0; b < kBands; ++b) {
-        e[12 * k + 2 * b] = sn;
-        e[12 * k + 2 * b + 1] = cs;
-        const float s2 = 2.0f * sn * cs;
-        const float c2 = cs * cs - sn * sn;
-        sn = s2;
-        cs = c2;
-      }
-    }
+    for (int k = 0; k < 3; ++k) encode_input(p[k], e + 12 * k);
     e[36] = e[37] = e[38] = e[39] = 0.0f;
     u32x4 x[5];
 #pragma unroll
@@ -347,11 +340,12 @@ __global__ __launch_bounds__(kEncRows * kInputs) void k_mlp_encode(
   const int64_t m = (int64_t)blockIdx.x * kEncRows + threadIdx.x % kEncRows;
   if (m >= npad) return;
   const float p = m < n ? feat[m * feat_stride + i] : 0.0f;
-  uint16_t row[2 * kBands];
-  encode_one(p, row);
+  float e[2 * kBands];
+  encode_input(p, e);
   for (int j = 0; j < 2 * kBands; ++j) {
-    xs[m * kEncPad + 2 * kBands * i + j] = row[j];
-    xf[(int64_t)(2 * kBands * i + j) * npad + m] = row[j];
+    const uint16_t h = bf16_rne(e[j]);
+    xs[m * kEncPad + 2 * kBands * i + j] = h;
+    xf[(int64_t)(2 * kBands * i + j) * npad + m] = h;
   }
   if (i == 0) {
     for (int f = kEnc; f < kEncPad; ++f) {
