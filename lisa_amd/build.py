@@ -64,7 +64,7 @@ def build(verbose_resources=False, defines=(), out_dir=OUT, cli=True):
 
 # Variant of the library with a 2-entry LDS traversal stack: every deeper
 # entry goes to the per-lane private overflow array, so a parity test run
-# against it exercises that path (the default 22-entry stack never
+# against it exercises that path (the default 20-entry stack never
 # overflows on the benchmark scenes).
 SMALL_STACK = os.path.join(OUT, "stack2")
 

@@ -65,10 +65,10 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #define LISA_MODE_WAVES 5  // the cache-mode path kernels
 #endif
 #ifndef LISA_EMIT_WAVES
-#define LISA_EMIT_WAVES 4  // the query kernel with the prefix-emission light samples
+#define LISA_EMIT_WAVES 5  // the query kernel with the prefix-emission light samples
 #endif
 #ifndef LISA_TILE_WAVES
-#define LISA_TILE_WAVES 5  // the batched wave-tile form
+#define LISA_TILE_WAVES 6  // the batched wave-tile form
 #endif
 // The BSDF of opaque materials, chosen at compile time as the reference
 // chooses it with an #include (shader.cu:4): a type with
