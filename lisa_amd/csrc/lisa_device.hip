@@ -3079,6 +3079,7 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
                   "FP64 subframe > 0 needs the FP64 sums of subframe 0 (or lisa_accum64_load)");
     HIP_TRY(ctx, ctx->c_sum64.ensure(npx));
   }
+  ctx->sum64_valid = false;  // set again when an FP64 render succeeds
   PathArgs a = make_args(p);
   a.tiles_x = (p->width + kTileW - 1) / kTileW;
   const uint32_t tiles_y = (p->height + kTileH - 1) / kTileH;
@@ -3484,6 +3485,10 @@ lisa_status lisa_batch_render(lisa_batch* b, float* elapsed_ms) {
   }
   if (b->cfg.form == 2 && (e = b->wf.ensure(wf_batch_slots(b))) != hipSuccess)
     return fail(nullptr, LISA_ERR_ALLOC, std::string("wavefront buffers: ") + hipGetErrorString(e));
+  for (int32_t i = 0; i < b->n; ++i) {  // each accum buffer will hold a plain render
+    b->ctxs[i]->j_seq = false;
+    b->ctxs[i]->sum64_valid = false;
+  }
   if ((e = hipEventRecord(b->ev0, b->stream)) != hipSuccess)
     return fail(nullptr, LISA_ERR_HIP, std::string("lisa_batch_render: ") + hipGetErrorString(e));
   if (b->exec)
@@ -3606,6 +3611,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
     rc = ensure_accum(ctx, npx);
     if (rc != LISA_OK) return rc;
   }
+  ctx->sum64_valid = false;  // the accum buffer will hold cached frames
   if (p->subframe == 0) {  // a new sequence: the judge starts over
     ctx->j_seq = false;  // set again when this frame succeeds
     ctx->j_nj = cp->judge_pixels;

@@ -384,6 +384,43 @@ def test_batched_wavefront_bit_exact(capi, oracle_mod):
         c.close()
 
 
+def test_batch_replay_ends_fp64_and_cached_sequences(capi):
+    """A batch replay overwrites each context's accum buffer: the FP64 sums
+    of an earlier LISA_FLAG_FP64_ACCUM render no longer describe it (saving
+    them and an FP64 subframe 1 are refused), and a cached sequence started
+    on the context cannot go on (its frame 1 is refused)."""
+    W, H, spp, B = 32, 24, 2, 3
+    sc = cornell(16, 8, W, H, spp, B)
+    c = capi.Context(0)
+    cache = capi.Cache(0, seed=1, lr=1e-3)
+    try:
+        c.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        U, V, Wv = camera(sc, capi)
+        eye = sc["camera"]["eye"]
+        params = lambda sub=0, flags=0: capi.render_params(W, H, spp, B, eye, U, V, Wv,  # noqa: E731
+                                                           subframe=sub, flags=flags)
+        c.render(params(flags=capi.FLAG_FP64_ACCUM))
+        assert c.accum64_save(W * H).shape == (W * H, 4)
+        b = capi.Batch([c], [params()])
+        try:
+            b.render()
+            with pytest.raises(capi.LisaError, match="no FP64 sums"):
+                c.accum64_save(W * H)
+            with pytest.raises(capi.LisaError, match="FP64 sums"):
+                c.render(params(sub=1, flags=capi.FLAG_FP64_ACCUM))
+            cp = capi.cache_frame_params(prefix_bounces=1, train_paths=256, train_bounces=B,
+                                         train_max=256)
+            capi.render_cached(c, cache, params(sub=0), cp)
+            b.render()
+            with pytest.raises(capi.LisaError, match="no cached frame 0"):
+                capi.render_cached(c, cache, params(sub=1), cp)
+        finally:
+            b.close()
+    finally:
+        cache.close()
+        c.close()
+
+
 def test_converged_within_3_sigma(capi, oracle_mod, ctx):
     """Independent seeds: the GPU's 256-spp pixel means and the oracle's
     (another subframe, so another RNG stream) agree within the per-pixel
