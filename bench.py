@@ -68,6 +68,7 @@ def parse():
                     help="training paths per frame (their rows at the query depth: ~2^16 on "
                          "the island)")
     ap.add_argument("--train-steps", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3, help="the cache's Adam learning rate")
     ap.add_argument("--rmse-limit-s", type=float, default=30.0,
                     help="give up a method after this much render time")
     ap.add_argument("--persistent", action="store_true",
@@ -363,7 +364,7 @@ def bench_rmse(a, workload=None, emit=True):
     # cache
     def run_cache(flags, frame_spp=None):
         frame_spp = frame_spp or a.cache_spp
-        cache = capi.Cache(0, seed=1, lr=1e-3)
+        cache = capi.Cache(0, seed=1, lr=a.lr)
         cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                      train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
                                      seed=3, flags=flags,
@@ -416,7 +417,7 @@ def bench_rmse(a, workload=None, emit=True):
            "config": {"workload": f"{sc['name']} learned cache vs plain", "width": W, "height": H,
                       "bounces": B, "triangles": ctx.T, "cache_prefix_bounces": a.prefix,
                       "cache_spp_per_frame": a.cache_spp, "train_paths": a.train_paths,
-                      "train_rows_max": 1 << 16, "train_steps": a.train_steps, "lr": 1e-3,
+                      "train_rows_max": 1 << 16, "train_steps": a.train_steps, "lr": a.lr,
                       "persistent_frames": a.cache_persistent,
                       "cache_emission": a.cache_emission,
                       "nee_loops": a.nee_loops if a.cache_emission else 1,
@@ -484,7 +485,7 @@ def bench_cache_frames(a, capi, ctx, sc, W, H, B, U, V, Wv):
     """`--cache-frames N`: N cached frames after `--warmup` untimed ones; per-frame
     wall time and the MLP's MFMA roofline (kernel times from HIP events)."""
     cam = sc["camera"]
-    cache = capi.Cache(0, seed=1, lr=1e-3)
+    cache = capi.Cache(0, seed=1, lr=a.lr)
     serial = not a.cache_overlap  # by default the training kernels are timed alone here
     cp = capi.cache_frame_params(prefix_bounces=a.prefix, train_paths=a.train_paths,
                                  train_bounces=B, train_max=1 << 16, train_steps=a.train_steps,
