@@ -134,6 +134,18 @@ lisa_status lisa_update_vertices_noalloc(lisa_ctx* ctx, const float* d_verts, co
 lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p,
                         float* accum_rgba_out, uint8_t* rgba8_out);
 
+/* Progressive subframes p->subframe .. p->subframe + n_subframes - 1 in one
+ * hipGraph launch: the chain of n_subframes plain launches is built on the
+ * first call for an image size and n (and for FP64 or not) and replayed by
+ * later calls, with each launch's subframe index and buffers set on the
+ * instantiated graph.  The accum buffer ends as after n_subframes lisa_render
+ * calls with those subframes, bit for bit: the running-mean lerp of
+ * src/LiSA/src/shader.cu:160-164, or with LISA_FLAG_FP64_ACCUM (the only flag
+ * taken) the FP64 sums.  The reference's display loop (render.cc:75-131)
+ * launches one subframe per frame; this is the same sequence for callers that
+ * refine an image offline, with one host call and one wait per run. */
+lisa_status lisa_render_progressive(lisa_ctx* ctx, const lisa_render_params* p, int32_t n_subframes);
+
 /* Copies the device accum buffer (float4 per pixel, n_pixels = w*h of the
  * last render) to rgba_out.  No counterpart in the reference, whose
  * frame_buffer is mapped host memory (CUDAOutputBuffer ZERO_COPY). */

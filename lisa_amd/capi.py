@@ -38,7 +38,7 @@ COUNTER_NAMES = ("rays_radiance", "rays_occlusion", "node_visits", "tri_tests",
 EXPORTS = (
     "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_update_vertices",
     "lisa_update_vertices_noalloc",
-    "lisa_render",
+    "lisa_render", "lisa_render_progressive",
     "lisa_copy_accum", "lisa_accum64_save", "lisa_accum64_load", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
     "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset",
     "lisa_batch_create", "lisa_batch_destroy", "lisa_batch_set_config", "lisa_batch_get_config",
@@ -111,6 +111,7 @@ def lib():
         "lisa_update_vertices": (st, [vp, f32p, f32p]),
         "lisa_update_vertices_noalloc": (st, [vp, vp, vp]),
         "lisa_render": (st, [vp, C.POINTER(RenderParams), f32p, u8p]),
+        "lisa_render_progressive": (st, [vp, C.POINTER(RenderParams), C.c_int32]),
         "lisa_copy_accum": (st, [vp, f32p, C.c_int64]),
         "lisa_accum64_save": (st, [vp, C.POINTER(C.c_double), C.c_int64]),
         "lisa_accum64_load": (st, [vp, C.POINTER(C.c_double), C.c_int64]),
@@ -311,6 +312,12 @@ class Context:
         self._ck(lib().lisa_render(self.h, C.byref(params), _p(acc, C.c_float),
                                    _p(rgb, C.c_uint8)))
         return acc, rgb
+
+    def render_progressive(self, params, n_subframes):
+        """Subframes params.subframe .. + n_subframes - 1 in one graph launch
+        (lisa_render_progressive); returns the accum buffer."""
+        self._ck(lib().lisa_render_progressive(self.h, C.byref(params), n_subframes))
+        return self.copy_accum(params.width * params.height)
 
     def copy_accum(self, n_pixels):
         acc = np.zeros((n_pixels, 4), np.float32)

@@ -2286,6 +2286,26 @@ hipError_t wf_pending(hipStream_t st, const WfState& ws, uint32_t* pending) {
   return e;
 }
 
+// lisa_render_progressive: n launches of the plain megakernel, one per
+// subframe, chained in one instantiated hipGraph; before each launch the
+// nodes' arguments (subframe index, buffers) are set on the executable graph,
+// so one graph serves every run of n subframes at one image size.
+struct ProgGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphNode_t> nodes;
+  uint32_t grid = 0;
+  bool f64 = false;
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    nodes.clear();
+    grid = 0;
+  }
+};
+
 struct lisa_ctx {
   int32_t device = 0;
   int32_t cus = 256;  // compute units (persistent grids)
@@ -2332,6 +2352,7 @@ struct lisa_ctx {
   DevBuf<float4> c_jout, c_jacc;
   DevBuf<float> c_jrgb;
   WfBuffers wf;  // the wavefront form of lisa_render (LISA_FLAG_WAVEFRONT)
+  ProgGraph pg;  // lisa_render_progressive's chain of subframe launches
   // FP64 progressive sums (LISA_FLAG_FP64_ACCUM full renders): r, g, b sums
   // and the sample count per pixel; valid while they belong to the accum
   // buffer's current sequence (an FP64 subframe 0, or lisa_accum64_load)
@@ -2983,6 +3004,7 @@ void lisa_destroy(lisa_ctx* c) {
   c->c_jrgb.release();
   c->wf.release();
   c->c_sum64.release();
+  c->pg.release();
   free_scene(c);
   (void)hipFree(c->d_accum);
   (void)hipFree(c->d_counters);
@@ -3129,6 +3151,98 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
     HIP_TRY(ctx, hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
     for (int i = 0; i < LISA_NCOUNTERS; ++i) ctx->counters[i] = h[i];
   }
+  return LISA_OK;
+}
+
+lisa_status lisa_render_progressive(lisa_ctx* ctx, const lisa_render_params* p, int32_t n_subframes) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  lisa_status rc = check_params(ctx, p);
+  if (rc != LISA_OK) return rc;
+  if (n_subframes <= 0 || n_subframes > 4096)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "n_subframes must be in 1..4096");
+  if ((uint64_t)p->subframe + (uint64_t)n_subframes > 0xFFFFFFFFull)
+    return fail(ctx, LISA_ERR_INVALID_ARG, "subframe + n_subframes overflows the subframe index");
+  if (p->flags & ~LISA_FLAG_FP64_ACCUM)
+    return fail(ctx, LISA_ERR_UNSUPPORTED, "progressive graphs take no flag but LISA_FLAG_FP64_ACCUM");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t npx = (size_t)p->width * p->height;
+  if (ctx->accum_px != npx) {
+    if (p->subframe > 0)
+      return fail(ctx, LISA_ERR_INVALID_ARG, "subframe > 0 but the accum buffer has another size");
+    rc = ensure_accum(ctx, npx);
+    if (rc != LISA_OK) return rc;
+  }
+  ctx->j_seq = false;  // the buffer now holds a plain render
+  const bool f64 = (p->flags & LISA_FLAG_FP64_ACCUM) != 0;
+  if (f64) {
+    if (p->subframe > 0 && !ctx->sum64_valid)
+      return fail(ctx, LISA_ERR_INVALID_ARG,
+                  "FP64 subframe > 0 needs the FP64 sums of subframe 0 (or lisa_accum64_load)");
+    HIP_TRY(ctx, ctx->c_sum64.ensure(npx));
+  }
+  ctx->sum64_valid = false;  // set again when an FP64 run succeeds
+  PathArgs a = make_args(p);
+  a.tiles_x = (p->width + kTileW - 1) / kTileW;
+  const uint32_t grid = a.tiles_x * ((p->height + kTileH - 1) / kTileH);
+  a.n_items = (int32_t)npx;
+  a.accum = ctx->d_accum;
+  a.accum_prev = ctx->d_accum;
+  a.counters = ctx->d_counters;
+  a.sum64 = f64 ? ctx->c_sum64.p : nullptr;
+  SceneView sv = view(ctx);
+  void* const fn = f64 ? reinterpret_cast<void*>(&k_path<false, false, true>)
+                       : reinterpret_cast<void*>(&k_path<false, false, false>);
+  auto node_params = [&](PathArgs& pa, void** args) {
+    hipKernelNodeParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    kp.func = fn;
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(kPathBlock);
+    kp.sharedMemBytes = 0;
+    args[0] = &sv;
+    args[1] = &pa;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    return kp;
+  };
+  ProgGraph& g = ctx->pg;
+  hipError_t e = hipSuccess;
+  if (!g.exec || g.nodes.size() != (size_t)n_subframes || g.grid != grid || g.f64 != f64) {
+    g.release();
+    e = hipGraphCreate(&g.graph, 0);
+    for (int32_t i = 0; e == hipSuccess && i < n_subframes; ++i) {
+      void* args[2];
+      hipKernelNodeParams kp = node_params(a, args);
+      hipGraphNode_t node = nullptr;
+      const hipGraphNode_t prev = i > 0 ? g.nodes.back() : nullptr;
+      e = hipGraphAddKernelNode(&node, g.graph, i > 0 ? &prev : nullptr, i > 0 ? 1 : 0, &kp);
+      if (e == hipSuccess) g.nodes.push_back(node);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      g.release();
+      return fail(ctx, LISA_ERR_HIP, std::string("progressive graph: ") + hipGetErrorString(e));
+    }
+    g.grid = grid;
+    g.f64 = f64;
+  }
+  for (int32_t i = 0; e == hipSuccess && i < n_subframes; ++i) {  // subframe p->subframe + i
+    PathArgs ai = a;
+    ai.subframe = p->subframe + (uint32_t)i;
+    void* args[2];
+    hipKernelNodeParams kp = node_params(ai, args);
+    e = hipGraphExecKernelNodeSetParams(g.exec, g.nodes[(size_t)i], &kp);
+  }
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+  if (e == hipSuccess) e = hipGraphLaunch(g.exec, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    g.release();
+    return fail(ctx, LISA_ERR_HIP, std::string("lisa_render_progressive: ") + hipGetErrorString(e));
+  }
+  HIP_TRY(ctx, hipEventElapsedTime(&ctx->render_ms, ctx->ev0, ctx->ev1));
+  ctx->sum64_valid = f64;
   return LISA_OK;
 }
 

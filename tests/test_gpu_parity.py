@@ -384,6 +384,46 @@ def test_batched_wavefront_bit_exact(capi, oracle_mod):
         c.close()
 
 
+def test_progressive_graph_same_images(capi, ctx):
+    """lisa_render_progressive: runs of subframes as one graph launch leave
+    the accum buffer bit-identical to the same subframes rendered one
+    lisa_render call at a time -- the FP32 running mean and the FP64 sums,
+    across runs of different lengths (the graph rebuilt), a run at another
+    image size and a repeated run (the graph replayed with new subframes)."""
+    W, H, spp, B = 40, 24, 3, 5
+    sc = cornell(16, 8, W, H, spp, B)
+    ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+    U, V, Wv = camera(sc, capi)
+    eye = sc["camera"]["eye"]
+    other = capi.Context(0)
+    try:
+        other.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        for flags in (0, capi.FLAG_FP64_ACCUM):
+            par = lambda sub, w=W, h=H: capi.render_params(w, h, spp, B, eye, U, V, Wv,  # noqa: E731
+                                                           subframe=sub, flags=flags)
+            seq = [ctx.render(par(k))[0] for k in range(9)]
+            runs = [(0, 3), (3, 3), (6, 2), (8, 1)]
+            for first, n in runs:
+                got = other.render_progressive(par(first), n)
+                want = seq[first + n - 1]
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (flags, first, n)
+            # another size (a new graph), then the first size again from subframe 0
+            small = other.render_progressive(par(0, 16, 8), 2)
+            ctx.render(par(0, 16, 8))
+            assert np.array_equal(small.view(np.uint32), ctx.render(par(1, 16, 8))[0].view(np.uint32))
+            got = other.render_progressive(par(0), 3)
+            assert np.array_equal(got.view(np.uint32), seq[2].view(np.uint32))
+        with pytest.raises(capi.LisaError, match="n_subframes"):
+            other.render_progressive(capi.render_params(W, H, spp, B, eye, U, V, Wv), 0)
+        with pytest.raises(capi.LisaError, match="no flag but"):
+            other.render_progressive(capi.render_params(W, H, spp, B, eye, U, V, Wv,
+                                                        flags=capi.FLAG_COUNT), 2)
+        with pytest.raises(capi.LisaError, match="another size"):
+            other.render_progressive(capi.render_params(8, 8, spp, B, eye, U, V, Wv, subframe=1), 2)
+    finally:
+        other.close()
+
+
 def test_batch_replay_ends_fp64_and_cached_sequences(capi):
     """A batch replay overwrites each context's accum buffer: the FP64 sums
     of an earlier LISA_FLAG_FP64_ACCUM render no longer describe it (saving
