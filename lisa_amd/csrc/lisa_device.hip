@@ -1133,6 +1133,20 @@ __global__ __launch_bounds__(256) void k_mean64(const double4* __restrict__ sum6
   accum[i] = make_float4((float)(v.x * inv), (float)(v.y * inv), (float)(v.z * inv), 1.0f);
 }
 
+// The training paths' pixels: item i of the frame's splitmix64 stream from
+// sm0 (the state before the first step), modulo the pixel count -- the same
+// values as the sequential generator, computed independently per item.
+__global__ __launch_bounds__(256) void k_train_pixels(int32_t* __restrict__ px, uint32_t n, uint64_t sm0,
+                                                      uint32_t npx) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = sm0 + (uint64_t)(i + 1u) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  px[i] = (int32_t)(z % npx);
+}
+
 __global__ __launch_bounds__(256) void k_blend(float4* accum, const float4* __restrict__ frame,
                                                int64_t n, float a, int first) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -2310,6 +2324,7 @@ struct lisa_ctx {
   int32_t device = 0;
   int32_t cus = 256;  // compute units (persistent grids)
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // cached frames: the training paths beside the frame's (made on first use)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   int32_t T = 0, M = 0;
@@ -2777,8 +2792,7 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
 // hits of radiance ray number `depth` > 0 only, into ctx->c_tfeat / c_ttgt, at
 // most `cap`; *rows = rows produced (may exceed cap).
 lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, size_t cap,
-                             uint32_t depth, uint32_t* rows) {
-  hipStream_t st = ctx->stream;
+                             uint32_t depth, uint32_t* rows, hipStream_t st) {
   // vertex slots per path: one hit per sample at a fixed depth, else up to one per bounce
   const size_t slots = depth > 0 ? 1 : a.bounces;
   HIP_TRY(ctx, ctx->c_trec.ensure(n * slots * 18));
@@ -3011,6 +3025,7 @@ void lisa_destroy(lisa_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete c;
 }
 
@@ -3772,34 +3787,32 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   // 1. online training on suffixes of full-length paths traced by the plain loop
   if (cp->train_paths > 0 && cp->train_steps > 0 && !taken_over && !full) {
     const auto t0 = std::chrono::steady_clock::now();
+    // With the overlap the frame's prefix paths start first, on the
+    // context's stream, and the training paths run beside them on a second
+    // stream (the two write disjoint buffers; the scene is read-only).
+    hipStream_t ts = st;
+    if (overlap) {
+      if (!ctx->stream2) HIP_TRY(ctx, hipStreamCreate(&ctx->stream2));
+      ts = ctx->stream2;
+      rc = trace_queries_launch(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, persistent);
+      if (rc != LISA_OK) return rc;
+      launched = true;
+    }
     const size_t nt = cp->train_paths;
     const size_t cap = cp->train_max > 0 ? cp->train_max : nt * cp->train_bounces;
     HIP_TRY(ctx, ctx->c_tpix.ensure(nt));
-    std::vector<int32_t> px(nt);
-    uint64_t sm = cp->seed * 0x9E3779B97F4A7C15ull + p->subframe + 1;
-    for (size_t i = 0; i < nt; ++i) {  // splitmix64
-      sm += 0x9E3779B97F4A7C15ull;
-      uint64_t z = sm;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-      z ^= z >> 31;
-      px[i] = (int32_t)(z % npx);
-    }
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->c_tpix.p, px.data(), 4 * nt, hipMemcpyHostToDevice, st));
+    k_train_pixels<<<grid_for((int64_t)nt, 256), 256, 0, ts>>>(
+        ctx->c_tpix.p, (uint32_t)nt, cp->seed * 0x9E3779B97F4A7C15ull + p->subframe + 1, (uint32_t)npx);
+    HIP_TRY(ctx, hipGetLastError());
     PathArgs a = base;
     a.spp = 1;
     a.bounces = cp->train_bounces;
     a.subframe = 0x40000000u ^ p->subframe;  // an RNG stream apart from the rendered samples
     uint32_t rows = 0;  // rows at the query depth: targets with the queries' remaining bounces
-    rc = trace_train_rows(ctx, a, ctx->c_tpix.p, nt, cap, cp->prefix_bounces + 1, &rows);
+    rc = trace_train_rows(ctx, a, ctx->c_tpix.p, nt, cap, cp->prefix_bounces + 1, &rows, ts);
     if (rc != LISA_OK) return rc;
     rows = std::min<uint32_t>(rows, (uint32_t)cap);
     const auto t1 = std::chrono::steady_clock::now();
-    if (overlap) {
-      rc = trace_queries_launch(ctx, qa, nullptr, npx, cp->prefix_bounces, ctx->c_frame.p, persistent);
-      if (rc != LISA_OK) return rc;
-      launched = true;
-    }
     if (rows > 0) {
       lisa_status r2 = lisa::cache_train_device(cache, ctx->c_tfeat.p, 12, ctx->c_ttgt.p, (int64_t)rows,
                                                 &loss, (int)cp->train_steps);
@@ -4002,7 +4015,7 @@ lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, co
   if (rc != LISA_OK) return rc;
   PathArgs a = cache_args(ctx, p);
   uint32_t rows = 0;
-  rc = trace_train_rows(ctx, a, ctx->c_tpix.p, (size_t)n, (size_t)cap, depth, &rows);
+  rc = trace_train_rows(ctx, a, ctx->c_tpix.p, (size_t)n, (size_t)cap, depth, &rows, ctx->stream);
   if (rc != LISA_OK) return rc;
   const size_t k = std::min<size_t>(rows, (size_t)cap);
   if (k > 0) {
