@@ -251,7 +251,7 @@ float lisa_cache_last_ms(const lisa_cache* c);
  *     mean with weight 1 / (p->subframe + 1) (frame p->subframe). */
 typedef struct lisa_cache_frame_params {
   uint32_t prefix_bounces; /* radiance rays traced before the cache answers (0: at the camera ray's hit) */
-  uint32_t train_paths;    /* training paths per frame (0: no training) */
+  uint32_t train_paths;    /* training paths per frame (0: no training; at most 2^24) */
   uint32_t train_bounces;  /* bounces of a training path */
   uint32_t train_max;      /* at most this many training rows (suffixes) per frame; 0: no cap */
   uint32_t train_steps;    /* Adam steps per frame over the frame's rows (at most 64) */

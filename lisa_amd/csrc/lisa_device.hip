@@ -3713,6 +3713,7 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
   PathArgs pre{};  // only emit_expect, nee_loops and nee_loops_deep are read
   rc = prefix_args(ctx, cp, pre);
   if (rc != LISA_OK) return rc;
+  if (cp->train_paths > (1u << 24)) return fail(ctx, LISA_ERR_INVALID_ARG, "train_paths must be at most 2^24");
   if (cp->train_paths > 0 && (cp->train_bounces == 0 || cp->train_bounces > 64))
     return fail(ctx, LISA_ERR_INVALID_ARG, "train_bounces must be in 1..64");
   if (cp->train_steps > 64) return fail(ctx, LISA_ERR_INVALID_ARG, "train_steps must be at most 64");

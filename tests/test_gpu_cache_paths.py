@@ -351,6 +351,9 @@ def test_cached_sequence_must_continue_frame0(capi, oracle_mod, ctx):
             capi.render_cached(ctx, cache, p(1), cp)
         capi.render_cached(ctx, cache, p(0), none)  # a new sequence without the judge
         capi.render_cached(ctx, cache, p(1), none)
+        huge = capi.cache_frame_params(prefix_bounces=2, train_paths=(1 << 24) + 1, train_steps=1)
+        with pytest.raises(capi.LisaError, match="train_paths"):
+            capi.render_cached(ctx, cache, p(2), huge)
     finally:
         cache.close()
 
