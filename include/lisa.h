@@ -161,7 +161,12 @@ lisa_status lisa_copy_accum(lisa_ctx* ctx, float* rgba_out, int64_t n_pixels);
 lisa_status lisa_accum64_save(lisa_ctx* ctx, double* sums_out, int64_t n_pixels);
 lisa_status lisa_accum64_load(lisa_ctx* ctx, const double* sums, int64_t n_pixels);
 
+/* Waits for the context's stream (the reference's CUDA_SYNC_CHECK after a
+ * launch, render.cc:57,143; every call here already waits before it returns). */
 lisa_status lisa_synchronize(lisa_ctx* ctx);
+/* Scene size, BVH build time and the last render's kernel time (HIP events).
+ * No counterpart in the reference, which prints the render's wall time only
+ * (render.cc:146-147). */
 lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out);
 /* Counters of the last call made with LISA_FLAG_COUNT (or lisa_trace_rays). */
 lisa_status lisa_get_counters(const lisa_ctx* ctx, uint64_t* counters /* LISA_NCOUNTERS */);
