@@ -108,19 +108,19 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profile(workload, spp):
-    """HBM bytes per launch of the timed kernel measured by two rocprofv3 --pmc
-    runs (FETCH_SIZE, WRITE_SIZE; scaled by the copy-kernel calibration) and
-    recorded in profiles/k_path_<workload>_pmc.json for the same spp, or None."""
+def profile_record(workload, spp):
+    """The plain megakernel's rocprofv3 figures recorded in
+    profiles/k_path_<workload>_pmc.json for the same spp, or {}:
+    hbm_bytes_per_launch (two --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled
+    by the copy-kernel calibration) and valu_lane_utilization
+    (SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU))."""
     path = os.path.join(ROOT, "profiles", f"k_path_{workload}_pmc.json")
     try:
         with open(path) as fh:
             rec = json.load(fh)
-        if int(rec["spp"]) != spp:
-            return None
-        return float(rec["hbm_bytes_per_launch"])
+        return rec if int(rec["spp"]) == spp else {}
     except (OSError, KeyError, ValueError):
-        return None
+        return {}
 
 
 def bench_plain(a, workload, steps, warmup, cpu=True):
@@ -169,13 +169,16 @@ def bench_plain(a, workload, steps, warmup, cpu=True):
     alg_bytes = (S_NODE * counts["node_visits"] + S_TRI * counts["tri_tests"]
                  + S_SHADE * counts["radiance_hits"] + S_PIXEL * W * H)
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile(workload, spp)
+    rec = {} if (a.persistent or a.wavefront) else profile_record(workload, spp)
+    traffic = rec.get("hbm_bytes_per_launch")
     resident = workload in ("cornell", "cornell65k")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic,
                 "dram_frac": (None if traffic is None else
                               round(traffic / (avg_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+                # fraction of a wave's 64 lanes active per VALU instruction
+                "lane_utilization": rec.get("valu_lane_utilization"),
                 "limiter": ("instruction issue under divergence; the scene is L2 / Infinity "
                             "Cache resident, so `frac` counts bytes served mostly from cache"
                             if resident else
