@@ -328,14 +328,16 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
  * radiance rays; the surface hit of radiance ray prefix_bounces + 1 ends the
  * path in a query (features,
  * throughput, index into `pixels`).  accum_out (4n) = per-pixel mean of the
- * radiance gathered before the queries.  Queries come in no fixed order;
- * *n_queries may exceed q_cap (the rest is dropped).
+ * radiance gathered before the queries.  Queries come in the order of
+ * `pixels`, a pixel's in sample order; *n_queries may exceed q_cap (only the
+ * first q_cap are copied).
  *
  * lisa_cache_trace_rows: plain paths (p->spp, p->bounces) on the listed
  * pixels; one row per surface hit (depth = 0) or per surface hit of radiance
  * ray number `depth`: features and the target (path radiance - radiance
- * gathered before the hit) / throughput on arrival.  Rows of one path are
- * contiguous, paths in no fixed order; *n_rows may exceed cap. */
+ * gathered before the hit) / throughput on arrival.  Rows come in the order
+ * of `pixels`, then sample, then depth along the path; *n_rows may exceed
+ * cap (only the first cap are copied). */
 lisa_status lisa_cache_trace_queries(lisa_ctx* ctx, const lisa_render_params* p,
                                      const lisa_cache_frame_params* cp, const int32_t* pixels, int32_t n,
                                      float* accum_out, float* q_feat, float* q_att, int32_t* q_item,

@@ -407,17 +407,16 @@ struct PathArgs {
   unsigned long long* counters;
   // learned radiance cache (path_lane's kMode 1 and 2)
   uint32_t cache_at;  // kMode 1: this radiance ray's surface hit queries the cache
-  float* q_feat;      // kMode 1: 12 features, 3 throughput, pixel item per query
-  float* q_att;
-  int32_t* q_item;
-  uint32_t* q_count;
-  uint32_t q_cap;
+  // kMode 1: one query slot per sample (item * spp + sample): 4 float4 (hit
+  // point and material index bits, incoming direction, shading normal,
+  // throughput) and a flag set to 1 when the sample queried (zeroed before
+  // the launch)
+  float4* q_rec;
+  uint8_t* q_has;
   float* t_rec;       // kMode 2: 18 floats per vertex slot (12 features, 3 throughput, 3 radiance so far)
   uint32_t t_slots;   // kMode 2: slots per training path
-  float* t_feat;      // kMode 2: compact training rows, 12 features each
-  float* t_tgt;       // kMode 2: 3 target floats per row
-  uint32_t* t_count;  // kMode 2: rows appended (may pass t_cap; rows past it are dropped)
-  uint32_t t_cap;
+  float* t_row;       // kMode 2: 15 floats (features, target) per row slot ((item * spp + sample) * t_slots + vertex)
+  uint8_t* t_n;       // kMode 2: rows of each sample (zeroed before the launch)
   uint32_t t_depth;   // kMode 2: > 0 records only the hits of radiance ray number t_depth
   int32_t emit_expect;      // queries: path_lane's kMode 3 instead of kMode 1
   int32_t nee_loops;        // kMode 3: NEE runs averaged at the camera ray's hit (>= 1)
@@ -476,8 +475,9 @@ struct Job {
 // again after every `src.next(j)`, which gives the lane its next pixel;
 // false: no work left).
 // kMode 0: the plain path loop.  kMode 1: the radiance ray numbered
-// p.cache_at ends at a surface hit by queuing a cache query (features, the
-// path throughput, the pixel); misses and emitters are handled exactly.
+// p.cache_at ends at a surface hit by writing a cache query (the hit, the
+// path throughput) at the sample's slot; misses and emitters are handled
+// exactly.
 // kMode 3: kMode 1 with the emission that the prefix's diffuse bounces would
 // hit taken in expectation (a light sample at each Lambertian vertex, the
 // bounce drawn on the condition that it hits no emitter) and each vertex's
@@ -487,7 +487,7 @@ struct Job {
 // the hit of radiance ray number t_depth only) records its features,
 // throughput and the radiance gathered so far; at the end of the sample the
 // suffix radiance (final - so far) / throughput of each vertex becomes a
-// training row.  Without accum_prev the launch writes its own mean (no
+// training row at the sample's row slots.  Without accum_prev the launch writes its own mean (no
 // running mean over subframes; the subframe then only picks the RNG stream).
 template <bool kRecord, bool kCount, class Src, int kMode = 0, bool kF64 = false,
           class Bsdf = LISA_BSDF>
@@ -544,18 +544,17 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   // accum_color += trace(...) (shader.cu:154) and the per-sample records.
   auto finish_sample = [&]() {
     if (kMode == 2 && t_used > 0) {
-      // one reservation per path; suffix radiance per unit throughput
-      const uint32_t row0 = atomicAdd(pp->t_count, t_used);
+      // the sample's rows at its own slots; suffix radiance per unit throughput
       const size_t base = (size_t)item * pp->t_slots;
-      for (uint32_t v = 0; v < t_used && row0 + v < pp->t_cap; ++v) {
+      float* row = pp->t_row + 15 * (rec_base + s) * pp->t_slots;
+      for (uint32_t v = 0; v < t_used; ++v, row += 15) {
         const float* r = pp->t_rec + 18 * (base + v);
-        float* f = pp->t_feat + 12 * (size_t)(row0 + v);
-        for (int q = 0; q < 12; ++q) f[q] = r[q];
-        float* tg = pp->t_tgt + 3 * (size_t)(row0 + v);
-        tg[0] = (color.x - r[15]) / r[12];
-        tg[1] = (color.y - r[16]) / r[13];
-        tg[2] = (color.z - r[17]) / r[14];
+        for (int q = 0; q < 12; ++q) row[q] = r[q];
+        row[12] = (color.x - r[15]) / r[12];
+        row[13] = (color.y - r[16]) / r[13];
+        row[14] = (color.z - r[17]) / r[14];
       }
+      pp->t_n[rec_base + s] = (uint8_t)t_used;
       t_used = 0;
     }
     if (kCount) c_smp++;
@@ -893,25 +892,14 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
           }
           bool cached = false;
           if (kQuery && b == pp->cache_at) {
-            const uint32_t slot = wave_reserve(pp->q_count);
-            if (slot < pp->q_cap) {
-              // the hit itself; k_cache_features turns it into the features
-              float* r = pp->q_feat + 12 * (size_t)slot;
-              r[0] = O.x;
-              r[1] = O.y;
-              r[2] = O.z;
-              r[3] = Din.x;
-              r[4] = Din.y;
-              r[5] = Din.z;
-              r[6] = N.x;
-              r[7] = N.y;
-              r[8] = N.z;
-              r[9] = __uint_as_float((uint32_t)mat);
-              pp->q_att[3 * (size_t)slot] = att.x;
-              pp->q_att[3 * (size_t)slot + 1] = att.y;
-              pp->q_att[3 * (size_t)slot + 2] = att.z;
-              pp->q_item[slot] = item;
-            }
+            // the hit itself at the sample's slot; k_compact_queries turns
+            // it into the features
+            float4* r = pp->q_rec + 4 * (rec_base + s);
+            r[0] = make_float4(O.x, O.y, O.z, __uint_as_float((uint32_t)mat));
+            r[1] = make_float4(Din.x, Din.y, Din.z, 0.0f);
+            r[2] = make_float4(N.x, N.y, N.z, 0.0f);
+            r[3] = make_float4(att.x, att.y, att.z, 0.0f);
+            pp->q_has[rec_base + s] = 1;
             cached = true;
             finish = true;
           }
@@ -1092,28 +1080,25 @@ __global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : 1) void 
   path_lane<false, false, PixelQueue, kMode>(src, stk + threadIdx.x);
 }
 
-// The features of the queries of a cached frame, in place: each 12-float
-// record holds the hit point, incoming direction, shading normal and the
-// material index as written by the path loop (path_lane's kMode 1).
-__global__ __launch_bounds__(256) void k_cache_features(float* __restrict__ q, uint32_t n, F3 bmin,
-                                                        F3 binv, const DevMat* __restrict__ mats) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  float* r = q + 12 * (size_t)i;
-  const F3 P = mk(r[0], r[1], r[2]), D = mk(r[3], r[4], r[5]), N = mk(r[6], r[7], r[8]);
-  const DevMat m = mats[__float_as_uint(r[9])];
-  cache_features(bmin, binv, P, D, N, m, r);
-}
-
 // frame[item] += throughput * max(network output, 0) / spp for every query.
+// The queries come in slot order, so those of one pixel are contiguous and in
+// sample order: the first of each run adds them all, in that order (no
+// atomics; the same sums on every run).
 __global__ __launch_bounds__(256) void k_cache_add(const float* __restrict__ pred,
                                                    const float* __restrict__ att,
                                                    const int32_t* __restrict__ item, uint32_t n,
                                                    float inv_spp, float4* frame) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  float* f = reinterpret_cast<float*>(frame + item[i]);
-  for (int c = 0; c < 3; ++c) atomicAdd(f + c, att[3 * i + c] * fmaxf(pred[3 * i + c], 0.0f) * inv_spp);
+  const int32_t it = item[i];
+  if (i > 0 && item[i - 1] == it) return;
+  float4 f = frame[it];
+  for (uint32_t j = i; j < n && item[j] == it; ++j) {
+    f.x = __fadd_rn(f.x, __fmul_rn(__fmul_rn(att[3 * j], fmaxf(pred[3 * j], 0.0f)), inv_spp));
+    f.y = __fadd_rn(f.y, __fmul_rn(__fmul_rn(att[3 * j + 1], fmaxf(pred[3 * j + 1], 0.0f)), inv_spp));
+    f.z = __fadd_rn(f.z, __fmul_rn(__fmul_rn(att[3 * j + 2], fmaxf(pred[3 * j + 2], 0.0f)), inv_spp));
+  }
+  frame[it] = f;
 }
 
 // Running mean over frames: accum = frame (first) or lerp(accum, frame, a).
@@ -1891,6 +1876,150 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t* s_tmp 
   return wbase + x - v;
 }
 
+// Order-keeping compaction of per-slot counts (the query flag of each
+// sample, the training rows of each sample): chunks of kScanItems slots, 16
+// per lane of a 256-lane workgroup; k_scan_chunks sums each chunk,
+// k_scan_top turns the sums into chunk offsets, and a compaction kernel
+// scans its chunk again and writes each slot's entries at their offsets.
+constexpr uint32_t kScanLane = 16;
+constexpr uint32_t kScanItems = 256 * kScanLane;
+
+// The 16 counts of this lane in chunk blockIdx.x (0 past n); i0 = the first's
+// slot.  One 16-byte load inside the array (i0 is a multiple of 16).
+__device__ __forceinline__ uint32_t chunk_counts(const uint8_t* __restrict__ cnt, size_t n,
+                                                 uint32_t c[kScanLane], size_t& i0) {
+  i0 = (size_t)blockIdx.x * kScanItems + kScanLane * threadIdx.x;
+  if (i0 + kScanLane <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(cnt + i0);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) c[k] = (w[k >> 2] >> (8 * (k & 3))) & 255u;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) c[k] = i0 + k < n ? cnt[i0 + k] : 0u;
+  }
+  uint32_t t = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) t += c[k];
+  return t;
+}
+
+__global__ __launch_bounds__(256) void k_scan_chunks(const uint8_t* __restrict__ cnt, size_t n,
+                                                     uint32_t* __restrict__ sums) {
+  __shared__ uint32_t s_tmp[4];
+  uint32_t c[kScanLane], total;
+  size_t i0;
+  (void)block_exscan256(chunk_counts(cnt, n, c, i0), s_tmp, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive prefix sums of the nb chunk sums, in place (one workgroup, 16
+// consecutive sums per lane); *total = their sum.
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nb,
+                                                  uint32_t* __restrict__ total) {
+  __shared__ uint32_t s_tmp[4];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += kScanItems) {
+    const uint32_t i = b0 + kScanLane * threadIdx.x;
+    uint32_t v[kScanLane], t = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      v[k] = i + k < nb ? sums[i + k] : 0u;
+      t += v[k];
+    }
+    uint32_t bt;
+    uint32_t e = carry + block_exscan256(t, s_tmp, bt);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (i + k < nb) sums[i + k] = e;
+      e += v[k];
+    }
+    carry += bt;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// The queries of a cached frame in slot order (pixel, then sample): each
+// flagged slot's features (from the hit point, incoming direction, shading
+// normal and material written by the path loop), throughput and pixel item
+// (slot / spp), at its place among the flagged slots.  The chunk's flagged
+// slots are listed in LDS first, so that every lane then works on one.
+__global__ __launch_bounds__(256) void k_compact_queries(
+    const uint8_t* __restrict__ has, const float4* __restrict__ rec, size_t n,
+    const uint32_t* __restrict__ offs, uint32_t spp, F3 bmin, F3 binv, const DevMat* __restrict__ mats,
+    float* __restrict__ feat, float* __restrict__ att, int32_t* __restrict__ item) {
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint16_t s_list[kScanItems];
+  uint32_t c[kScanLane], total;
+  size_t i0;
+  uint32_t lp = block_exscan256(chunk_counts(has, n, c, i0), s_tmp, total);
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (c[k]) s_list[lp++] = (uint16_t)(kScanLane * threadIdx.x + k);
+  __syncthreads();
+  const uint32_t base = offs[blockIdx.x];
+  const size_t chunk0 = (size_t)blockIdx.x * kScanItems;
+  for (uint32_t j = threadIdx.x; j < total; j += 256) {
+    const size_t slot = chunk0 + s_list[j];
+    const uint32_t pos = base + j;
+    const float4 P = rec[4 * slot], D = rec[4 * slot + 1], N = rec[4 * slot + 2], t = rec[4 * slot + 3];
+    const DevMat m = mats[__float_as_uint(P.w)];
+    cache_features(bmin, binv, mk(P.x, P.y, P.z), mk(D.x, D.y, D.z), mk(N.x, N.y, N.z), m,
+                   feat + 12 * (size_t)pos);
+    att[3 * (size_t)pos] = t.x;
+    att[3 * (size_t)pos + 1] = t.y;
+    att[3 * (size_t)pos + 2] = t.z;
+    item[pos] = (int32_t)(slot / spp);
+  }
+}
+
+// The training rows in slot order (pixel, sample, vertex): the rows of
+// sample p (cnt[p] of them, at row slots p * slots + v) follow those of the
+// samples before it; rows at or past cap are dropped.  The chunk's samples
+// with rows are listed in LDS with their first row (<< 7 | count), so that
+// every lane then copies the rows of one.
+__global__ __launch_bounds__(256) void k_compact_rows(const uint8_t* __restrict__ cnt,
+                                                      const float* __restrict__ row, size_t n,
+                                                      uint32_t slots, const uint32_t* __restrict__ offs,
+                                                      uint32_t cap, float* __restrict__ feat,
+                                                      float* __restrict__ tgt) {
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint16_t s_list[kScanItems];
+  __shared__ uint32_t s_row[kScanItems];
+  uint32_t c[kScanLane], rows_total, listed;
+  size_t i0;
+  uint32_t r0 = block_exscan256(chunk_counts(cnt, n, c, i0), s_tmp, rows_total);
+  uint32_t nz = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) nz += c[k] != 0u;
+  uint32_t lp = block_exscan256(nz, s_tmp, listed);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (c[k]) {
+      s_list[lp] = (uint16_t)(kScanLane * threadIdx.x + k);
+      s_row[lp] = (r0 << 7) | c[k];
+      ++lp;
+    }
+    r0 += c[k];
+  }
+  __syncthreads();
+  const uint32_t base = offs[blockIdx.x];
+  const size_t chunk0 = (size_t)blockIdx.x * kScanItems;
+  for (uint32_t j = threadIdx.x; j < listed; j += 256) {
+    const size_t p = chunk0 + s_list[j];
+    const uint32_t nr = s_row[j] & 127u;
+    uint32_t pos = base + (s_row[j] >> 7);
+    for (uint32_t v = 0; v < nr && pos < cap; ++v, ++pos) {
+      const float* r = row + 15 * (p * slots + v);
+      float* f = feat + 12 * (size_t)pos;
+      for (int q = 0; q < 12; ++q) f[q] = r[q];
+      tgt[3 * (size_t)pos] = r[12];
+      tgt[3 * (size_t)pos + 1] = r[13];
+      tgt[3 * (size_t)pos + 2] = r[14];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBuildBlock) void k_sort_hist(const uint32_t* __restrict__ keys,
                                                            int32_t n, int shift, uint32_t* hist,
                                                            int32_t n_tiles) {
@@ -2345,6 +2474,12 @@ struct lisa_ctx {
   DevBuf<float> c_qfeat, c_qatt, c_qout, c_trec, c_tfeat, c_ttgt;
   DevBuf<int32_t> c_qitem, c_tpix;
   DevBuf<uint32_t> c_qcount;
+  // query and training-row slots before compaction (one per sample), the
+  // flags / row counts of those slots and the compaction's chunk offsets
+  DevBuf<float4> c_qrec;
+  DevBuf<float> c_trow;
+  DevBuf<uint8_t> c_qhas, c_tn;
+  DevBuf<uint32_t> c_qscan, c_tscan;
   DevBuf<float> c_ecdf;     // area prefix sums of the emitting triangles (LISA_CACHE_EMISSION)
   float emit_area = 0.0f;
   // the plain tracer judging cached frames (judge_pixels > 0): its pixels,
@@ -2787,33 +2922,55 @@ PathArgs cache_args(const lisa_ctx* ctx, const lisa_render_params* p) {
   return a;
 }
 
+// Order-keeping compaction of n per-slot counts on stream st: chunk offsets
+// into `offs`, the total into *d_total (device).
+lisa_status scan_counts(lisa_ctx* ctx, const uint8_t* cnt, size_t n, DevBuf<uint32_t>& offs,
+                        uint32_t* d_total, hipStream_t st) {
+  const size_t nb = (n + kScanItems - 1) / kScanItems;
+  HIP_TRY(ctx, offs.ensure(nb));
+  if (nb > 0) k_scan_chunks<<<(unsigned)nb, 256, 0, st>>>(cnt, n, offs.p);
+  k_scan_top<<<1, 256, 0, st>>>(offs.p, (uint32_t)nb, d_total);
+  HIP_TRY(ctx, hipGetLastError());
+  return LISA_OK;
+}
+
 // Training paths (path loop kMode 2) on n listed device pixels with a's spp
 // and bounces: rows (features, suffix targets) of every surface hit, or of the
-// hits of radiance ray number `depth` > 0 only, into ctx->c_tfeat / c_ttgt, at
-// most `cap`; *rows = rows produced (may exceed cap).
+// hits of radiance ray number `depth` > 0 only, into ctx->c_tfeat / c_ttgt in
+// slot order (pixel, sample, vertex), at most `cap`; *rows = rows produced
+// (may exceed cap).
 lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, size_t n, size_t cap,
                              uint32_t depth, uint32_t* rows, hipStream_t st) {
   // vertex slots per path: one hit per sample at a fixed depth, else up to one per bounce
   const size_t slots = depth > 0 ? 1 : a.bounces;
+  const size_t paths = n * a.spp;
   HIP_TRY(ctx, ctx->c_trec.ensure(n * slots * 18));
+  HIP_TRY(ctx, ctx->c_trow.ensure(paths * slots * 15));
+  HIP_TRY(ctx, ctx->c_tn.ensure(paths));
   HIP_TRY(ctx, ctx->c_tfeat.ensure(cap * 12));
   HIP_TRY(ctx, ctx->c_ttgt.ensure(cap * 3));
   HIP_TRY(ctx, ctx->c_tacc.ensure(n));
   HIP_TRY(ctx, ctx->c_qcount.ensure(64));
-  HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p + 1, 0, 4, st));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->c_tn.p, 0, paths, st));
   a.pixels = d_pix;
   a.n_items = (int32_t)n;
   a.accum = ctx->c_tacc.p;
   a.accum_prev = nullptr;
   a.t_rec = ctx->c_trec.p;
   a.t_slots = (uint32_t)slots;
-  a.t_feat = ctx->c_tfeat.p;
-  a.t_tgt = ctx->c_ttgt.p;
-  a.t_count = ctx->c_qcount.p + 1;
-  a.t_cap = (uint32_t)cap;
+  a.t_row = ctx->c_trow.p;
+  a.t_n = ctx->c_tn.p;
   a.t_depth = depth;
   k_path_mode<2><<<grid_for((int64_t)n, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
   HIP_TRY(ctx, hipGetLastError());
+  lisa_status rc = scan_counts(ctx, ctx->c_tn.p, paths, ctx->c_tscan, ctx->c_qcount.p + 1, st);
+  if (rc != LISA_OK) return rc;
+  if (paths > 0) {
+    k_compact_rows<<<(unsigned)((paths + kScanItems - 1) / kScanItems), 256, 0, st>>>(
+        ctx->c_tn.p, ctx->c_trow.p, paths, (uint32_t)slots, ctx->c_tscan.p, (uint32_t)cap, ctx->c_tfeat.p,
+        ctx->c_ttgt.p);
+    HIP_TRY(ctx, hipGetLastError());
+  }
   HIP_TRY(ctx, hipMemcpyAsync(rows, ctx->c_qcount.p + 1, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   return LISA_OK;
@@ -2821,19 +2978,22 @@ lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, si
 
 // Prefix paths (path loop kMode 1, or 3 with a.emit_expect): the full image in tiles (d_pix null) or
 // n listed device pixels; radiance gathered before the cache into `frame`
-// (per pixel mean), queries into ctx->c_q* (capacity n * spp).  The launch
-// returns without waiting; trace_queries_finish waits for it, reads the
-// query count (*nq) and turns the query records into features.
+// (per pixel mean), a query slot per sample.  The launch returns without
+// waiting; trace_queries_finish compacts the queries into ctx->c_q* in slot
+// order (pixel, then sample) with their features, and waits for the count
+// (*nq).
 lisa_status trace_queries_launch(lisa_ctx* ctx, PathArgs& a, const int32_t* d_pix, size_t n, uint32_t prefix,
                                  float4* frame, bool persistent, bool full = false) {
   hipStream_t st = ctx->stream;
-  const size_t qcap = full ? 1 : n * a.spp;
-  HIP_TRY(ctx, ctx->c_qfeat.ensure(qcap * 12));
-  HIP_TRY(ctx, ctx->c_qatt.ensure(qcap * 3));
-  HIP_TRY(ctx, ctx->c_qout.ensure(qcap * 3));
-  HIP_TRY(ctx, ctx->c_qitem.ensure(qcap));
+  const size_t slots = full ? 0 : n * a.spp;
+  HIP_TRY(ctx, ctx->c_qrec.ensure(slots * 4));
+  HIP_TRY(ctx, ctx->c_qhas.ensure(slots));
+  HIP_TRY(ctx, ctx->c_qfeat.ensure(slots * 12));
+  HIP_TRY(ctx, ctx->c_qatt.ensure(slots * 3));
+  HIP_TRY(ctx, ctx->c_qout.ensure(slots * 3));
+  HIP_TRY(ctx, ctx->c_qitem.ensure(slots));
   HIP_TRY(ctx, ctx->c_qcount.ensure(64));
-  HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p, 0, 4, st));
+  if (slots > 0) HIP_TRY(ctx, hipMemsetAsync(ctx->c_qhas.p, 0, slots, st));
   a.pixels = d_pix;
   a.n_items = (int32_t)n;
   a.accum = frame;
@@ -2842,11 +3002,8 @@ lisa_status trace_queries_launch(lisa_ctx* ctx, PathArgs& a, const int32_t* d_pi
   // query (radiance ray numbers start at 1)
   if (!full) a.bounces = prefix + 1;
   a.cache_at = full ? 0u : prefix + 1;
-  a.q_feat = ctx->c_qfeat.p;
-  a.q_att = ctx->c_qatt.p;
-  a.q_item = ctx->c_qitem.p;
-  a.q_count = ctx->c_qcount.p;
-  a.q_cap = (uint32_t)qcap;
+  a.q_rec = ctx->c_qrec.p;
+  a.q_has = ctx->c_qhas.p;
   if (persistent) {
     lisa_status rc = a.emit_expect ? launch_persistent<3>(ctx, a, 2) : launch_persistent<1>(ctx, a, 2);
     if (rc != LISA_OK) return rc;
@@ -2869,14 +3026,19 @@ lisa_status trace_queries_launch(lisa_ctx* ctx, PathArgs& a, const int32_t* d_pi
 
 lisa_status trace_queries_finish(lisa_ctx* ctx, const PathArgs& a, uint32_t* nq) {
   hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, hipMemcpyAsync(nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  const uint32_t nf = std::min<uint32_t>(*nq, a.q_cap);
-  if (nf > 0) {
-    k_cache_features<<<(nf + 255) / 256, 256, 0, st>>>(ctx->c_qfeat.p, nf, a.bmin, a.binv, ctx->d_mats);
+  const size_t slots = a.cache_at == 0 ? 0 : (size_t)a.n_items * a.spp;
+  if (slots > 0) {
+    lisa_status rc = scan_counts(ctx, ctx->c_qhas.p, slots, ctx->c_qscan, ctx->c_qcount.p, st);
+    if (rc != LISA_OK) return rc;
+    k_compact_queries<<<(unsigned)((slots + kScanItems - 1) / kScanItems), 256, 0, st>>>(
+        ctx->c_qhas.p, ctx->c_qrec.p, slots, ctx->c_qscan.p, a.spp, a.bmin, a.binv, ctx->d_mats,
+        ctx->c_qfeat.p, ctx->c_qatt.p, ctx->c_qitem.p);
     HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipStreamSynchronize(st));  // the network reads them on the cache's stream
+  } else {
+    HIP_TRY(ctx, hipMemsetAsync(ctx->c_qcount.p, 0, 4, st));
   }
+  HIP_TRY(ctx, hipMemcpyAsync(nq, ctx->c_qcount.p, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));  // the network reads them on the cache's stream
   return LISA_OK;
 }
 
@@ -3011,6 +3173,12 @@ void lisa_destroy(lisa_ctx* c) {
   c->c_qitem.release();
   c->c_tpix.release();
   c->c_qcount.release();
+  c->c_qrec.release();
+  c->c_trow.release();
+  c->c_qhas.release();
+  c->c_tn.release();
+  c->c_qscan.release();
+  c->c_tscan.release();
   c->c_ecdf.release();
   c->c_jpix.release();
   c->c_jout.release();
@@ -4011,6 +4179,8 @@ lisa_status lisa_cache_trace_rows(lisa_ctx* ctx, const lisa_render_params* p, co
   if (rc != LISA_OK) return rc;
   if (p->bounces == 0 || p->bounces > 64) return fail(ctx, LISA_ERR_INVALID_ARG, "bounces must be in 1..64");
   if (!n_rows || cap <= 0 || !feat || !tgt) return fail(ctx, LISA_ERR_INVALID_ARG, "output arrays");
+  if ((size_t)n * p->spp * (depth > 0 ? 1 : p->bounces) >= (1ull << 31))
+    return fail(ctx, LISA_ERR_INVALID_ARG, "n * spp * row slots per sample >= 2^31");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   rc = upload_pixels(ctx, p, pixels, n);
   if (rc != LISA_OK) return rc;

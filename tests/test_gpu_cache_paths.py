@@ -35,11 +35,6 @@ def _scene(capi, oracle_mod, ctx, W=64, H=64):
     return sc, o, camera(sc, capi)
 
 
-def _order(item, feat):
-    keys = [feat[:, k] for k in (2, 1, 0)] + [item]
-    return np.lexsort(keys)
-
-
 def _close(a, b, rtol=1e-5):
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     return np.all(np.abs(a - b) <= rtol * np.abs(b) + 1e-30)
@@ -65,11 +60,11 @@ def test_queries_match_oracle(capi, oracle_mod, ctx, spp, prefix, emit, loops, d
                         nee_loops=loops, nee_loops_deep=deep)
     print("queries", g["n"], r["n"], flush=True)
     assert g["n"] == r["n"] > 0
-    gi, ri = _order(g["item"], g["feat"]), _order(r["item"], r["feat"])
-    assert np.array_equal(g["item"][gi], r["item"][ri])
-    assert np.array_equal(g["feat"][gi][:, EXACT], r["feat"][ri][:, EXACT])
-    assert np.abs(g["feat"][gi][:, TRIG] - r["feat"][ri][:, TRIG]).max() <= 2e-6
-    assert np.array_equal(g["att"][gi], r["att"][ri])
+    # the oracle's order: listed pixel, then sample
+    assert np.array_equal(g["item"], r["item"])
+    assert np.array_equal(g["feat"][:, EXACT], r["feat"][:, EXACT])
+    assert np.abs(g["feat"][:, TRIG] - r["feat"][:, TRIG]).max() <= 2e-6
+    assert np.array_equal(g["att"], r["att"])
     assert _close(g["accum"], r["accum"])
 
 
@@ -83,23 +78,22 @@ def test_training_rows_match_oracle(capi, oracle_mod, ctx):
     r = o.cache_train_rows(W, H, 2, 7, eye, U, V, Wv, px, subframe=5, threads=8)
     print("rows", g["n"], r["n"], flush=True)
     assert g["n"] == r["n"] > 1024
-    gi = np.lexsort([g["tgt"][:, 0], g["feat"][:, 2], g["feat"][:, 1], g["feat"][:, 0]])
-    ri = np.lexsort([r["tgt"][:, 0], r["feat"][:, 2], r["feat"][:, 1], r["feat"][:, 0]])
-    assert np.array_equal(g["feat"][gi][:, EXACT], r["feat"][ri][:, EXACT])
-    assert np.abs(g["feat"][gi][:, TRIG] - r["feat"][ri][:, TRIG]).max() <= 2e-6
+    # the oracle's order: listed pixel, then sample, then vertex
+    assert np.array_equal(g["feat"][:, EXACT], r["feat"][:, EXACT])
+    assert np.abs(g["feat"][:, TRIG] - r["feat"][:, TRIG]).max() <= 2e-6
     # targets bit-exact, including the non-finite ones of zero-throughput channels
-    assert np.array_equal(g["tgt"][gi], r["tgt"][ri], equal_nan=True)
+    assert np.array_equal(g["tgt"], r["tgt"], equal_nan=True)
     # a cap keeps the first rows only and reports the full count
     g2 = capi.cache_trace_rows(ctx, p, px, cap=100)
     assert g2["n"] == g["n"] and g2["feat"].shape[0] == 100
+    assert np.array_equal(g2["feat"], g["feat"][:100])
+    assert np.array_equal(g2["tgt"], g["tgt"][:100], equal_nan=True)
     # rows at one depth (the query depth of lisa_render_cached)
     g3 = capi.cache_trace_rows(ctx, p, px, depth=3)
     r3 = o.cache_train_rows(W, H, 2, 7, eye, U, V, Wv, px, subframe=5, depth=3, threads=8)
     assert g3["n"] == r3["n"] > 0
-    gi = np.lexsort([g3["tgt"][:, 0], g3["feat"][:, 2], g3["feat"][:, 1], g3["feat"][:, 0]])
-    ri = np.lexsort([r3["tgt"][:, 0], r3["feat"][:, 2], r3["feat"][:, 1], r3["feat"][:, 0]])
-    assert np.array_equal(g3["feat"][gi][:, EXACT], r3["feat"][ri][:, EXACT])
-    assert np.array_equal(g3["tgt"][gi], r3["tgt"][ri], equal_nan=True)
+    assert np.array_equal(g3["feat"][:, EXACT], r3["feat"][:, EXACT])
+    assert np.array_equal(g3["tgt"], r3["tgt"], equal_nan=True)
 
 
 def test_forced_fallback_is_the_plain_progressive_render(capi, oracle_mod, ctx):
@@ -212,6 +206,38 @@ def test_persistent_cached_frames_same_image(capi, oracle_mod, ctx, emission):
         b.close()
 
 
+def test_trained_cached_frames_are_reproducible(capi, oracle_mod, ctx):
+    """Queries and training rows take fixed slots (pixel, sample) and each
+    pixel's queries are summed in sample order, so a trained cached sequence
+    is the same bits whatever the launch form and stream order: tiled paths
+    with training beside the frame on a second stream, and a persistent
+    launch with training first, give the same images and weights."""
+    W, H = 96, 80
+    sc, o, (U, V, Wv) = _scene(capi, oracle_mod, ctx, W, H)
+    eye = sc["camera"]["eye"]
+    other = capi.Context(0)
+    a, b = capi.Cache(0, seed=1), capi.Cache(0, seed=1)
+    try:
+        other.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        kw = dict(prefix_bounces=2, train_paths=6000, train_bounces=7, train_steps=3, seed=4)
+        overlap = capi.cache_frame_params(**kw)
+        serial = capi.cache_frame_params(flags=capi.CACHE_PERSISTENT | capi.CACHE_SERIAL, **kw)
+        for f in range(4):
+            p = capi.render_params(W, H, 3, 7, eye, U, V, Wv, subframe=f)
+            s1 = capi.render_cached(ctx, a, p, overlap)
+            s2 = capi.render_cached(other, b, p, serial)
+            assert s1["queries"] == s2["queries"] > W * H
+            assert s1["train_samples"] == s2["train_samples"] > 0
+            assert s1["loss"] == s2["loss"]
+            g1, g2 = ctx.copy_accum(W * H), other.copy_accum(W * H)
+            assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32)), f
+        assert np.array_equal(a.weights().view(np.uint32), b.weights().view(np.uint32))
+    finally:
+        other.close()
+        a.close()
+        b.close()
+
+
 def test_variance_reduced_prefix_on_the_gpu(capi, oracle_mod, ctx):
     """The prefix radiance of a cached frame (the image with a cache that
     answers 0) from the GPU's two prefix forms on a small island: the
@@ -267,9 +293,9 @@ def test_emission_prefix_after_the_emitters_are_gone(capi, oracle_mod, ctx):
                                                                  flags=capi.CACHE_EMISSION), px)
     assert a["n"] == b["n"] > 0
     assert np.array_equal(a["accum"].view(np.uint32), b["accum"].view(np.uint32))
-    ia, ib = _order(a["item"], a["feat"]), _order(b["item"], b["feat"])
-    assert np.array_equal(a["feat"][ia], b["feat"][ib])
-    assert np.array_equal(a["att"][ia], b["att"][ib])
+    assert np.array_equal(a["item"], b["item"])
+    assert np.array_equal(a["feat"], b["feat"])
+    assert np.array_equal(a["att"], b["att"])
 
 
 def test_plain_tracer_judges_the_cache(capi, oracle_mod, ctx):
