@@ -108,9 +108,10 @@ def parse():
     return ap.parse_args()
 
 
-def profile_record(workload, spp):
+def profile_record(workload, spp, bvh):
     """The plain megakernel's rocprofv3 figures recorded in
-    profiles/k_path_<workload>_pmc.json for the same spp, or {}:
+    profiles/k_path_<workload>_pmc.json for the same spp and tree walk
+    ("binary" or "4-wide"; a record without the key is of the binary walk), or {}:
     hbm_bytes_per_launch (two --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled
     by the copy-kernel calibration) and valu_lane_utilization
     (SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU))."""
@@ -118,7 +119,7 @@ def profile_record(workload, spp):
     try:
         with open(path) as fh:
             rec = json.load(fh)
-        return rec if int(rec["spp"]) == spp else {}
+        return rec if int(rec["spp"]) == spp and rec.get("bvh", "binary") == bvh else {}
     except (OSError, KeyError, ValueError):
         return {}
 
@@ -141,8 +142,9 @@ def bench_plain(a, workload, steps, warmup, cpu=True):
     ctx = capi.Context(0)
     ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
     bvh_ms = ctx.stats()["bvh_build_ms"]
-    print(f"[bench] scene {sc['name']}: {ctx.T} triangles, LBVH build {bvh_ms:.2f} ms",
-          file=sys.stderr, flush=True)
+    wide = ctx.stats()["wide_tree"]  # scenes of >= 2^20 triangles: the 4-wide walk
+    print(f"[bench] scene {sc['name']}: {ctx.T} triangles, LBVH build {bvh_ms:.2f} ms"
+          + (" (with its 4-wide form)" if wide else ""), file=sys.stderr, flush=True)
 
     # exact work per step (counting instantiation of the same kernel)
     p = capi.render_params(W, H, spp, B, cam["eye"], U, V, Wv, flags=capi.FLAG_COUNT)
@@ -169,7 +171,8 @@ def bench_plain(a, workload, steps, warmup, cpu=True):
     alg_bytes = (S_NODE * counts["node_visits"] + S_TRI * counts["tri_tests"]
                  + S_SHADE * counts["radiance_hits"] + S_PIXEL * W * H)
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
-    rec = {} if (a.persistent or a.wavefront) else profile_record(workload, spp)
+    rec = {} if (a.persistent or a.wavefront) else profile_record(workload, spp,
+                                                                  "4-wide" if wide else "binary")
     traffic = rec.get("hbm_bytes_per_launch")
     resident = workload in ("cornell", "cornell65k")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -185,7 +188,9 @@ def bench_plain(a, workload, steps, warmup, cpu=True):
                             "instruction issue under divergence, with the scene's nodes and "
                             "triangles streamed from HBM"),
                 "kernel": ("k_path_pq<0>" if a.persistent else
-                           "k_wf_trace + k_wf_shade rounds" if a.wavefront else "k_path<false,false,false>"),
+                           "k_wf_trace + k_wf_shade rounds" if a.wavefront else
+                           "k_path<false,false,false,true> (4-wide walk)" if wide else
+                           "k_path<false,false,false>"),
                 "alg_bytes_per_launch": alg_bytes,
                 "avg_kernel_ms": round(avg_kernel_ms, 3)}
 
@@ -226,7 +231,9 @@ def bench_plain(a, workload, steps, warmup, cpu=True):
                                   + ("wavefront form" if a.wavefront else "megakernel")
                                   + (" (persistent launch)" if a.persistent else ""),
                       "width": W, "height": H,
-                      "spp": spp, "bounces": B, "triangles": ctx.T, "parallelism": "single GPU"},
+                      "spp": spp, "bounces": B, "triangles": ctx.T,
+                      "bvh": "binary + 4-wide form" if wide else "binary",
+                      "parallelism": "single GPU"},
            "samples_per_s": round(W * H * spp * steps / wall, 1),
            "rays_per_step": rays, "bvh_build_ms": round(bvh_ms, 3),
            "roofline": roofline, "cpu_baseline": cpu_res}

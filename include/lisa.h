@@ -86,6 +86,9 @@ typedef struct lisa_stats {
   int32_t num_nodes;  /* internal BVH nodes (64 B each) */
   int32_t num_emitters; /* triangles whose material emits (the NEE light BVH) */
   uint64_t device_bytes; /* device memory held by the context */
+  int32_t wide_tree;  /* 1: the scene's tree also has its 4-wide form, which the uncounted path
+                         kernels walk (lisa_set_wide_threshold) */
+  int32_t reserved;
 } lisa_stats;
 
 typedef struct lisa_ctx lisa_ctx;
@@ -168,6 +171,15 @@ lisa_status lisa_synchronize(lisa_ctx* ctx);
  * No counterpart in the reference, which prints the render's wall time only
  * (render.cc:146-147). */
 lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out);
+/* Scenes set from now on with at least min_triangles triangles (default
+ * 2^20) also get the 4-wide form of their tree (128 bytes per inner node: each
+ * node's children, inner children replaced by their own two), and the path
+ * kernels that count nothing walk it: fewer, wider visits, which pay where
+ * the tree streams from HBM.  The hits are the same (closest hit, equal t to
+ * the lower triangle id); node-visit counters (LISA_FLAG_COUNT) stay those of
+ * the binary tree.  0 gives every scene the wide form.  No counterpart in
+ * the reference (OptiX chooses its acceleration structure). */
+lisa_status lisa_set_wide_threshold(lisa_ctx* ctx, int64_t min_triangles);
 /* Counters of the last call made with LISA_FLAG_COUNT (or lisa_trace_rays). */
 lisa_status lisa_get_counters(const lisa_ctx* ctx, uint64_t* counters /* LISA_NCOUNTERS */);
 /* The hipStream_t the context launches on (for external event timing). */

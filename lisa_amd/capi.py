@@ -39,7 +39,8 @@ EXPORTS = (
     "lisa_create", "lisa_destroy", "lisa_last_error", "lisa_set_scene", "lisa_update_vertices",
     "lisa_update_vertices_noalloc",
     "lisa_render", "lisa_render_progressive",
-    "lisa_copy_accum", "lisa_accum64_save", "lisa_accum64_load", "lisa_synchronize", "lisa_get_stats", "lisa_get_counters",
+    "lisa_copy_accum", "lisa_accum64_save", "lisa_accum64_load", "lisa_synchronize", "lisa_get_stats",
+    "lisa_set_wide_threshold", "lisa_get_counters",
     "lisa_get_stream", "lisa_get_bvh", "lisa_trace_rays", "lisa_trace_subset",
     "lisa_batch_create", "lisa_batch_destroy", "lisa_batch_set_config", "lisa_batch_get_config",
     "lisa_batch_render", "lisa_batch_autotune",
@@ -74,7 +75,7 @@ class RenderParams(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("render_ms", C.c_float), ("bvh_build_ms", C.c_float), ("num_tris", C.c_int32),
                 ("num_nodes", C.c_int32), ("num_emitters", C.c_int32),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("wide_tree", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BatchConfig(C.Structure):
@@ -117,6 +118,7 @@ def lib():
         "lisa_accum64_load": (st, [vp, C.POINTER(C.c_double), C.c_int64]),
         "lisa_synchronize": (st, [vp]),
         "lisa_get_stats": (st, [vp, C.POINTER(Stats)]),
+        "lisa_set_wide_threshold": (st, [vp, C.c_int64]),
         "lisa_get_counters": (st, [vp, u64p]),
         "lisa_get_stream": (vp, [vp]),
         "lisa_get_bvh": (st, [vp, C.c_int32, u32p, u32p, u32p, f32p]),
@@ -337,12 +339,18 @@ class Context:
     def synchronize(self):
         self._ck(lib().lisa_synchronize(self.h))
 
+    def set_wide_threshold(self, min_triangles):
+        """Scenes set from now on with at least min_triangles triangles get the 4-wide
+        tree that the uncounted path kernels walk (lisa_set_wide_threshold)."""
+        self._ck(lib().lisa_set_wide_threshold(self.h, int(min_triangles)))
+
     def stats(self):
         s = Stats()
         self._ck(lib().lisa_get_stats(self.h, C.byref(s)))
         return {"render_ms": s.render_ms, "bvh_build_ms": s.bvh_build_ms,
                 "num_tris": s.num_tris, "num_nodes": s.num_nodes,
-                "num_emitters": s.num_emitters, "device_bytes": s.device_bytes}
+                "num_emitters": s.num_emitters, "device_bytes": s.device_bytes,
+                "wide_tree": bool(s.wide_tree)}
 
     def counters(self):
         c = np.zeros(NCOUNTERS, np.uint64)

@@ -7,6 +7,8 @@
 //                           c0.lo.xyz c0.hi.xyz c1.lo.xyz c1.hi.xyz | ref0 ref1 - -
 //                           (child boxes live in the parent; ref bit 31 = leaf,
 //                           low bits = sorted triangle / node index)
+//   wide   float4[8 * nI]   scenes of >= 2^20 triangles: the 4-wide form (k_wide),
+//                           one 128-byte line per internal node
 //   tris   float4[3 * T]    48 bytes per triangle in Morton order:
 //                           v0.xyz|prim id, v1.xyz|material id, v2.xyz|0
 //   nrm    float4[3 * T]    vertex normals in the same order (shading gather)
@@ -61,6 +63,9 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #ifndef LISA_PATH_MIN_BLOCKS
 #define LISA_PATH_MIN_BLOCKS 6
 #endif
+#ifndef LISA_WIDE_WAVES
+#define LISA_WIDE_WAVES 6  // k_path over the 4-wide tree
+#endif
 #ifndef LISA_MODE_WAVES
 #define LISA_MODE_WAVES 5  // the cache-mode path kernels
 #endif
@@ -108,6 +113,7 @@ struct BvhView {
   const float4* __restrict__ nodes;
   const float4* __restrict__ tris;
   int32_t T;
+  const float4* __restrict__ wide;  // the 4-wide form of `nodes` (null: none)
 };
 
 struct SceneView {
@@ -349,6 +355,108 @@ __device__ __forceinline__ int32_t closest(const BvhView& bv, const RayT& r, flo
   return best_k;
 }
 
+// The closest / any-hit query of closest() over the 4-wide form of the tree
+// (k_wide): up to four slab tests per visit, leaves tested at once, the
+// nearest inner child visited next and the others pushed far to near.  The
+// accepted triangle is closest()'s (the same acceptance rule, which does not
+// depend on the order of the tests); the visits differ, so the counting
+// builds keep closest().  The stack holds up to 3 entries per level of a
+// tree of at most 32 levels.  Fewer, wider visits pay where the tree streams
+// from HBM (large scenes); on a cache-resident tree the binary walk issues
+// fewer instructions.
+constexpr int kWideStackMax = 96;
+template <int kStride>
+__device__ __forceinline__ int32_t closest_wide(const BvhView& bv, const RayT& r, float tmin,
+                                                float tlim, uint32_t lim_prim, bool any, float& t_out,
+                                                uint32_t* stk) {
+  volatile uint32_t spill[kWideStackMax - kStackLds];
+  float best_t = tlim;
+  float cull_t = tlim * kWiden;
+  uint32_t best_prim = lim_prim;
+  int32_t best_k = -1;
+  int sp = 0;
+  uint32_t node = 0;
+  const uint32_t max_visits = (uint32_t)bv.T + 1u;
+  for (uint32_t it = 0; it < max_visits; ++it) {
+    const float4* np = bv.wide + 8 * (size_t)node;
+    const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5], rf = np[6];
+    uint32_t ref[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z),
+                       __float_as_uint(rf.w)};
+    float tn[4];
+    bool h[4];
+    h[0] = (ref[0] != kEmpty) & slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, tmin, cull_t, tn[0]);
+    h[1] = (ref[1] != kEmpty) & slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, tmin, cull_t, tn[1]);
+    h[2] = (ref[2] != kEmpty) & slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, tmin, cull_t, tn[2]);
+    h[3] = (ref[3] != kEmpty) & slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, tmin, cull_t, tn[3]);
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+      if (h[ch] && (ref[ch] & kLeaf)) {
+        const uint32_t k = ref[ch] & ~kLeaf;
+        const float4* tp = bv.tris + 3 * (size_t)k;
+        const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+        const uint32_t prim = __float_as_uint(t0.w);
+        float t;
+        if (watertight(r, xyz(t0), xyz(t1), xyz(t2), t) && t > tmin &&
+            (t < best_t || (t == best_t && prim < best_prim))) {
+          best_t = t;
+          cull_t = t * kWiden;
+          best_prim = prim;
+          best_k = (int32_t)k;
+        }
+        h[ch] = false;
+      }
+    }
+    if (any && best_k >= 0) break;
+    // inner children still before the (possibly nearer) best hit, nearest first
+    float key[4];
+    int m = 0;
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+      h[ch] = h[ch] && tn[ch] <= cull_t;
+      key[ch] = h[ch] ? tn[ch] : INFINITY;
+      m += h[ch] ? 1 : 0;
+    }
+    if (m == 0) {
+      if (sp == 0) break;
+      --sp;
+      if (sp < kStackLds)
+        node = stk[sp * kStride];
+      else
+        node = spill[sp - kStackLds];
+      continue;
+    }
+    // sorting network on (key, ref): (0,1) (2,3) (0,2) (1,3) (1,2)
+    auto cswap = [&](int i, int j) {
+      const bool sw = key[j] < key[i];
+      const float ki = key[i], kj = key[j];
+      const uint32_t ri = ref[i], rj = ref[j];
+      key[i] = sw ? kj : ki;
+      key[j] = sw ? ki : kj;
+      ref[i] = sw ? rj : ri;
+      ref[j] = sw ? ri : rj;
+    };
+    cswap(0, 1);
+    cswap(2, 3);
+    cswap(0, 2);
+    cswap(1, 3);
+    cswap(1, 2);
+    if (sp + 3 > kWideStackMax) break;  // unreachable for a valid tree (depth < 64)
+#pragma unroll
+    for (int q = 3; q >= 1; --q) {
+      if (q < m) {
+        if (sp < kStackLds)
+          stk[sp * kStride] = ref[q];
+        else
+          spill[sp - kStackLds] = ref[q];
+        ++sp;
+      }
+    }
+    node = ref[0];
+  }
+  t_out = best_t;
+  return best_k;
+}
+
 // ---------------------------------------------------------------------------
 // Ray batch kernel (hit-id parity, lisa_trace_rays).
 // ---------------------------------------------------------------------------
@@ -369,8 +477,11 @@ __global__ __launch_bounds__(kRayBlock) void k_trace_rays(SceneView sc, const fl
   float t = 0.0f;
   if (sc.all.T > 0 && ray_valid(d)) {
     const RayT r = make_ray(o, d);
-    const int32_t k = closest<kCount, kRayBlock>(sc.all, r, tmin, tmax, 0xFFFFFFFFu, false, t,
-                                                 stk + threadIdx.x, cnt);
+    int32_t k = closest<kCount, kRayBlock>(sc.all, r, tmin, tmax, 0xFFFFFFFFu, false, t,
+                                           stk + threadIdx.x, cnt);
+    // with a 4-wide tree, the hit of its walk (which the path kernels use;
+    // the counts stay the binary walk's)
+    if (sc.all.wide) k = closest_wide<kRayBlock>(sc.all, r, tmin, tmax, 0xFFFFFFFFu, false, t, stk + threadIdx.x);
     if (k >= 0) prim = (int32_t)__float_as_uint(sc.all.tris[3 * (size_t)k].w);
   }
   prim_out[i] = prim;
@@ -489,7 +600,7 @@ struct Job {
 // suffix radiance (final - so far) / throughput of each vertex becomes a
 // training row at the sample's row slots.  Without accum_prev the launch writes its own mean (no
 // running mean over subframes; the subframe then only picks the RNG stream).
-template <bool kRecord, bool kCount, class Src, int kMode = 0, bool kF64 = false,
+template <bool kRecord, bool kCount, class Src, int kMode = 0, bool kF64 = false, bool kWide = false,
           class Bsdf = LISA_BSDF>
 __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
   Job job;
@@ -838,8 +949,12 @@ __device__ __forceinline__ void path_lane(Src& src, uint32_t* my_stk) {
     if (valid && scp->all.T > 0) {
       const RayT r = make_ray(O, D);
       const bool any = qstage >= 2;
-      k = closest<kCount, kPathBlock>(scp->all, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
-                                      any, t, my_stk, tc);
+      if constexpr (kWide && !kCount)
+        k = closest_wide<kPathBlock>(scp->all, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
+                                     any, t, my_stk);
+      else
+        k = closest<kCount, kPathBlock>(scp->all, r, kTmin, any ? tE : kTmax, any ? primE : 0xFFFFFFFFu,
+                                        any, t, my_stk, tc);
     }
 
     if (qstage == 0) {
@@ -1006,21 +1121,21 @@ struct OnePixel {
   }
 };
 
-template <bool kRecord, bool kCount, bool kF64 = false>
-__global__ __launch_bounds__(kPathBlock, (kRecord || kCount) ? 4 : LISA_PATH_MIN_BLOCKS) void k_path(SceneView sc, PathArgs p) {
+template <bool kRecord, bool kCount, bool kF64 = false, bool kWide = false>
+__global__ __launch_bounds__(kPathBlock, (kRecord || kCount) ? 4 : (kWide ? LISA_WIDE_WAVES : LISA_PATH_MIN_BLOCKS)) void k_path(SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kLaneLds * kPathBlock];
   OnePixel src{sc, p, false};
-  path_lane<kRecord, kCount, OnePixel, 0, kF64>(src, stk + threadIdx.x);
+  path_lane<kRecord, kCount, OnePixel, 0, kF64, kWide>(src, stk + threadIdx.x);
 }
 
 // Path loop in a cache mode (1: cached render of the full image in tiles,
 // 2: training paths on a pixel list).
-template <int kMode>
+template <int kMode, bool kWide = false>
 __global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : LISA_MODE_WAVES) void k_path_mode(
     SceneView sc, PathArgs p) {
   __shared__ uint32_t stk[kLaneLds * kPathBlock];
   OnePixel src{sc, p, false};
-  path_lane<false, false, OnePixel, kMode>(src, stk + threadIdx.x);
+  path_lane<false, false, OnePixel, kMode, false, kWide>(src, stk + threadIdx.x);
 }
 
 // One scene's pixels from a device counter, for persistent launches: the
@@ -1072,12 +1187,12 @@ struct PixelQueue {
 // Persistent form of the path loop (kMode as in k_path_mode, 0 = plain):
 // `total` work indices (pixels of the list, or 64 per 8x8 tile) handed out
 // by `counter`, which must be zero at launch.
-template <int kMode>
+template <int kMode, bool kWide = false>
 __global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : 1) void k_path_pq(
     SceneView sc, PathArgs p, uint32_t* counter, uint32_t total, uint32_t chunk) {
   __shared__ uint32_t stk[kLaneLds * kPathBlock];
   PixelQueue src{sc, p, counter, total, chunk, 0u, 0u};
-  path_lane<false, false, PixelQueue, kMode>(src, stk + threadIdx.x);
+  path_lane<false, false, PixelQueue, kMode, false, kWide>(src, stk + threadIdx.x);
 }
 
 // frame[item] += throughput * max(network output, 0) / spp for every query.
@@ -2292,6 +2407,58 @@ __global__ void k_single_root(const float4* __restrict__ tris, float4* nodes) {
   nodes[3] = make_float4(__uint_as_float(kLeaf), __uint_as_float(kEmpty), 0.0f, 0.0f);
 }
 
+// Child ch (0, 1) of the BVH2 node at np: its box and ref.
+__device__ __forceinline__ uint32_t node_child(const float4* np, int ch, float lo[3], float hi[3]) {
+  const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+  if (ch == 0) {
+    lo[0] = a.x, lo[1] = a.y, lo[2] = a.z, hi[0] = a.w, hi[1] = b.x, hi[2] = b.y;
+    return __float_as_uint(d.x);
+  }
+  lo[0] = b.z, lo[1] = b.w, lo[2] = c.x, hi[0] = c.y, hi[1] = c.z, hi[2] = c.w;
+  return __float_as_uint(d.y);
+}
+
+// The 4-wide form of the tree: for every BVH2 inner node n, 128 bytes at
+// wide[8n..8n+8) holding n's children, each inner child replaced by its own
+// two children, so that a walk from the root visits the inner nodes of even
+// depth only (an inner slot's ref is the grandchild's own index).  Structure
+// of arrays over the four slots: lo.x, lo.y, lo.z, hi.x, hi.y, hi.z (one
+// float4 each), the refs (kEmpty: no child; bit 31: leaf), one float4 unused.
+__global__ __launch_bounds__(kBuildBlock) void k_wide(const float4* __restrict__ nodes, int32_t nI,
+                                                      float4* __restrict__ wide) {
+  const int32_t n = blockIdx.x * kBuildBlock + threadIdx.x;
+  if (n >= nI) return;
+  float lo[3][4] = {}, hi[3][4] = {};
+  uint32_t ref[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+  int j = 0;
+  const float4* np = nodes + 4 * (size_t)n;
+  for (int ch = 0; ch < 2; ++ch) {
+    float l[3], h[3];
+    const uint32_t r = node_child(np, ch, l, h);
+    if (r == kEmpty) continue;
+    if (r & kLeaf) {
+      for (int a = 0; a < 3; ++a) lo[a][j] = l[a], hi[a][j] = h[a];
+      ref[j++] = r;
+      continue;
+    }
+    const float4* gp = nodes + 4 * (size_t)r;
+    for (int gc = 0; gc < 2; ++gc) {
+      const uint32_t g = node_child(gp, gc, l, h);
+      if (g == kEmpty) continue;
+      for (int a = 0; a < 3; ++a) lo[a][j] = l[a], hi[a][j] = h[a];
+      ref[j++] = g;
+    }
+  }
+  float4* w = wide + 8 * (size_t)n;
+  for (int a = 0; a < 3; ++a) {
+    w[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+    w[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+  }
+  w[6] = make_float4(__uint_as_float(ref[0]), __uint_as_float(ref[1]), __uint_as_float(ref[2]),
+                     __uint_as_float(ref[3]));
+  w[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2301,6 +2468,7 @@ __global__ void k_single_root(const float4* __restrict__ tris, float4* nodes) {
 struct DevBvh {
   int32_t T = 0, nI = 0;
   float4* nodes = nullptr;
+  float4* wide = nullptr;  // the 4-wide form (k_wide), 128 bytes per inner node; null: none
   float4* tris = nullptr;
   uint32_t* morton = nullptr;
   uint32_t* prim = nullptr;       // global triangle ids, Morton order
@@ -2433,12 +2601,19 @@ hipError_t wf_pending(hipStream_t st, const WfState& ws, uint32_t* pending) {
 // subframe, chained in one instantiated hipGraph; before each launch the
 // nodes' arguments (subframe index, buffers) are set on the executable graph,
 // so one graph serves every run of n subframes at one image size.
+// Triangle count from which a scene's tree also gets its 4-wide form
+// (closest_wide): 2^20 triangles (64 MB of nodes and 48 MB of triangle
+// records) no longer stay in the caches, and the fewer, wider visits pay;
+// below it the binary walk is the faster one.
+constexpr int64_t kWideMinTris = 1 << 20;
+
 struct ProgGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<hipGraphNode_t> nodes;
   uint32_t grid = 0;
   bool f64 = false;
+  const void* fn = nullptr;  // the kernel of the nodes
   void release() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -2446,6 +2621,7 @@ struct ProgGraph {
     graph = nullptr;
     nodes.clear();
     grid = 0;
+    fn = nullptr;
   }
 };
 
@@ -2503,6 +2679,9 @@ struct lisa_ctx {
   DevBuf<float> c_jrgb;
   WfBuffers wf;  // the wavefront form of lisa_render (LISA_FLAG_WAVEFRONT)
   ProgGraph pg;  // lisa_render_progressive's chain of subframe launches
+  // scenes of at least this many triangles get the 4-wide form of their tree
+  // and the path kernels walk it (lisa_set_wide_threshold)
+  int64_t wide_min_tris = kWideMinTris;
   // FP64 progressive sums (LISA_FLAG_FP64_ACCUM full renders): r, g, b sums
   // and the sample count per pixel; valid while they belong to the accum
   // buffer's current sequence (an FP64 subframe 0, or lisa_accum64_load)
@@ -2551,6 +2730,7 @@ lisa_status fail(lisa_ctx* ctx, lisa_status code, const std::string& msg) {
 
 void free_bvh(DevBvh& b) {
   (void)hipFree(b.nodes);
+  (void)hipFree(b.wide);
   (void)hipFree(b.tris);
   (void)hipFree(b.morton);
   (void)hipFree(b.prim);
@@ -2575,8 +2755,8 @@ void free_scene(lisa_ctx* c) {
 }
 
 SceneView view(const lisa_ctx* c) {
-  return SceneView{BvhView{c->all.nodes, c->all.tris, c->all.T},
-                   BvhView{c->emit.nodes, c->emit.tris, c->emit.T},
+  return SceneView{BvhView{c->all.nodes, c->all.tris, c->all.T, c->all.wide},
+                   BvhView{c->emit.nodes, c->emit.tris, c->emit.T, nullptr},
                    c->d_nrm,
                    c->d_mats,
                    make_float4(c->ecull_c[0], c->ecull_c[1], c->ecull_c[2], 0.0f),
@@ -2675,24 +2855,37 @@ lisa_status refit_levels(lisa_ctx* ctx, DevBvh& b) {
   return LISA_OK;
 }
 
+// The 4-wide form of b from its nodes (when b has one), on stream st.
+lisa_status wide_nodes(lisa_ctx* ctx, DevBvh& b, hipStream_t st) {
+  if (!b.wide || b.T == 0) return LISA_OK;
+  k_wide<<<grid_for(b.nI, kBuildBlock), kBuildBlock, 0, st>>>(b.nodes, b.nI, b.wide);
+  HIP_TRY(ctx, hipGetLastError());
+  return LISA_OK;
+}
+
 // LBVH over the n triangles gid[0..n) (all T when gid is null) of the soup
 // already on the device: centroid bounds -> 30-bit Morton codes -> 4-pass LDS
 // radix sort -> Morton-ordered triangle records -> Karras hierarchy -> refit.
 // Enqueued on ctx->stream; `nrm_out` (optional) receives the normal records.
 lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d_n,
-                      const int32_t* d_mat, const uint32_t* d_gid, int32_t n, float4* nrm_out) {
+                      const int32_t* d_mat, const uint32_t* d_gid, int32_t n, float4* nrm_out,
+                      bool wide = false) {
   hipStream_t st = ctx->stream;
   b.T = n;
   b.nI = n > 1 ? n - 1 : 1;
   const size_t nodes_b = 64 * (size_t)b.nI, tris_b = 48 * (size_t)std::max(n, 1);
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.nodes), nodes_b));
+  if (wide) {
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.wide), 2 * nodes_b));
+    b.bytes += 2 * nodes_b;
+  }
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.tris), tris_b));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.morton), 4 * (size_t)std::max(n, 1)));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.prim), 4 * (size_t)std::max(n, 1)));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.parent), 4 * (size_t)std::max(n, 1)));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.leaf_parent), 4 * (size_t)std::max(n, 1)));
   HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&b.range), 8 * (size_t)std::max(n, 1)));
-  b.bytes = nodes_b + tris_b + 16 * (size_t)std::max(n, 1);
+  b.bytes += nodes_b + tris_b + 16 * (size_t)std::max(n, 1);
   HIP_TRY(ctx, hipMemsetAsync(b.nodes, 0, nodes_b, st));
   if (n == 0) return LISA_OK;
   Scratch S;
@@ -2732,6 +2925,8 @@ lisa_status build_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
     lisa_status rc = refit_levels(ctx, b);
     if (rc != LISA_OK) return rc;
   }
+  lisa_status rw = wide_nodes(ctx, b, st);
+  if (rw != LISA_OK) return rw;
   HIP_TRY(ctx, hipGetLastError());
   // scratch is freed on return: wait for the kernels that use it
   HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -2749,8 +2944,10 @@ lisa_status refit_bvh(lisa_ctx* ctx, DevBvh& b, const float* d_v, const float* d
   } else {
     lisa_status rc = refit_levels(ctx, b);
     if (rc != LISA_OK) return rc;
-    HIP_TRY(ctx, hipStreamSynchronize(st));
   }
+  lisa_status rw = wide_nodes(ctx, b, st);
+  if (rw != LISA_OK) return rw;
+  HIP_TRY(ctx, hipStreamSynchronize(st));
   HIP_TRY(ctx, hipGetLastError());
   return LISA_OK;
 }
@@ -2774,8 +2971,11 @@ lisa_status launch_persistent(lisa_ctx* ctx, PathArgs a, int slot) {
     a.tiles_x = (a.width + 7) / 8;
     total = a.tiles_x * ((a.height + 7) / 8) * 64u;
   }
-  k_path_pq<kMode><<<(unsigned)(ctx->cus * kPersistentBlocksPerCu), kPathBlock, 0, st>>>(
-      view(ctx), a, counter, total, 1u);
+  const unsigned blocks = (unsigned)(ctx->cus * kPersistentBlocksPerCu);
+  if (ctx->all.wide)
+    k_path_pq<kMode, true><<<blocks, kPathBlock, 0, st>>>(view(ctx), a, counter, total, 1u);
+  else
+    k_path_pq<kMode><<<blocks, kPathBlock, 0, st>>>(view(ctx), a, counter, total, 1u);
   HIP_TRY(ctx, hipGetLastError());
   return LISA_OK;
 }
@@ -2961,7 +3161,10 @@ lisa_status trace_train_rows(lisa_ctx* ctx, PathArgs a, const int32_t* d_pix, si
   a.t_row = ctx->c_trow.p;
   a.t_n = ctx->c_tn.p;
   a.t_depth = depth;
-  k_path_mode<2><<<grid_for((int64_t)n, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
+  if (ctx->all.wide)
+    k_path_mode<2, true><<<grid_for((int64_t)n, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
+  else
+    k_path_mode<2><<<grid_for((int64_t)n, kPathBlock), kPathBlock, 0, st>>>(view(ctx), a);
   HIP_TRY(ctx, hipGetLastError());
   lisa_status rc = scan_counts(ctx, ctx->c_tn.p, paths, ctx->c_tscan, ctx->c_qcount.p + 1, st);
   if (rc != LISA_OK) return rc;
@@ -3015,8 +3218,13 @@ lisa_status trace_queries_launch(lisa_ctx* ctx, PathArgs& a, const int32_t* d_pi
       a.tiles_x = (a.width + kTileW - 1) / kTileW;
       blocks = a.tiles_x * ((a.height + kTileH - 1) / kTileH);
     }
-    if (a.emit_expect)
+    const bool wide = ctx->all.wide != nullptr;
+    if (a.emit_expect && wide)
+      k_path_mode<3, true><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
+    else if (a.emit_expect)
       k_path_mode<3><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
+    else if (wide)
+      k_path_mode<1, true><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
     else
       k_path_mode<1><<<blocks, kPathBlock, 0, st>>>(view(ctx), a);
     HIP_TRY(ctx, hipGetLastError());
@@ -3108,6 +3316,8 @@ lisa_status lisa_update_vertices_noalloc(lisa_ctx* ctx, const float* d_verts, co
       lisa_status rc = refit_levels_on(ctx, *b, ta, tb);
       if (rc != LISA_OK) return rc;
     }
+    lisa_status rw = wide_nodes(ctx, *b, st);
+    if (rw != LISA_OK) return rw;
   }
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
@@ -3249,7 +3459,8 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
   }
   hipStream_t st = ctx->stream;
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  lisa_status rc = build_bvh(ctx, ctx->all, d_v, d_n, d_mat, nullptr, T, ctx->d_nrm);
+  lisa_status rc = build_bvh(ctx, ctx->all, d_v, d_n, d_mat, nullptr, T, ctx->d_nrm,
+                             (int64_t)T >= ctx->wide_min_tris);
   if (rc == LISA_OK)
     rc = build_bvh(ctx, ctx->emit, d_v, d_n, d_mat, d_gid, (int32_t)emit_ids.size(), nullptr);
   if (rc != LISA_OK) return rc;
@@ -3309,8 +3520,12 @@ lisa_status lisa_render(lisa_ctx* ctx, const lisa_render_params* p, float* accum
     k_path<false, true, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
   else if (count)
     k_path<false, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
+  else if (a.fp64 && ctx->all.wide)
+    k_path<false, false, true, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
   else if (a.fp64)
     k_path<false, false, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
+  else if (ctx->all.wide)
+    k_path<false, false, false, true><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
   else
     k_path<false, false><<<grid, kPathBlock, 0, st>>>(view(ctx), a);
   HIP_TRY(ctx, hipGetLastError());
@@ -3373,8 +3588,11 @@ lisa_status lisa_render_progressive(lisa_ctx* ctx, const lisa_render_params* p, 
   a.counters = ctx->d_counters;
   a.sum64 = f64 ? ctx->c_sum64.p : nullptr;
   SceneView sv = view(ctx);
-  void* const fn = f64 ? reinterpret_cast<void*>(&k_path<false, false, true>)
-                       : reinterpret_cast<void*>(&k_path<false, false, false>);
+  const bool wide = ctx->all.wide != nullptr;
+  void* const fn = f64 ? (wide ? reinterpret_cast<void*>(&k_path<false, false, true, true>)
+                               : reinterpret_cast<void*>(&k_path<false, false, true>))
+                       : (wide ? reinterpret_cast<void*>(&k_path<false, false, false, true>)
+                               : reinterpret_cast<void*>(&k_path<false, false, false>));
   auto node_params = [&](PathArgs& pa, void** args) {
     hipKernelNodeParams kp;
     std::memset(&kp, 0, sizeof(kp));
@@ -3390,7 +3608,7 @@ lisa_status lisa_render_progressive(lisa_ctx* ctx, const lisa_render_params* p, 
   };
   ProgGraph& g = ctx->pg;
   hipError_t e = hipSuccess;
-  if (!g.exec || g.nodes.size() != (size_t)n_subframes || g.grid != grid || g.f64 != f64) {
+  if (!g.exec || g.nodes.size() != (size_t)n_subframes || g.grid != grid || g.f64 != f64 || g.fn != fn) {
     g.release();
     e = hipGraphCreate(&g.graph, 0);
     for (int32_t i = 0; e == hipSuccess && i < n_subframes; ++i) {
@@ -3408,6 +3626,7 @@ lisa_status lisa_render_progressive(lisa_ctx* ctx, const lisa_render_params* p, 
     }
     g.grid = grid;
     g.f64 = f64;
+    g.fn = fn;
   }
   for (int32_t i = 0; e == hipSuccess && i < n_subframes; ++i) {  // subframe p->subframe + i
     PathArgs ai = a;
@@ -4008,7 +4227,10 @@ lisa_status lisa_render_cached(lisa_ctx* ctx, lisa_cache* cache, const lisa_rend
       if (rc != LISA_OK) return rc;
     } else {
       const unsigned tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
-      k_path<false, false><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
+      if (ctx->all.wide)
+        k_path<false, false, false, true><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
+      else
+        k_path<false, false><<<tiles, kPathBlock, 0, st>>>(view(ctx), a);
       HIP_TRY(ctx, hipGetLastError());
     }
     HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -4203,6 +4425,13 @@ lisa_status lisa_synchronize(lisa_ctx* ctx) {
   return LISA_OK;
 }
 
+lisa_status lisa_set_wide_threshold(lisa_ctx* ctx, int64_t min_triangles) {
+  if (!ctx) return fail(nullptr, LISA_ERR_INVALID_ARG, "ctx is NULL");
+  if (min_triangles < 0) return fail(ctx, LISA_ERR_INVALID_ARG, "min_triangles < 0");
+  ctx->wide_min_tris = min_triangles;
+  return LISA_OK;
+}
+
 lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out) {
   if (!ctx || !out) return fail(nullptr, LISA_ERR_INVALID_ARG, "NULL argument");
   out->render_ms = ctx->render_ms;
@@ -4211,6 +4440,8 @@ lisa_status lisa_get_stats(const lisa_ctx* ctx, lisa_stats* out) {
   out->num_nodes = ctx->all.nI;
   out->num_emitters = ctx->emit.T;
   out->device_bytes = ctx->scene_bytes + ctx->accum_bytes;
+  out->wide_tree = ctx->all.wide ? 1 : 0;
+  out->reserved = 0;
   return LISA_OK;
 }
 

@@ -1,6 +1,7 @@
 """Helpers shared by the tests: scenes, cameras, oracle <-> device setup."""
 from __future__ import annotations
 
+import ctypes as C
 import functools
 
 import numpy as np
@@ -28,3 +29,22 @@ def random_rays(n, seed=0, inside=True):
     D = rs.normal(size=(n, 3))
     D /= np.linalg.norm(D, axis=1, keepdims=True)
     return O, D.astype(np.float32)
+
+
+class DeviceArray:
+    """A float32 array copied into device memory (hipMalloc / hipMemcpy)."""
+
+    def __init__(self, a):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self.ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.ptr), max(a.nbytes, 4)) == 0
+        assert self.hip.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()

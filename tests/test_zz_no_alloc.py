@@ -9,33 +9,12 @@ triangles is refused with a status before anything runs.  The vertices are
 handed over in device memory the test allocates itself through the HIP
 runtime the library uses.
 """
-import ctypes as C
-
 import numpy as np
 import pytest
 
-from tests.scene_util import camera, cornell
+from tests.scene_util import DeviceArray, camera, cornell
 
 pytestmark = pytest.mark.gpu
-
-
-class DeviceArray:
-    """A float32 array copied into device memory (hipMalloc / hipMemcpy)."""
-
-    def __init__(self, a):
-        self.hip = C.CDLL("libamdhip64.so.7")
-        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.hip.hipFree.argtypes = [C.c_void_p]
-        a = np.ascontiguousarray(a, dtype=np.float32)
-        self.ptr = C.c_void_p()
-        assert self.hip.hipMalloc(C.byref(self.ptr), max(a.nbytes, 4)) == 0
-        assert self.hip.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host to device
-
-    def free(self):
-        if self.ptr:
-            self.hip.hipFree(self.ptr)
-            self.ptr = C.c_void_p()
 
 
 def _noalloc(capi):
