@@ -114,7 +114,8 @@ lisa_status lisa_set_scene(lisa_ctx* ctx, const float* verts, const float* norma
  * for the T triangles of the current scene: triangle records are rewritten
  * in the build's Morton order and both LBVHs are refit bottom-up, keeping
  * topology and Morton codes (animated / re-posed geometry; cheaper than a
- * rebuild, box quality degrades with large motion).  No counterpart in the
+ * rebuild, box quality degrades with large motion); a 4-wide form of the
+ * tree is rebuilt from the refitted one.  No counterpart in the
  * reference, whose GAS is built once (optix_wrapper.cc:104-106). */
 lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float* normals);
 
@@ -124,8 +125,9 @@ lisa_status lisa_update_vertices(lisa_ctx* ctx, const float* verts, const float*
  * normals already in DEVICE memory (caller-owned), and no hipMalloc / hipFree
  * of its own -- the refit's union tables live in a device array of the library
  * (64 MB per device, held until the process ends).  At most 1048576
- * triangles (LISA_ERR_INVALID_ARG beyond, before anything runs).  Every call
- * on one device shares that array: calls that overlap on one GPU (from
+ * triangles (LISA_ERR_INVALID_ARG beyond, before anything runs).  A 4-wide
+ * form of the tree (lisa_set_wide_threshold) is rewritten in place.  Every
+ * call on one device shares that array: calls that overlap on one GPU (from
  * several threads or contexts) race.  No counterpart in the reference. */
 lisa_status lisa_update_vertices_noalloc(lisa_ctx* ctx, const float* d_verts, const float* d_normals);
 
