@@ -3,7 +3,10 @@ tests/test_gpu_stack_overflow.py with LISA_LIBRARY pointing at the build
 whose LDS stack holds only 2 entries, so nearly every push of a traversal on
 the 65k-triangle Cornell box goes to the overflow array): closest hits of
 65,536 seeded rays and a 512-pixel x 4-spp path subset (per-sample records
-and counters) equal the oracle's.  Prints OK on success."""
+and counters) equal the oracle's; through the 4-wide walk (forced on the
+same scene), whose stack takes up to three entries per visit, the hits equal
+the oracle's and a render equals the binary walk's bit for bit.  Prints OK on
+success."""
 import os
 import sys
 
@@ -41,6 +44,18 @@ def main():
             a, b = g[k].astype(np.float64), r[k].astype(np.float64)
             assert np.all(np.abs(a - b) <= 1e-5 * np.abs(b) + 1e-30), k
         assert np.array_equal(g["counters"], r["counters"])
+        print("binary walk ok", flush=True)
+        p = capi.render_params(W, H, 4, 7, cam["eye"], U, V, Wv)
+        want, _ = ctx.render(p)
+    with capi.Context(0) as wide:
+        wide.set_wide_threshold(0)
+        wide.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        assert wide.stats()["wide_tree"]
+        gp, gt = wide.trace_rays(O, D)
+        assert np.array_equal(gp, rp) and np.array_equal(gt.view(np.uint32), rt.view(np.uint32))
+        got, _ = wide.render(p)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        print("4-wide walk ok", flush=True)
     print("OK", flush=True)
 
 

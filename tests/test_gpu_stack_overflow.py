@@ -1,13 +1,14 @@
 """GPU: the traversal stack's private overflow path against the oracle.
 
-The path kernels keep the first 22 traversal-stack entries of each lane in
+The path kernels keep the first 20 traversal-stack entries of each lane in
 LDS and deeper ones in a per-lane private array.  A Karras tree over the
-benchmark scenes never needs more than 17 entries, so the normal build never
-reaches the overflow; the library built with a 2-entry LDS stack
+benchmark scenes never needs more than 18 entries in the binary walk, so the
+normal build rarely reaches the overflow; the library built with a 2-entry LDS stack
 (lisa_amd/_build/stack2, made by __graft_entry__.build) sends nearly every
 push there.  tests/overflow_check.py runs against that library in a child
 process (one library per process) and checks hits and per-sample path
-records bit for bit against the oracle.
+records bit for bit against the oracle, and the 4-wide walk's hits and a
+render.
 """
 import os
 import subprocess
