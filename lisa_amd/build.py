@@ -37,8 +37,8 @@ def _run(cmd):
 
 def build(verbose_resources=False, defines=(), out_dir=OUT, cli=True):
     """Compile liblisa.so (and the CLI) into out_dir; `defines` are extra
-    -D tuning macros (LISA_STACK_LDS, LISA_PATH_MIN_BLOCKS, LISA_MODE_WAVES, LISA_EMIT_WAVES,
-    LISA_TILE_WAVES; LISA_PATH_BLOCK must stay 64: a workgroup is one wave)."""
+    -D tuning macros (LISA_STACK_LDS, LISA_PATH_MIN_BLOCKS, LISA_WIDE_WAVES, LISA_MODE_WAVES,
+    LISA_EMIT_WAVES, LISA_TILE_WAVES; LISA_PATH_BLOCK must stay 64: a workgroup is one wave)."""
     os.makedirs(out_dir, exist_ok=True)
     dev_o = os.path.join(out_dir, "lisa_device.o")
     cache_o = os.path.join(out_dir, "lisa_cache.o")
