@@ -56,10 +56,12 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #define LISA_PATH_BLOCK 64
 #endif
 // Waves per SIMD the plain path kernel k_path is compiled for: 6 with a
-// 24-entry LDS stack (6 KB per wave) at 80 VGPRs and a few spills in cold
-// code.  The cache-mode paths run at 5 (their extra state spills more at
-// 6), the counting and recording builds at 4, and the batched forms keep
-// the registers they need.
+// 20-entry LDS stack (24 LDS words per lane with the lane state: 6 KB per
+// 64-lane workgroup) at 80 VGPRs and no spills.  The cache-mode paths run at
+// 5, except the tiled prefix-emission mode (6: its spills sit around the
+// walks and the island's frames ran 2 % faster than at 5; its persistent
+// form would spill far more at 6 and stays at 5), the counting and recording
+// builds at 4, and the batched forms keep the registers they need.
 #ifndef LISA_PATH_MIN_BLOCKS
 #define LISA_PATH_MIN_BLOCKS 6
 #endif
@@ -70,7 +72,7 @@ constexpr int kNeeTries = 30;   // shader.cu:199
 #define LISA_MODE_WAVES 5  // the cache-mode path kernels
 #endif
 #ifndef LISA_EMIT_WAVES
-#define LISA_EMIT_WAVES 5  // the query kernel with the prefix-emission light samples
+#define LISA_EMIT_WAVES 6  // the query kernel with the prefix-emission light samples
 #endif
 #ifndef LISA_TILE_WAVES
 #define LISA_TILE_WAVES 6  // the batched wave-tile form
@@ -1188,7 +1190,7 @@ struct PixelQueue {
 // `total` work indices (pixels of the list, or 64 per 8x8 tile) handed out
 // by `counter`, which must be zero at launch.
 template <int kMode, bool kWide = false>
-__global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_EMIT_WAVES : 1) void k_path_pq(
+__global__ __launch_bounds__(kPathBlock, kMode == 3 ? LISA_MODE_WAVES : 1) void k_path_pq(
     SceneView sc, PathArgs p, uint32_t* counter, uint32_t total, uint32_t chunk) {
   __shared__ uint32_t stk[kLaneLds * kPathBlock];
   PixelQueue src{sc, p, counter, total, chunk, 0u, 0u};
