@@ -10,6 +10,9 @@ the scenes the benchmarks time, at their full triangle counts:
 * the 2048^2 fBm island (8,380,418 triangles): the same, plus the cached
   renderer's prefix paths with the benchmark's settings (prefix 2, the
   variance-reduced prefix with 8 NEE runs at the camera ray's hit, 1 later);
+* for both (scenes of more than 2^20 triangles, whose renders walk the 4-wide
+  form of the tree): a full-image render bit-identical to a context that
+  walks the binary tree;
 * four scenes of the 64-scene batch (heightfields, spheres and boxes, 15k to
   1.1M triangles, random Lambertian / dielectric n in [1.3, 1.7] / emissive
   materials, 256^2..768^2, 32 spp, 8 bounces) rendered together in one
@@ -97,6 +100,21 @@ def _path_subset(capi, ctx, o, sc, npx, spp, bounces, seed):
     return U, V, Wv
 
 
+def _wide_equals_binary(capi, ctx, sc, U, V, Wv):
+    """A full-image render through the 4-wide walk equals the binary walk's."""
+    assert ctx.stats()["wide_tree"]
+    W, H = sc["width"], sc["height"]
+    p = capi.render_params(W, H, 2, sc["bounces"], sc["camera"]["eye"], U, V, Wv)
+    with capi.Context(0) as plain:
+        plain.set_wide_threshold(1 << 62)
+        plain.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
+        assert not plain.stats()["wide_tree"]
+        a, _ = ctx.render(p)
+        b, _ = plain.render(p)
+    print("full image: wide walk == binary walk", flush=True)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_icosphere_level9_full_size(capi, oracle_mod):
     sc = scenes.displaced_icosphere(level=9)
     assert sc["verts"].shape[0] > 5_242_880
@@ -105,7 +123,8 @@ def test_icosphere_level9_full_size(capi, oracle_mod):
         ctx.set_scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
         o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
         _bvh_and_rays(ctx, o, sc, seed=9)
-        _path_subset(capi, ctx, o, sc, 256, 4, 12, seed=9)
+        U, V, Wv = _path_subset(capi, ctx, o, sc, 256, 4, 12, seed=9)
+        _wide_equals_binary(capi, ctx, sc, U, V, Wv)
 
 
 def test_island_2048_full_size(capi, oracle_mod):
@@ -117,6 +136,7 @@ def test_island_2048_full_size(capi, oracle_mod):
         o = oracle_mod.Scene(sc["verts"], sc["normals"], sc["mat_idx"], sc["materials"])
         _bvh_and_rays(ctx, o, sc, seed=5)
         U, V, Wv = _path_subset(capi, ctx, o, sc, 256, 4, 12, seed=5)
+        _wide_equals_binary(capi, ctx, sc, U, V, Wv)
         # the cached renderer's prefix paths with the benchmark's settings
         W, H = sc["width"], sc["height"]
         eye = sc["camera"]["eye"]
